@@ -1,0 +1,115 @@
+/*
+ * pgc.h - C-ABI of the MI355X product-graph cycle solvers (libpgc.so).
+ *
+ * Drop-in boundary for the reference's GPU solver seam. Plain pointers and
+ * sizes only; every entry point returns PGC_OK (0) or a negative status and
+ * never exits the process (the reference calls exit(-1) on an unknown solver
+ * name, cost_time_ratio_solver.cpp:896-899). pgc_last_error() describes the
+ * last failure of the calling thread.
+ *
+ * Threading: one context per host thread; a context owns one HIP stream on
+ * one device. Inputs are caller-owned and copied; device memory is owned by
+ * the context/graph objects.
+ */
+#ifndef PGC_H_
+#define PGC_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pgc_context pgc_context;
+typedef struct pgc_graph pgc_graph;
+
+#define PGC_OK 0
+#define PGC_ERR_ARG (-1)
+#define PGC_ERR_HIP (-2)
+#define PGC_ERR_SOLVER (-3)
+#define PGC_ERR_CAPACITY (-4)
+#define PGC_ERR_UNSUPPORTED (-5)
+
+/* Solver rule flags. lawlergpu / dijkstragpu are the defaults (0 for
+ * dijkstragpu); the CPU solvers' rules are selectable for parity studies. */
+#define PGC_LAWLER_PICK_LAST 1u  /* negative-cycle candidate of the reference's GPU path:
+                                    last reporting vertex (ctr_bnb_subroutine.cu:489-500);
+                                    without it: lowest (D, vertex) (cost_time_ratio_solver.cpp:430-437) */
+#define PGC_MINCOST_CPU_RULES 2u /* dijkstracpu: double lower-bound keys, branching seed
+                                    cycle.front()-numNodes (dijkstra_solver.cpp:318-381) */
+
+typedef struct pgc_result {
+  int32_t passes_layers_once; /* the reference's passes_layers_just_once */
+  float objective;            /* ratio in q-units (Lawler) or raw cost (min-cost), as fp32 */
+  int64_t n_edges;            /* edges of the returned cycle */
+  int64_t sweeps;             /* Bellman-Ford sweeps over all layers */
+  int64_t bf_calls;           /* Bellman-Ford calls */
+  int64_t lawler_iters;       /* Lawler iterations (0 for min-cost) */
+  int64_t branches;           /* branch-and-bound branches explored */
+  double relaxations;         /* edge relaxations attempted (E_x per sweep, summed) */
+  double kernel_ms;           /* device time of the solver kernels (HIP events) */
+  double total_ms;            /* wall time of the call */
+} pgc_result;
+
+const char* pgc_last_error(void);
+int pgc_abi_version(void);
+
+/* Context: device ordinal + one stream. */
+int pgc_context_create(int device, pgc_context** out);
+void pgc_context_destroy(pgc_context* ctx);
+
+/* Builds the layered product graph and its per-edge cost/time on the device.
+ * Replaces ProductGraphGenerators::generate() / generateConjugate()
+ * (product_graph_generators.cpp:21-89), i.e. ProductSpace / ConjProductSpace
+ * (product_spaces/product_space.cpp:19-132, conj_product_space.cpp:70-228) and
+ * DeformationEnergy / HigherOrderDeformationEnergy
+ * (energy/deformationEnergy.cpp:11-103, higherOrderDeformationEnergy.cpp:209-339).
+ * VX [nVX x 3], VY [nVY x 3], feat_diff [nVX x nVY] row-major float64; EX
+ * [nEX x 2], EY [nEY x 2] int32. NX/NY [n x 3] (conjugate only, else NULL). */
+int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const int32_t* EX, int64_t nEX,
+                       const double* VY, int64_t nVY, const int32_t* EY, int64_t nEY, const double* feat_diff,
+                       int conjugate, int regularising, int prune, const char* cost_mode, const char* time_mode,
+                       const double* NX, const double* NY, double rl_alpha, double rl_c, double rl_pwr,
+                       pgc_graph** out);
+
+/* Takes the solver inputs of the reference's seam as host arrays: the
+ * arguments of CostTimeRatioSolver / DijkstraSolver
+ * (cost_time_ratio_solver.hpp:107, dijkstra_solver.hpp:83): productspace
+ * [n_edges x ps_cols] row-major, SRC/TRGT [n_edges], cost [n_edges x n_cols]
+ * column-major (Eigen layout), numNodesPerLayer. */
+int pgc_graph_from_arrays(pgc_context* ctx, const int32_t* productspace, int ps_cols, const int32_t* src,
+                          const int32_t* trgt, int64_t n_edges, const double* cost, int n_cols,
+                          int64_t nodes_per_layer, pgc_graph** out);
+
+/* Optional branch graph of the min-cost branching (the reference's
+ * branchGraph, product_space.cpp:122-131, conj_product_space.cpp:177-225):
+ * CSR over layer-0 nodes, values as the reference stores them. */
+int pgc_graph_set_branch_graph(pgc_graph* g, const int64_t* offsets, const int64_t* values);
+
+int pgc_graph_info(const pgc_graph* g, int64_t* n_edges, int32_t* ps_cols, int32_t* n_cost_cols,
+                   int64_t* nodes_per_layer, int64_t* n_layers);
+int pgc_graph_get_product_space(pgc_graph* g, int32_t* out);   /* n_edges x ps_cols row-major */
+int pgc_graph_get_cost(pgc_graph* g, double* out);             /* n_edges x n_cost_cols column-major */
+int pgc_graph_set_cost(pgc_graph* g, const double* cost);      /* same layout; replaces the energy */
+void pgc_graph_destroy(pgc_graph* g);
+
+/* Lawler minimum cost/time-ratio cycle with branch and bound ("lawlergpu").
+ * Replaces CostTimeRatioSolver::run -> solveWithBnbGPULawler
+ * (cost_time_ratio_solver.cpp:868-900, cost_time_ratio_solver_cuda.cpp:175-331).
+ * Quantizes the graph's cost in place to q-units, as the reference does
+ * (cost_time_ratio_solver.cpp:875-882). Writes the cycle's edge ids (walk
+ * order) and, if rows != NULL, their product-space rows. */
+int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int max_depth, uint32_t flags, int32_t* edge_ids,
+                    int32_t* rows, int64_t capacity, pgc_result* res);
+
+/* Minimum-cost once-wrapping cycle with branch and bound ("dijkstragpu").
+ * Replaces DijkstraSolver::run -> solveWithBnBCuda
+ * (dijkstra_solver.cpp:563-579, dijkstra_solver_cuda.cpp:136-322). */
+int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int max_depth, uint32_t flags, int32_t* edge_ids,
+                       int32_t* rows, int64_t capacity, pgc_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PGC_H_ */

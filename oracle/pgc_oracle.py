@@ -1,0 +1,150 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so). TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, the smoke check and the benchmark's CPU-baseline leg; never
+by the product package. Mirrors the reference's CPU path:
+product_graph_generators/product_spaces/product_space.cpp:19-132,
+product_graph_generators/energy/deformationEnergy.cpp:11-103,
+product_graph_generators/cost_time_ratio_solver/cost_time_ratio_solver.cpp:42-905,
+product_graph_generators/dijkstra_solver/dijkstra_solver.cpp:41-579.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_product_space.restype = C.c_long
+        L.orc_product_space.argtypes = [P, C.c_long, P, C.c_long, C.c_int, P, P, P]
+        L.orc_energy.restype = C.c_int
+        L.orc_energy.argtypes = [P, C.c_long, P, C.c_long, P, C.c_long, P, C.c_int, C.c_char_p, C.c_char_p, P]
+        L.orc_median.restype = C.c_double
+        L.orc_median.argtypes = [P, C.c_long]
+        L.orc_quantize.restype = None
+        L.orc_quantize.argtypes = [P, C.c_long, C.c_int]
+        L.orc_ctr_solve.restype = C.c_int
+        L.orc_ctr_solve.argtypes = [P, P, C.c_long, P, C.c_int, C.c_long, C.c_int, C.c_int, P, C.c_long,
+                                    P, P, P, P]
+        L.orc_dijkstra_solve.restype = C.c_int
+        L.orc_dijkstra_solve.argtypes = [P, P, C.c_long, P, C.c_long, C.c_int, P, P, C.c_int, P, C.c_long,
+                                         P, P, P, P]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_set_threads.argtypes = [C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def product_space(EX, EY, prune=False):
+    EX = np.ascontiguousarray(EX, dtype=np.int32)
+    EY = np.ascontiguousarray(EY, dtype=np.int32)
+    nVX = int(EX.max()) + 1
+    n = len(EY) * ((len(EX) if not prune else 0) + len(EX) + nVX)
+    rows = np.zeros((n, 4), np.int32)
+    src = np.zeros(n, np.int32)
+    trgt = np.zeros(n, np.int32)
+    m = lib().orc_product_space(_p(EX), len(EX), _p(EY), len(EY), int(prune), _p(rows), _p(src), _p(trgt))
+    assert m == n
+    return rows, src, trgt
+
+
+def energy(VX, VY, rows, FD, reg=False, cost_mode="vanilla", time_mode=""):
+    VX = np.ascontiguousarray(VX, dtype=np.float64)
+    VY = np.ascontiguousarray(VY, dtype=np.float64)
+    FD = np.ascontiguousarray(FD, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    E = len(rows)
+    out = np.zeros(2 * E, np.float64)
+    nc = lib().orc_energy(_p(VX), len(VX), _p(VY), len(VY), _p(rows), E, _p(FD), int(reg),
+                          cost_mode.encode(), time_mode.encode(), _p(out))
+    return out[: nc * E].reshape(nc, E).T.copy()  # [E, ncols]
+
+
+def median(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_median(_p(x), len(x))
+
+
+def quantize(cost):
+    """cost [E, ncols] float64 -> quantized copy."""
+    cm = np.ascontiguousarray(np.asarray(cost, np.float64).T).copy()
+    lib().orc_quantize(_p(cm), cm.shape[1], cm.shape[0])
+    return cm.T.copy()
+
+
+def ctr_solve(src, trgt, cost, npl, K, gpu_pick=False):
+    """Lawler ratio solve. Returns (edge_ids, objective float32, passes, stats, quantized cost)."""
+    src = np.ascontiguousarray(src, np.int32)
+    trgt = np.ascontiguousarray(trgt, np.int32)
+    cost = np.asarray(cost, np.float64)
+    if cost.ndim == 1:
+        cost = cost[:, None]
+    cm = np.ascontiguousarray(cost.T).copy()
+    E = len(src)
+    L = (int(src.max()) + 1) // npl
+    cap = max(64, 4 * (L + 1) * K * npl)
+    out = np.zeros(cap, np.int32)
+    n = C.c_long(0)
+    obj = C.c_float(0)
+    passes = C.c_int(0)
+    stats = np.zeros(5, np.int64)
+    rc = lib().orc_ctr_solve(_p(src), _p(trgt), E, _p(cm), cm.shape[0], npl, K, int(gpu_pick), _p(out), cap,
+                             C.byref(n), C.byref(obj), C.byref(passes), _p(stats))
+    if rc != 0:
+        raise RuntimeError("oracle ctr solve failed: %s" % lib().orc_last_error().decode())
+    return out[: n.value].copy(), np.float32(obj.value), bool(passes.value), stats, cm.T.copy()
+
+
+def dijkstra_solve(src, trgt, cost0, npl, K, branch_graph=None, cpu_rules=False):
+    """Minimum-cost solve on raw column-0 costs. branch_graph: list of value lists or None."""
+    src = np.ascontiguousarray(src, np.int32)
+    trgt = np.ascontiguousarray(trgt, np.int32)
+    cost0 = np.ascontiguousarray(np.asarray(cost0, np.float64).reshape(-1))
+    E = len(src)
+    L = (int(src.max()) + 1) // npl
+    cap = max(64, 4 * (L + 1) * K)
+    out = np.zeros(cap, np.int32)
+    n = C.c_long(0)
+    obj = C.c_float(0)
+    passes = C.c_int(0)
+    stats = np.zeros(5, np.int64)
+    if branch_graph is not None:
+        off = np.zeros(npl + 1, np.int64)
+        vals = []
+        for v in range(npl):
+            s = sorted(branch_graph[v]) if v < len(branch_graph) else []
+            vals += s
+            off[v + 1] = len(vals)
+        val = np.array(vals if vals else [0], np.int64)
+        bo, bv = _p(off), _p(val)
+    else:
+        bo = bv = None
+    rc = lib().orc_dijkstra_solve(_p(src), _p(trgt), E, _p(cost0), npl, K, bo, bv, int(cpu_rules), _p(out), cap,
+                                  C.byref(n), C.byref(obj), C.byref(passes), _p(stats))
+    if rc != 0:
+        raise RuntimeError("oracle dijkstra solve failed: %s" % lib().orc_last_error().decode())
+    return out[: n.value].copy(), np.float32(obj.value), bool(passes.value), stats

@@ -1,0 +1,433 @@
+// On-device construction of the layered product graph and its per-edge
+// cost/time, q-unit quantization, and the sliced-ELL in-edge layout used by
+// the solver kernels.
+//
+// Reference counterparts:
+//   ProductSpace::computeCombinations        product_spaces/product_space.cpp:19-132
+//   DeformationEnergy::computeEnergy         energy/deformationEnergy.cpp:11-103
+//   robustFloatToIntConversion + median      cost_time_ratio_solver.cpp:709-721, helper/utils.hpp:196-209
+//   buildingInGraph (+ VECSET flattening)    cost_time_ratio_solver.cpp:42-66, ctr_bnb_subroutine.cuh:232-287
+// All kernels are HBM-streaming integer/fp64 work: one thread per product
+// edge (or per slot), coalesced row-major writes, no LDS staging needed
+// except for the 256-bin radix-select histograms.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <math.h>
+#include <stdint.h>
+
+#include "pgc_internal.h"
+
+namespace pgc {
+
+// ------------------------------------------------------- product space (a1)
+// Edge e of layer i = e / B, r = e % B with B = 2|EX| + nVX (|EX| + nVX when
+// intralayer edges are pruned): r < 2|EX| -> X edge j = r/2, intralayer when r
+// is even; otherwise the degenerate edge of vertex r - 2|EX|.
+__global__ void k_product_space(const int32_t* __restrict__ EX, int64_t nEX, const int32_t* __restrict__ EY,
+                                int64_t nEY, int64_t nVX, int prune, int32_t* __restrict__ rows,
+                                int32_t* __restrict__ src, int32_t* __restrict__ trgt, int64_t E) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int64_t per_x = prune ? 1 : 2;
+  const int64_t B = per_x * nEX + nVX;
+  const int64_t i = e / B;
+  const int64_t r = e - i * B;
+  const int64_t t = (i + 1) % nEY;
+  const int32_t y0 = EY[2 * i], y1 = EY[2 * i + 1];
+  int32_t r0, r1, x0, x1;
+  int64_t s, d;
+  if (r < per_x * nEX) {
+    const int64_t j = r / per_x;
+    const bool intra = !prune && (r % 2 == 0);
+    x0 = EX[2 * j];
+    x1 = EX[2 * j + 1];
+    r0 = y0;
+    r1 = intra ? y0 : y1;
+    s = i * nVX + x0;
+    d = (intra ? i : t) * nVX + x1;
+  } else {
+    const int64_t x = r - per_x * nEX;
+    x0 = x1 = (int32_t)x;
+    r0 = y0;
+    r1 = y1;
+    s = i * nVX + x;
+    d = t * nVX + x;
+  }
+  int4 row;
+  row.x = r0;
+  row.y = r1;
+  row.z = x0;
+  row.w = x1;
+  reinterpret_cast<int4*>(rows)[e] = row;
+  src[e] = (int32_t)s;
+  trgt[e] = (int32_t)d;
+}
+
+// ------------------------------------------------------------- energy (a3)
+// FD is |VX| x |VY| row-major (fp64); out is column-major E x ncols.
+__global__ void k_energy(const double* __restrict__ VX, const double* __restrict__ VY,
+                         const int32_t* __restrict__ rows, int64_t E, const double* __restrict__ FD, int64_t nVY,
+                         int lineIntegral, int lengthMode, double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int4 row = reinterpret_cast<const int4*>(rows)[e];
+  const int64_t y0 = row.x, y1 = row.y, x0 = row.z, x1 = row.w;
+  double len = 1.0;
+  if (lineIntegral) {
+    len = 0;
+    for (int j = 0; j < 3; ++j) {
+      const double a = VY[3 * y0 + j] - VY[3 * y1 + j];
+      len = len + __dmul_rn(a, a);
+      const double b = VX[3 * x0 + j] - VX[3 * x1 + j];
+      len = len + __dmul_rn(b, b);
+    }
+    len = sqrt(len);
+  }
+  const double fd = FD[x0 * nVY + y0] + FD[x1 * nVY + y1];
+  if (!lengthMode) {
+    out[e] = __dmul_rn(__dmul_rn(0.5, fd), len);
+  } else {
+    out[e] = fd;
+    out[E + e] = len;
+  }
+}
+
+// ------------------------------------------- exact median by radix select
+// Order-preserving map of doubles to unsigned integers.
+__device__ __forceinline__ uint64_t dkey(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dkey_inv(uint64_t k) {
+  const uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)u);
+}
+
+struct SelectState {
+  uint64_t prefix[2];
+  uint64_t mask[2];
+  int64_t k[2];
+  uint32_t hist[2][256];
+  double value[2];
+};
+
+__global__ void k_select_init(SelectState* st, int64_t k0, int64_t k1) {
+  const int t = threadIdx.x;
+  if (t < 256) {
+    st->hist[0][t] = 0;
+    st->hist[1][t] = 0;
+  }
+  if (t == 0) {
+    st->prefix[0] = st->prefix[1] = 0;
+    st->mask[0] = st->mask[1] = 0;
+    st->k[0] = k0;
+    st->k[1] = k1;
+  }
+}
+
+__global__ void k_select_hist(const double* __restrict__ x, int64_t n, SelectState* st, int shift) {
+  __shared__ uint32_t h[2][256];
+  h[0][threadIdx.x] = 0;
+  h[1][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t p0 = st->prefix[0], m0 = st->mask[0], p1 = st->prefix[1], m1 = st->mask[1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t kk = dkey(x[i]);
+    const uint32_t b = (uint32_t)(kk >> shift) & 255u;
+    if ((kk & m0) == p0) atomicAdd(&h[0][b], 1u);
+    if ((kk & m1) == p1) atomicAdd(&h[1][b], 1u);
+  }
+  __syncthreads();
+  if (h[0][threadIdx.x]) atomicAdd(&st->hist[0][threadIdx.x], h[0][threadIdx.x]);
+  if (h[1][threadIdx.x]) atomicAdd(&st->hist[1][threadIdx.x], h[1][threadIdx.x]);
+}
+
+__global__ void k_select_step(SelectState* st, int shift) {
+  if (threadIdx.x < 2) {
+    const int t = threadIdx.x;
+    int64_t cum = 0;
+    int b = 0;
+    for (; b < 256; ++b) {
+      const int64_t c = st->hist[t][b];
+      if (st->k[t] < cum + c) break;
+      cum += c;
+    }
+    if (b > 255) b = 255;
+    st->k[t] -= cum;
+    st->prefix[t] |= ((uint64_t)b) << shift;
+    st->mask[t] |= ((uint64_t)255) << shift;
+    if (shift == 0) st->value[t] = dkey_inv(st->prefix[t]);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) (&st->hist[0][0])[i] = 0;
+}
+
+// col <- min(round(s*col), 100*scaling) with s = scaling / (2 max(median, 0.001))
+__global__ void k_quantize(double* __restrict__ col, int64_t n, const SelectState* st, int even, double scaling) {
+  const double med = even ? (st->value[0] + st->value[1]) / 2.0 : st->value[0];
+  const double m = med > 0.001 ? med : 0.001;
+  const double s = scaling / (2.0 * m);
+  const double maxVal = 100 * scaling;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double q = round(__dmul_rn(s, col[i]));
+    if (q > maxVal) q = maxVal;
+    col[i] = q;
+  }
+}
+
+// ------------------------------------------------- sliced-ELL in-edge layout
+// class 0: source on the target's layer (intralayer), class 1: source on the
+// previous layer (interlayer). Targets on layer 0 whose source id is > npl
+// move to the closing copy numNodes + target (the reference's dummy nodes).
+__device__ __forceinline__ void classify(int64_t s, int64_t t, int64_t npl, int64_t numNodes, int64_t& node,
+                                         int& cls) {
+  if (t < npl && s > npl) t += numNodes;
+  node = t;
+  const int64_t lt = t / npl, ls = s / npl;
+  cls = (ls == lt) ? 0 : ((ls == lt - 1) ? 1 : -1);
+}
+
+__global__ void k_count(const int32_t* __restrict__ src, const int32_t* __restrict__ trgt, int64_t E, int64_t npl,
+                        int64_t numNodes, int32_t* __restrict__ cnt, int32_t* __restrict__ dropped) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int64_t node;
+  int cls;
+  classify(src[e], trgt[e], npl, numNodes, node, cls);
+  const int64_t nTot = numNodes + npl;
+  if (cls < 0 || node >= nTot) {
+    atomicAdd(dropped, 1);
+    return;
+  }
+  atomicAdd(&cnt[cls * nTot + node], 1);
+}
+
+// width (in slots per lane) of every (class, layer, chunk); chunks = 64 vertices
+__global__ void k_chunk_width(const int32_t* __restrict__ cnt, int64_t npl, int64_t nLayers, int nChunks,
+                              int32_t* __restrict__ width) {
+  const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t per = nLayers * nChunks;
+  if (g >= 2 * per) return;
+  const int cls = (int)(g / per);
+  const int64_t r = g - cls * per;
+  const int64_t layer = r / nChunks;
+  const int c = (int)(r - layer * nChunks);
+  const int64_t v = (int64_t)c * 64 + lane;
+  int w = 0;
+  if (v < npl) w = cnt[cls * nLayers * npl + layer * npl + v];
+  for (int o = 32; o > 0; o >>= 1) w = max(w, __shfl_xor(w, o));
+  if (lane == 0) width[g] = w;
+}
+
+// exclusive scan of width*64 for both classes (single workgroup, n small)
+__global__ void k_scan_offsets(const int32_t* __restrict__ width, int64_t per, int32_t* __restrict__ off0,
+                               int32_t* __restrict__ off1, int64_t* __restrict__ totals) {
+  __shared__ int64_t part[1024];
+  for (int cls = 0; cls < 2; ++cls) {
+    const int32_t* w = width + cls * per;
+    int32_t* off = cls == 0 ? off0 : off1;
+    const int64_t chunk = (per + blockDim.x - 1) / blockDim.x;
+    const int64_t a = threadIdx.x * chunk, b = min(per, a + chunk);
+    int64_t s = 0;
+    for (int64_t i = a; i < b; ++i) s += (int64_t)w[i] * 64;
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t run = 0;
+      for (int i = 0; i < (int)blockDim.x; ++i) {
+        const int64_t x = part[i];
+        part[i] = run;
+        run += x;
+      }
+      totals[cls] = run;
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int64_t i = a; i < b; ++i) {
+      off[i] = (int32_t)run;
+      run += (int64_t)w[i] * 64;
+    }
+    if (threadIdx.x == blockDim.x - 1) off[per] = (int32_t)totals[cls];
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_slots(const int32_t* __restrict__ src, const int32_t* __restrict__ trgt, int64_t E,
+                             int64_t npl, int64_t numNodes, int nChunks, int32_t* __restrict__ fill,
+                             const int32_t* __restrict__ off0, const int32_t* __restrict__ off1,
+                             int32_t* __restrict__ s0, int32_t* __restrict__ s1, int32_t* __restrict__ e0,
+                             int32_t* __restrict__ e1) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int64_t node;
+  int cls;
+  const int64_t s = src[e];
+  classify(s, trgt[e], npl, numNodes, node, cls);
+  const int64_t nTot = numNodes + npl;
+  if (cls < 0 || node >= nTot) return;
+  const int pos = atomicAdd(&fill[cls * nTot + node], 1);
+  const int64_t layer = node / npl;
+  const int64_t v = node - layer * npl;
+  const int c = (int)(v >> 6), lane = (int)(v & 63);
+  const int32_t* off = cls == 0 ? off0 : off1;
+  const int64_t slot = (int64_t)off[layer * nChunks + c] + (int64_t)pos * 64 + lane;
+  const int64_t sl = s - (cls == 0 ? layer : layer - 1) * npl;
+  (cls == 0 ? s0 : s1)[slot] = (int32_t)sl;
+  (cls == 0 ? e0 : e1)[slot] = (int32_t)e;
+}
+
+// order the slots of every node by (source, edge id)
+__global__ void k_sort_slots(const int32_t* __restrict__ cnt, int64_t npl, int64_t nLayers, int nChunks,
+                             const int32_t* __restrict__ off0, const int32_t* __restrict__ off1,
+                             int32_t* __restrict__ s0, int32_t* __restrict__ s1, int32_t* __restrict__ e0,
+                             int32_t* __restrict__ e1) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nTot = nLayers * npl;
+  if (g >= 2 * nTot) return;
+  const int cls = (int)(g / nTot);
+  const int64_t node = g - cls * nTot;
+  const int n = cnt[g];
+  if (n <= 1) return;
+  const int64_t layer = node / npl;
+  const int64_t v = node - layer * npl;
+  const int c = (int)(v >> 6), lane = (int)(v & 63);
+  const int32_t* off = cls == 0 ? off0 : off1;
+  int32_t* S = cls == 0 ? s0 : s1;
+  int32_t* Ee = cls == 0 ? e0 : e1;
+  const int64_t base = (int64_t)off[layer * nChunks + c] + lane;
+  for (int i = 1; i < n; ++i) {
+    const int32_t ks = S[base + (int64_t)i * 64], ke = Ee[base + (int64_t)i * 64];
+    int j = i - 1;
+    while (j >= 0) {
+      const int32_t js = S[base + (int64_t)j * 64], je = Ee[base + (int64_t)j * 64];
+      if (js < ks || (js == ks && je < ke)) break;
+      S[base + (int64_t)(j + 1) * 64] = js;
+      Ee[base + (int64_t)(j + 1) * 64] = je;
+      --j;
+    }
+    S[base + (int64_t)(j + 1) * 64] = ks;
+    Ee[base + (int64_t)(j + 1) * 64] = ke;
+  }
+}
+
+// per-slot cost (float q-units or raw double) and time gathered by edge id
+template <typename CT>
+__global__ void k_fill_costs(const int32_t* __restrict__ eid, const int32_t* __restrict__ s, int64_t nSlots,
+                             const double* __restrict__ energy, int64_t E, int ncols, CT* __restrict__ cost,
+                             float* __restrict__ time) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nSlots) return;
+  if (s[i] < 0) {
+    cost[i] = (CT)0;
+    if (time) time[i] = 0.f;
+    return;
+  }
+  const int64_t e = eid[i];
+  cost[i] = (CT)energy[e];
+  if (time) time[i] = (float)energy[E + e];
+}
+
+// rows of the given edges (output matching)
+__global__ void k_gather_rows(const int32_t* __restrict__ rows, int cols, const int32_t* __restrict__ eids,
+                              int64_t n, int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * cols) return;
+  const int64_t r = i / cols, c = i - r * cols;
+  out[i] = rows[(int64_t)eids[r] * cols + c];
+}
+
+// layer-0 intralayer in-edges (branch graph of the min-cost branching)
+__global__ void k_int_max(const int32_t* __restrict__ x, int64_t n, int32_t* __restrict__ out) {
+  int m = -2147483647 - 1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, x[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// ------------------------------------------------------------ launchers
+static inline unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_product_space(const int32_t* EX, int64_t nEX, const int32_t* EY, int64_t nEY, int64_t nVX,
+                                int prune, int32_t* rows, int32_t* src, int32_t* trgt, int64_t E, hipStream_t st) {
+  hipLaunchKernelGGL(k_product_space, dim3(nb(E, 256)), dim3(256), 0, st, EX, nEX, EY, nEY, nVX, prune, rows, src,
+                     trgt, E);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const double* VX, const double* VY, const int32_t* rows, int64_t E, const double* FD,
+                         int64_t nVY, int lineIntegral, int lengthMode, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy, dim3(nb(E, 256)), dim3(256), 0, st, VX, VY, rows, E, FD, nVY, lineIntegral,
+                     lengthMode, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_quantize(double* col, int64_t n, void* selectState, hipStream_t st) {
+  SelectState* S = (SelectState*)selectState;
+  const int64_t k1 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
+  const int64_t k2 = (n % 2 == 0) ? k1 + 1 : k1;
+  hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, st, S, k1, k2);
+  const unsigned g = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 255) / 256));
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_select_hist, dim3(g), dim3(256), 0, st, col, n, S, shift);
+    hipLaunchKernelGGL(k_select_step, dim3(1), dim3(256), 0, st, S, shift);
+  }
+  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, st, col, n, S, (int)(n % 2 == 0), 100000.0);
+  return hipGetLastError();
+}
+
+size_t select_state_bytes() { return sizeof(SelectState); }
+
+hipError_t launch_layout_count(const int32_t* src, const int32_t* trgt, int64_t E, int64_t npl, int64_t numNodes,
+                               int32_t* cnt, int32_t* dropped, hipStream_t st) {
+  hipLaunchKernelGGL(k_count, dim3(nb(E, 256)), dim3(256), 0, st, src, trgt, E, npl, numNodes, cnt, dropped);
+  return hipGetLastError();
+}
+
+hipError_t launch_layout_offsets(const int32_t* cnt, int64_t npl, int64_t nLayers, int nChunks, int32_t* width,
+                                 int32_t* off0, int32_t* off1, int64_t* totals, hipStream_t st) {
+  const int64_t groups = 2 * nLayers * nChunks;
+  hipLaunchKernelGGL(k_chunk_width, dim3(nb(groups, 4)), dim3(256), 0, st, cnt, npl, nLayers, nChunks, width);
+  hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, width, nLayers * nChunks, off0, off1, totals);
+  return hipGetLastError();
+}
+
+hipError_t launch_layout_fill(const int32_t* src, const int32_t* trgt, int64_t E, int64_t npl, int64_t numNodes,
+                              int nChunks, int32_t* fill, const int32_t* cnt, const int32_t* off0,
+                              const int32_t* off1, int32_t* s0, int32_t* s1, int32_t* e0, int32_t* e1,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_slots, dim3(nb(E, 256)), dim3(256), 0, st, src, trgt, E, npl, numNodes, nChunks, fill,
+                     off0, off1, s0, s1, e0, e1);
+  const int64_t nLayers = numNodes / npl + 1;
+  hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nLayers * npl, 256)), dim3(256), 0, st, cnt, npl, nLayers, nChunks,
+                     off0, off1, s0, s1, e0, e1);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_costs(const int32_t* eid, const int32_t* s, int64_t nSlots, const double* energy, int64_t E,
+                             int ncols, bool asDouble, void* cost, float* time, hipStream_t st) {
+  if (nSlots == 0) return hipSuccess;
+  if (asDouble)
+    hipLaunchKernelGGL(k_fill_costs<double>, dim3(nb(nSlots, 256)), dim3(256), 0, st, eid, s, nSlots, energy, E,
+                       ncols, (double*)cost, time);
+  else
+    hipLaunchKernelGGL(k_fill_costs<float>, dim3(nb(nSlots, 256)), dim3(256), 0, st, eid, s, nSlots, energy, E,
+                       ncols, (float*)cost, time);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const int32_t* rows, int cols, const int32_t* eids, int64_t n, int32_t* out,
+                              hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_rows, dim3(nb(n * cols, 256)), dim3(256), 0, st, rows, cols, eids, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_int_max(const int32_t* x, int64_t n, int32_t* out, hipStream_t st) {
+  const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+  hipLaunchKernelGGL(k_int_max, dim3(g), dim3(256), 0, st, x, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace pgc

@@ -1,0 +1,997 @@
+// C-ABI of the MI355X product-graph cycle solvers (include/pgc.h).
+//
+// Host side: graph and scratch management, and the branch-and-bound drivers
+// around the device solver kernels:
+//   solveWithBnbGPULawler   cost_time_ratio_solver_cuda.cpp:175-331 (+ cost_time_ratio_solver.cpp:741-864)
+//   solveWithBnBCuda        dijkstra_solver_cuda.cpp:136-322 (+ dijkstra_solver.cpp:407-559)
+// Every Lawler subroutine, sweep, fold, cycle walk and lambda update runs on
+// the device (solver.hip); per branch the host reads back one small result
+// record. Error paths return a status instead of exiting the process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pgc.h"
+#include "bnb_heap.h"
+#include "pgc_internal.h"
+
+namespace pgc {
+// graph_build.hip
+hipError_t launch_product_space(const int32_t*, int64_t, const int32_t*, int64_t, int64_t, int, int32_t*, int32_t*,
+                                int32_t*, int64_t, hipStream_t);
+hipError_t launch_energy(const double*, const double*, const int32_t*, int64_t, const double*, int64_t, int, int,
+                         double*, hipStream_t);
+hipError_t launch_quantize(double*, int64_t, void*, hipStream_t);
+size_t select_state_bytes();
+hipError_t launch_layout_count(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int32_t*, int32_t*,
+                               hipStream_t);
+hipError_t launch_layout_offsets(const int32_t*, int64_t, int64_t, int, int32_t*, int32_t*, int32_t*, int64_t*,
+                                 hipStream_t);
+hipError_t launch_layout_fill(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int, int32_t*,
+                              const int32_t*, const int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*, int32_t*,
+                              hipStream_t);
+hipError_t launch_fill_costs(const int32_t*, const int32_t*, int64_t, const double*, int64_t, int, bool, void*,
+                             float*, hipStream_t);
+hipError_t launch_gather_rows(const int32_t*, int, const int32_t*, int64_t, int32_t*, hipStream_t);
+hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
+// solver.hip
+size_t solver_lds_bytes(int npl, int K);
+hipError_t launch_solver(const SolverJob*, int, int, bool, bool, bool, size_t, hipStream_t);
+hipError_t launch_lookup(const SolverJob*, const int32_t*, int, int32_t*, bool, bool, hipStream_t);
+}  // namespace pgc
+
+using namespace pgc;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                       \
+  do {                                                                                                     \
+    hipError_t e_ = (expr);                                                                                \
+    if (e_ != hipSuccess) return fail(PGC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+struct pgc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* select_state = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+// Device scratch of one solver job (sized for one max_depth).
+struct JobScratch {
+  int K = 0;
+  int64_t predN = 0, chainN = 0;
+  int cyc_cap = 0;
+  int32_t *pred = nullptr, *wnext = nullptr, *wlen = nullptr, *chain = nullptr;
+  int32_t *tmp_nodes = nullptr, *tmp_eid = nullptr, *neg_nodes = nullptr, *neg_eid = nullptr, *neg_len = nullptr;
+  double *tmp_c = nullptr, *tmp_t = nullptr, *neg_c = nullptr, *neg_t = nullptr;
+  int32_t *hits = nullptr, *back = nullptr, *branch = nullptr, *nodes_io = nullptr, *eids_io = nullptr;
+  double* lb_key = nullptr;
+  JobResult* res = nullptr;
+  SolverJob* djob = nullptr;
+  void release() {
+    void* ps[] = {pred, wnext, wlen, chain, tmp_nodes, tmp_eid, neg_nodes, neg_eid, neg_len, tmp_c, tmp_t,
+                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, res, djob};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    *this = JobScratch();
+  }
+};
+
+}  // namespace
+
+struct pgc_graph {
+  pgc_context* ctx = nullptr;
+  int64_t E = 0, npl = 0, numNodes = 0, L = 0;
+  int ps_cols = 4, ncols = 1;
+  int32_t *d_rows = nullptr, *d_src = nullptr, *d_trgt = nullptr;
+  double* d_energy = nullptr;
+  // layout
+  bool layout = false;
+  int nChunks = 0;
+  int64_t nLayers = 0;
+  int64_t nslots[2] = {0, 0};
+  int32_t *d_off[2] = {nullptr, nullptr}, *d_s[2] = {nullptr, nullptr}, *d_e[2] = {nullptr, nullptr};
+  int32_t* d_cnt = nullptr;
+  void* d_cost[2] = {nullptr, nullptr};
+  float* d_time[2] = {nullptr, nullptr};
+  int cost_kind = 0;  // 0 none, 1 float q-units (+time), 2 double raw
+  int64_t dropped = 0;
+  int64_t intra_slots_valid = 0, inter_slots_valid = 0;
+  // branch graph (host CSR over layer-0 nodes)
+  bool has_bg = false;
+  std::vector<int64_t> bg_off, bg_val;
+  JobScratch js;
+  ~pgc_graph() {
+    js.release();
+    void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_off[0], d_off[1], d_s[0], d_s[1], d_e[0], d_e[1],
+                  d_cnt, d_cost[0], d_cost[1], d_time[0], d_time[1]};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+const char* pgc_last_error(void) { return g_last_error.c_str(); }
+int pgc_abi_version(void) { return 1; }
+
+int pgc_context_create(int device, pgc_context** out) {
+  if (!out) return fail(PGC_ERR_ARG, "out is null");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(PGC_ERR_ARG, "invalid device ordinal");
+  HIPCHK(hipSetDevice(device));
+  auto* c = new pgc_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->select_state, select_state_bytes());
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(PGC_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return PGC_OK;
+}
+
+void pgc_context_destroy(pgc_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->select_state) (void)hipFree(c->select_state);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"
+
+namespace {
+
+int finish_graph_meta(pgc_graph* g) {
+  // numNodes = max(SRC) + 1, numLayers = numNodes / npl (run(), :870-871)
+  int32_t* d_max = nullptr;
+  HIPCHK(dalloc(&d_max, 1));
+  const int32_t init = -1;
+  HIPCHK(hipMemcpyAsync(d_max, &init, 4, hipMemcpyHostToDevice, g->ctx->stream));
+  HIPCHK(launch_int_max(g->d_src, g->E, d_max, g->ctx->stream));
+  int32_t mx = -1;
+  HIPCHK(hipMemcpyAsync(&mx, d_max, 4, hipMemcpyDeviceToHost, g->ctx->stream));
+  HIPCHK(hipStreamSynchronize(g->ctx->stream));
+  (void)hipFree(d_max);
+  g->numNodes = (int64_t)mx + 1;
+  g->L = g->numNodes / g->npl;
+  if (g->L < 1) return fail(PGC_ERR_ARG, "product graph has fewer nodes than one layer");
+  if (g->numNodes % g->npl != 0)
+    return fail(PGC_ERR_UNSUPPORTED, "max(SRC)+1 is not a multiple of the nodes per layer");
+  return PGC_OK;
+}
+
+int build_layout(pgc_graph* g) {
+  if (g->layout) return PGC_OK;
+  hipStream_t st = g->ctx->stream;
+  const int64_t npl = g->npl;
+  g->nLayers = g->L + 1;
+  g->nChunks = (int)((npl + 63) / 64);
+  const int64_t nTot = g->nLayers * npl;
+  const int64_t per = g->nLayers * g->nChunks;
+  int32_t *d_dropped = nullptr, *d_width = nullptr, *d_fill = nullptr;
+  int64_t* d_tot = nullptr;
+  HIPCHK(dalloc(&g->d_cnt, 2 * nTot));
+  HIPCHK(dalloc(&d_fill, 2 * nTot));
+  HIPCHK(dalloc(&d_dropped, 1));
+  HIPCHK(dalloc(&d_width, 2 * per));
+  HIPCHK(dalloc(&d_tot, 2));
+  HIPCHK(dalloc(&g->d_off[0], per + 1));
+  HIPCHK(dalloc(&g->d_off[1], per + 1));
+  HIPCHK(hipMemsetAsync(g->d_cnt, 0, 2 * nTot * 4, st));
+  HIPCHK(hipMemsetAsync(d_fill, 0, 2 * nTot * 4, st));
+  HIPCHK(hipMemsetAsync(d_dropped, 0, 4, st));
+  HIPCHK(launch_layout_count(g->d_src, g->d_trgt, g->E, npl, g->numNodes, g->d_cnt, d_dropped, st));
+  HIPCHK(launch_layout_offsets(g->d_cnt, npl, g->nLayers, g->nChunks, d_width, g->d_off[0], g->d_off[1], d_tot, st));
+  int64_t tot[2];
+  int32_t dropped = 0;
+  HIPCHK(hipMemcpyAsync(tot, d_tot, sizeof(tot), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&dropped, d_dropped, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (tot[0] >= (int64_t)1 << 31 || tot[1] >= (int64_t)1 << 31)
+    return fail(PGC_ERR_UNSUPPORTED, "in-edge layout exceeds 2^31 slots");
+  g->dropped = dropped;
+  for (int c = 0; c < 2; ++c) {
+    g->nslots[c] = tot[c];
+    HIPCHK(dalloc(&g->d_s[c], tot[c]));
+    HIPCHK(dalloc(&g->d_e[c], tot[c]));
+    HIPCHK(hipMemsetAsync(g->d_s[c], 0xff, std::max<int64_t>(tot[c], 1) * 4, st));
+    HIPCHK(hipMemsetAsync(g->d_e[c], 0xff, std::max<int64_t>(tot[c], 1) * 4, st));
+  }
+  HIPCHK(launch_layout_fill(g->d_src, g->d_trgt, g->E, npl, g->numNodes, g->nChunks, d_fill, g->d_cnt, g->d_off[0],
+                            g->d_off[1], g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
+  HIPCHK(hipStreamSynchronize(st));
+  (void)hipFree(d_dropped);
+  (void)hipFree(d_width);
+  (void)hipFree(d_fill);
+  (void)hipFree(d_tot);
+  g->layout = true;
+  return PGC_OK;
+}
+
+// kind 1: float q-unit cost (+ float time when a time column exists); kind 2: double raw cost
+int fill_costs(pgc_graph* g, int kind) {
+  hipStream_t st = g->ctx->stream;
+  for (int c = 0; c < 2; ++c) {
+    if (g->d_cost[c]) (void)hipFree(g->d_cost[c]);
+    if (g->d_time[c]) (void)hipFree(g->d_time[c]);
+    g->d_cost[c] = nullptr;
+    g->d_time[c] = nullptr;
+    const int64_t n = g->nslots[c];
+    HIPCHK(hipMalloc(&g->d_cost[c], std::max<int64_t>(n, 1) * (kind == 2 ? 8 : 4)));
+    const bool withTime = kind == 1 && g->ncols > 1;
+    if (withTime) HIPCHK(dalloc(&g->d_time[c], n));
+    HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], n, g->d_energy, g->E, g->ncols, kind == 2, g->d_cost[c],
+                             withTime ? g->d_time[c] : nullptr, st));
+  }
+  g->cost_kind = kind;
+  return PGC_OK;
+}
+
+int ensure_scratch(pgc_graph* g, int K) {
+  JobScratch& s = g->js;
+  if (s.K == K && s.pred) return PGC_OK;
+  s.release();
+  const int64_t npl = g->npl, L = g->L;
+  s.K = K;
+  s.predN = (L + 1) * K * npl;
+  const int64_t stride = L * K + 2;
+  s.chainN = npl * stride;
+  const int64_t cap = 100 * L * K + 2;  // the reference's cycle buffer (cost_time_ratio_solver_cuda.cpp:194)
+  if (cap >= ((int64_t)1 << 31) || s.predN >= ((int64_t)1 << 31))
+    return fail(PGC_ERR_UNSUPPORTED, "expanded graph exceeds int32 node ids");
+  s.cyc_cap = (int)cap;
+  HIPCHK(dalloc(&s.pred, s.predN));
+  HIPCHK(dalloc(&s.wnext, npl));
+  HIPCHK(dalloc(&s.wlen, npl));
+  HIPCHK(dalloc(&s.chain, s.chainN));
+  HIPCHK(dalloc(&s.tmp_nodes, cap));
+  HIPCHK(dalloc(&s.tmp_eid, cap));
+  HIPCHK(dalloc(&s.neg_nodes, cap));
+  HIPCHK(dalloc(&s.neg_eid, cap));
+  HIPCHK(dalloc(&s.neg_len, 1));
+  HIPCHK(dalloc(&s.tmp_c, cap));
+  HIPCHK(dalloc(&s.tmp_t, cap));
+  HIPCHK(dalloc(&s.neg_c, cap));
+  HIPCHK(dalloc(&s.neg_t, cap));
+  HIPCHK(dalloc(&s.hits, npl));
+  HIPCHK(dalloc(&s.back, npl));
+  HIPCHK(dalloc(&s.branch, npl));
+  HIPCHK(dalloc(&s.nodes_io, cap));
+  HIPCHK(dalloc(&s.eids_io, cap));
+  HIPCHK(dalloc(&s.lb_key, npl));
+  HIPCHK(dalloc(&s.res, 1));
+  HIPCHK(dalloc(&s.djob, 1));
+  return PGC_OK;
+}
+
+SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
+  JobScratch& s = g->js;
+  SolverJob j;
+  std::memset(&j, 0, sizeof(j));
+  j.npl = (int32_t)g->npl;
+  j.L = (int32_t)g->L;
+  j.K = K;
+  j.nChunks = g->nChunks;
+  j.numNodes = g->numNodes;
+  j.mode = mode;
+  j.flags = flags;
+  j.branchId = 0;
+  j.chain_stride = (int32_t)(g->L * K + 2);
+  j.cyc_cap = s.cyc_cap;
+  j.ratio_in = 0;
+  j.upper_bound = std::numeric_limits<double>::infinity();
+  j.time_const = 100000.0 / 2;  // float2intScaling / 2 (cost_time_ratio_solver.cpp:770-777)
+  j.tol = (double)(float)1e-8;   // the solver's float tolerance member (:79)
+  j.max_sweeps = std::numeric_limits<int64_t>::max();
+  if (const char* cap = std::getenv("PGC_SWEEP_CAP")) j.max_sweeps = std::max<int64_t>(1, std::atoll(cap));
+  for (int c = 0; c < 2; ++c) {
+    SlotClass& S = c == 0 ? j.intra : j.inter;
+    S.off = g->d_off[c];
+    S.src = g->d_s[c];
+    S.cost = g->d_cost[c];
+    S.time = g->d_time[c];
+    S.eid = g->d_e[c];
+  }
+  j.branch = s.branch;
+  j.pred = s.pred;
+  j.wnext = s.wnext;
+  j.wlen = s.wlen;
+  j.chain = s.chain;
+  j.tmp_nodes = s.tmp_nodes;
+  j.tmp_eid = s.tmp_eid;
+  j.tmp_c = s.tmp_c;
+  j.tmp_t = s.tmp_t;
+  j.neg_nodes = s.neg_nodes;
+  j.neg_eid = s.neg_eid;
+  j.neg_c = s.neg_c;
+  j.neg_t = s.neg_t;
+  j.neg_len = s.neg_len;
+  j.hits = s.hits;
+  j.lb_key = s.lb_key;
+  j.back = s.back;
+  j.res = s.res;
+  return j;
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+// one solver launch: upload job, run, read result (kernel time accumulated)
+int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, double& kms) {
+  pgc_context* c = g->ctx;
+  hipStream_t st = c->stream;
+  HIPCHK(hipMemcpyAsync(g->js.djob, &j, sizeof(j), hipMemcpyHostToDevice, st));
+  const size_t lds = solver_lds_bytes((int)g->npl, j.K);
+  const int maxc = (g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE);
+  HIPCHK(hipEventRecord(c->ev0, st));
+  HIPCHK(launch_solver(g->js.djob, 1, maxc, mincost, g->d_time[0] != nullptr, g->cost_kind == 2, lds, st));
+  HIPCHK(hipEventRecord(c->ev1, st));
+  HIPCHK(hipMemcpyAsync(&out, g->js.res, sizeof(out), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  kms += ms;
+  return PGC_OK;
+}
+
+int check_solvable(pgc_graph* g, int K) {
+  if (K < 1) return fail(PGC_ERR_ARG, "max_depth must be >= 1");
+  if (solver_lds_bytes((int)g->npl, K) > 160 * 1024 - 1024)
+    return fail(PGC_ERR_UNSUPPORTED,
+                "distance table (2K+1)*npl*8 bytes exceeds the 160 KiB LDS of one workgroup");
+  const int maxc = (int)((g->npl + 63) / 64 + 15) / 16;
+  if (maxc > 8) return fail(PGC_ERR_UNSUPPORTED, "nodes per layer above 8192");
+  return PGC_OK;
+}
+
+// E_x: relaxations attempted per sweep = intra slots*(K-1) + inter slots*K
+double relax_per_sweep(pgc_graph* g, int K) {
+  return (double)g->intra_slots_valid * (K - 1) + (double)g->inter_slots_valid * K;
+}
+
+int count_valid_slots(pgc_graph* g) {
+  // number of edges in each class (sum of in-degrees)
+  const int64_t nTot = g->nLayers * g->npl;
+  std::vector<int32_t> cnt(2 * nTot);
+  HIPCHK(hipMemcpy(cnt.data(), g->d_cnt, cnt.size() * 4, hipMemcpyDeviceToHost));
+  int64_t a = 0, b = 0;
+  for (int64_t i = 0; i < nTot; ++i) {
+    a += cnt[i];
+    b += cnt[nTot + i];
+  }
+  g->intra_slots_valid = a;
+  g->inter_slots_valid = b;
+  return PGC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgc_graph_from_arrays(pgc_context* ctx, const int32_t* productspace, int ps_cols, const int32_t* src,
+                          const int32_t* trgt, int64_t n_edges, const double* cost, int n_cols,
+                          int64_t nodes_per_layer, pgc_graph** out) {
+  if (!ctx || !out || !src || !trgt || !cost || n_edges <= 0 || nodes_per_layer <= 0)
+    return fail(PGC_ERR_ARG, "invalid arguments");
+  if (n_cols < 1 || n_cols > 2) return fail(PGC_ERR_ARG, "cost must have 1 or 2 columns");
+  if (ps_cols != 4 && ps_cols != 6) return fail(PGC_ERR_ARG, "product space must have 4 or 6 columns");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::unique_ptr<pgc_graph> g(new pgc_graph());
+  g->ctx = ctx;
+  g->E = n_edges;
+  g->npl = nodes_per_layer;
+  g->ps_cols = ps_cols;
+  g->ncols = n_cols;
+  hipStream_t st = ctx->stream;
+  HIPCHK(dalloc(&g->d_rows, n_edges * ps_cols));
+  HIPCHK(dalloc(&g->d_src, n_edges));
+  HIPCHK(dalloc(&g->d_trgt, n_edges));
+  HIPCHK(dalloc(&g->d_energy, n_edges * n_cols));
+  if (productspace)
+    HIPCHK(hipMemcpyAsync(g->d_rows, productspace, n_edges * ps_cols * 4, hipMemcpyHostToDevice, st));
+  else
+    HIPCHK(hipMemsetAsync(g->d_rows, 0xff, n_edges * ps_cols * 4, st));
+  HIPCHK(hipMemcpyAsync(g->d_src, src, n_edges * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g->d_trgt, trgt, n_edges * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(g->d_energy, cost, n_edges * n_cols * 8, hipMemcpyHostToDevice, st));
+  int rc = finish_graph_meta(g.get());
+  if (rc) return rc;
+  *out = g.release();
+  return PGC_OK;
+}
+
+int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const int32_t* EX, int64_t nEX,
+                       const double* VY, int64_t nVY, const int32_t* EY, int64_t nEY, const double* feat_diff,
+                       int conjugate, int regularising, int prune, const char* cost_mode, const char* time_mode,
+                       const double* NX, const double* NY, double rl_alpha, double rl_c, double rl_pwr,
+                       pgc_graph** out) {
+  (void)NX;
+  (void)NY;
+  (void)rl_alpha;
+  (void)rl_c;
+  (void)rl_pwr;
+  if (!ctx || !out || !VX || !EX || !VY || !EY || !feat_diff || nVX <= 0 || nEX <= 0 || nVY <= 0 || nEY <= 0)
+    return fail(PGC_ERR_ARG, "invalid arguments");
+  if (conjugate) return fail(PGC_ERR_UNSUPPORTED, "conjugate product graph is not available in this build");
+  int32_t mx = -1;
+  for (int64_t i = 0; i < 2 * nEX; ++i) {
+    if (EX[i] < 0) return fail(PGC_ERR_ARG, "EX holds a negative vertex id");
+    mx = std::max(mx, EX[i]);
+  }
+  const int64_t nVXe = (int64_t)mx + 1;  // product_space.cpp:20
+  if (nVXe != nVX)
+    return fail(PGC_ERR_ARG, "max(EX)+1 differs from |VX|: the product space and the solver would disagree");
+  for (int64_t i = 0; i < 2 * nEY; ++i)
+    if (EY[i] < 0 || EY[i] >= nVY) return fail(PGC_ERR_ARG, "EY holds an id outside [0, |VY|)");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::unique_ptr<pgc_graph> g(new pgc_graph());
+  g->ctx = ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t E = nEY * ((prune ? 1 : 2) * nEX + nVXe);
+  g->E = E;
+  g->npl = nVX;
+  g->ps_cols = 4;
+  const bool lengthMode = cost_mode && time_mode && std::strcmp(cost_mode, "feature") == 0 &&
+                          std::strcmp(time_mode, "lengthnormalisation") == 0;
+  g->ncols = lengthMode ? 2 : 1;
+  int32_t *dEX = nullptr, *dEY = nullptr;
+  double *dVX = nullptr, *dVY = nullptr, *dFD = nullptr;
+  HIPCHK(dalloc(&dEX, 2 * nEX));
+  HIPCHK(dalloc(&dEY, 2 * nEY));
+  HIPCHK(dalloc(&dVX, 3 * nVX));
+  HIPCHK(dalloc(&dVY, 3 * nVY));
+  HIPCHK(dalloc(&dFD, nVX * nVY));
+  HIPCHK(dalloc(&g->d_rows, E * 4));
+  HIPCHK(dalloc(&g->d_src, E));
+  HIPCHK(dalloc(&g->d_trgt, E));
+  HIPCHK(dalloc(&g->d_energy, E * g->ncols));
+  HIPCHK(hipMemcpyAsync(dEX, EX, 2 * nEX * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dEY, EY, 2 * nEY * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dVX, VX, 3 * nVX * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dVY, VY, 3 * nVY * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dFD, feat_diff, nVX * nVY * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(launch_product_space(dEX, nEX, dEY, nEY, nVXe, prune ? 1 : 0, g->d_rows, g->d_src, g->d_trgt, E, st));
+  HIPCHK(launch_energy(dVX, dVY, g->d_rows, E, dFD, nVY, (regularising || lengthMode) ? 1 : 0, lengthMode ? 1 : 0,
+                       g->d_energy, st));
+  HIPCHK(hipStreamSynchronize(st));
+  (void)hipFree(dEX);
+  (void)hipFree(dEY);
+  (void)hipFree(dVX);
+  (void)hipFree(dVY);
+  (void)hipFree(dFD);
+  // numNodes / numLayers as the solver derives them
+  g->numNodes = nEY * nVXe;
+  g->L = nEY;
+  if (prune) {
+    // branchGraph[x] = out-neighbour vertex ids (product_space.cpp:122-131)
+    std::vector<std::vector<int64_t>> bg(nVX);
+    for (int64_t j = 0; j < nEX; ++j) bg[EX[2 * j]].push_back(EX[2 * j + 1]);
+    g->bg_off.assign(nVX + 1, 0);
+    g->bg_val.clear();
+    for (int64_t v = 0; v < nVX; ++v) {
+      std::sort(bg[v].begin(), bg[v].end());
+      bg[v].erase(std::unique(bg[v].begin(), bg[v].end()), bg[v].end());
+      for (int64_t x : bg[v]) g->bg_val.push_back(x);
+      g->bg_off[v + 1] = (int64_t)g->bg_val.size();
+    }
+    g->has_bg = true;
+  }
+  *out = g.release();
+  return PGC_OK;
+}
+
+int pgc_graph_set_branch_graph(pgc_graph* g, const int64_t* offsets, const int64_t* values) {
+  if (!g || !offsets) return fail(PGC_ERR_ARG, "invalid arguments");
+  g->bg_off.assign(offsets, offsets + g->npl + 1);
+  g->bg_val.assign(values, values + offsets[g->npl]);
+  g->has_bg = true;
+  return PGC_OK;
+}
+
+int pgc_graph_info(const pgc_graph* g, int64_t* n_edges, int32_t* ps_cols, int32_t* n_cost_cols,
+                   int64_t* nodes_per_layer, int64_t* n_layers) {
+  if (!g) return fail(PGC_ERR_ARG, "graph is null");
+  if (n_edges) *n_edges = g->E;
+  if (ps_cols) *ps_cols = g->ps_cols;
+  if (n_cost_cols) *n_cost_cols = g->ncols;
+  if (nodes_per_layer) *nodes_per_layer = g->npl;
+  if (n_layers) *n_layers = g->L;
+  return PGC_OK;
+}
+
+int pgc_graph_get_product_space(pgc_graph* g, int32_t* out) {
+  if (!g || !out) return fail(PGC_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(g->ctx->device));
+  HIPCHK(hipMemcpyAsync(out, g->d_rows, g->E * g->ps_cols * 4, hipMemcpyDeviceToHost, g->ctx->stream));
+  HIPCHK(hipStreamSynchronize(g->ctx->stream));
+  return PGC_OK;
+}
+
+int pgc_graph_get_cost(pgc_graph* g, double* out) {
+  if (!g || !out) return fail(PGC_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(g->ctx->device));
+  HIPCHK(hipMemcpyAsync(out, g->d_energy, g->E * g->ncols * 8, hipMemcpyDeviceToHost, g->ctx->stream));
+  HIPCHK(hipStreamSynchronize(g->ctx->stream));
+  return PGC_OK;
+}
+
+int pgc_graph_set_cost(pgc_graph* g, const double* cost) {
+  if (!g || !cost) return fail(PGC_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(g->ctx->device));
+  HIPCHK(hipMemcpyAsync(g->d_energy, cost, g->E * g->ncols * 8, hipMemcpyHostToDevice, g->ctx->stream));
+  HIPCHK(hipStreamSynchronize(g->ctx->stream));
+  g->cost_kind = 0;
+  return PGC_OK;
+}
+
+void pgc_graph_destroy(pgc_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->ctx->device);
+  delete g;
+}
+
+// ------------------------------------------------------------ Lawler + BnB
+int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
+                    int64_t capacity, pgc_result* res) {
+  Timer wall;
+  if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
+  if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
+  int rc = check_solvable(g, K);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t npl = g->npl;
+  // robustFloatToIntConversion of every column, in place (:875-882)
+  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
+  if ((rc = build_layout(g))) return rc;
+  if ((rc = count_valid_slots(g))) return rc;
+  if ((rc = fill_costs(g, 1))) return rc;
+  if ((rc = ensure_scratch(g, K))) return rc;
+  JobScratch& s = g->js;
+  HIPCHK(hipMemsetAsync(s.pred, 0xff, s.predN * 4, st));
+  HIPCHK(hipMemsetAsync(s.neg_len, 0, 4, st));
+  double kms = 0;
+  int64_t sweeps = 0, bfc = 0, its = 0, branches = 0;
+
+  // initial upper bound (findInitalUpperBound)
+  SolverJob job = make_job(g, K, MODE_UPPER_BOUND, flags);
+  JobResult r;
+  if ((rc = run_job(g, job, false, r, kms))) return rc;
+  if (r.error) return fail(PGC_ERR_SOLVER, "upper-bound search failed (error " + std::to_string(r.error) + ")");
+  sweeps += r.sweeps;
+  bfc += r.bf_calls;
+  const double ratioUpperBound = r.ratio_ub;
+
+  // branch and bound over layer-0 vertex partitions (:794-851)
+  std::vector<int32_t> branch(npl, 0);
+  int maxBranchId = 0;
+  BnbHeap heap(npl);
+  heap.push(0, 0);
+  std::vector<char> valid(npl, 0);
+  struct Cyc {
+    std::vector<int32_t> eid;
+    std::vector<double> c, t;
+  };
+  std::map<int, Cyc> validCycles;
+  Cyc finalCyc;
+  bool fromValid = false;
+  bool branchDirty = true;
+  std::vector<int32_t> hits;
+  job.mode = MODE_RATIO;
+  while (!heap.empty()) {
+    const auto top = heap.pop();
+    const int bid = (int)top.second;
+    if (valid[bid]) {
+      finalCyc = validCycles.at(bid);
+      fromValid = true;
+      break;
+    }
+    branches++;
+    if (branchDirty) {
+      HIPCHK(hipMemcpyAsync(s.branch, branch.data(), npl * 4, hipMemcpyHostToDevice, st));
+      branchDirty = false;
+    }
+    job.branchId = bid;
+    job.ratio_in = ratioUpperBound;
+    if ((rc = run_job(g, job, false, r, kms))) return rc;
+    if (r.error) return fail(PGC_ERR_SOLVER, "Lawler subroutine failed (error " + std::to_string(r.error) + ")");
+    sweeps += r.sweeps;
+    bfc += r.bf_calls;
+    its += r.lawler_iters;
+    const bool wraps = r.num_wraps > 1;
+    const double lbBranch = r.ratio;
+    if (ratioUpperBound < heap.peak() && !wraps) break;
+    if (!wraps) {
+      heap.push(lbBranch, bid);
+      valid[bid] = 1;
+      Cyc cy;
+      const int m = std::max(0, r.neg_len - 1);
+      cy.eid.resize(m);
+      cy.c.resize(m);
+      cy.t.resize(m);
+      if (m) {
+        HIPCHK(hipMemcpyAsync(cy.eid.data(), s.neg_eid, m * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(cy.c.data(), s.neg_c, m * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(cy.t.data(), s.neg_t, m * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+      }
+      validCycles.emplace(bid, std::move(cy));
+      continue;
+    }
+    if (r.num_in_branch == 1) continue;
+    // branch(): peel zero-layer hits 2.. into their own branches (:142-179)
+    hits.resize(std::min<int64_t>(r.n_hits, npl));
+    if (!hits.empty()) {
+      HIPCHK(hipMemcpyAsync(hits.data(), s.hits, hits.size() * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+    int offset = 0;
+    for (int32_t h : hits) {
+      if (offset > 0 && h >= 0 && h < npl) branch[h] = maxBranchId + offset;
+      offset++;
+    }
+    for (int64_t v = 0; v < npl; ++v)
+      if (branch[v] == bid) branch[v] = maxBranchId;
+    maxBranchId += offset;
+    branchDirty = true;
+    for (int32_t h : hits)
+      if (h >= 0 && h < npl) heap.push(lbBranch, branch[h]);
+  }
+  if (!fromValid) {
+    const int m = std::max(0, (int)0);
+    (void)m;
+    int32_t nl = 0;
+    HIPCHK(hipMemcpyAsync(&nl, s.neg_len, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int mm = std::max(0, nl - 1);
+    finalCyc.eid.resize(mm);
+    finalCyc.c.resize(mm);
+    finalCyc.t.resize(mm);
+    if (mm) {
+      HIPCHK(hipMemcpyAsync(finalCyc.eid.data(), s.neg_eid, mm * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(finalCyc.c.data(), s.neg_c, mm * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(finalCyc.t.data(), s.neg_t, mm * 8, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+  // output (:857-862): ratio of the cycle and its product-space rows
+  double cs = 0, ts = 0;
+  for (size_t i = 0; i < finalCyc.c.size(); ++i) {
+    cs += finalCyc.c[i];
+    ts += finalCyc.t[i];
+  }
+  const int64_t n = (int64_t)finalCyc.eid.size();
+  if (n > capacity) return fail(PGC_ERR_CAPACITY, "output capacity too small: need " + std::to_string(n));
+  if (edge_ids) std::copy(finalCyc.eid.begin(), finalCyc.eid.end(), edge_ids);
+  if (rows && n) {
+    HIPCHK(hipMemcpyAsync(s.eids_io, finalCyc.eid.data(), n * 4, hipMemcpyHostToDevice, st));
+    int32_t* d_out = nullptr;
+    HIPCHK(dalloc(&d_out, n * g->ps_cols));
+    HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, s.eids_io, n, d_out, st));
+    HIPCHK(hipMemcpyAsync(rows, d_out, n * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(d_out);
+  }
+  res->passes_layers_once = 1;
+  res->objective = (float)(cs / ts);
+  res->n_edges = n;
+  res->sweeps = sweeps;
+  res->bf_calls = bfc;
+  res->lawler_iters = its;
+  res->branches = branches;
+  res->relaxations = relax_per_sweep(g, K) * (double)sweeps;
+  res->kernel_ms = kms;
+  res->total_ms = wall.ms();
+  return PGC_OK;
+}
+
+// ---------------------------------------------------------- min-cost + BnB
+int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
+                       int64_t capacity, pgc_result* res) {
+  Timer wall;
+  if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
+  if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
+  int rc = check_solvable(g, K);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t npl = g->npl, L = g->L;
+  const bool cpuRules = (flags & PGC_MINCOST_CPU_RULES) != 0;
+  if ((rc = build_layout(g))) return rc;
+  if ((rc = count_valid_slots(g))) return rc;
+  if ((rc = fill_costs(g, 2))) return rc;
+  if ((rc = ensure_scratch(g, K))) return rc;
+  JobScratch& s = g->js;
+  HIPCHK(hipMemsetAsync(s.pred, 0xff, s.predN * 4, st));
+  // branch graph: the given one, else the in-edges of layer-0 nodes (ascending edge id)
+  if (!g->has_bg) {
+    const int nC = g->nChunks;
+    std::vector<int32_t> off(nC + 1), sl, el;
+    HIPCHK(hipMemcpy(off.data(), g->d_off[0], (nC + 1) * 4, hipMemcpyDeviceToHost));
+    const int64_t n0 = off[nC];
+    sl.resize(n0);
+    el.resize(n0);
+    if (n0) {
+      HIPCHK(hipMemcpy(sl.data(), g->d_s[0], n0 * 4, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(el.data(), g->d_e[0], n0 * 4, hipMemcpyDeviceToHost));
+    }
+    g->bg_off.assign(npl + 1, 0);
+    g->bg_val.clear();
+    for (int64_t v = 0; v < npl; ++v) {
+      const int c = (int)(v >> 6), lane = (int)(v & 63);
+      const int w = (off[c + 1] - off[c]) / 64;
+      std::vector<int64_t> ids;
+      for (int j = 0; j < w; ++j) {
+        const int64_t idx = off[c] + (int64_t)j * 64 + lane;
+        if (sl[idx] < 0) break;
+        ids.push_back(el[idx]);
+      }
+      std::sort(ids.begin(), ids.end());
+      g->bg_val.insert(g->bg_val.end(), ids.begin(), ids.end());
+      g->bg_off[v + 1] = (int64_t)g->bg_val.size();
+    }
+    g->has_bg = true;
+    g->bg_val.push_back(0);
+    g->bg_val.pop_back();
+  }
+  // SRC ids of branch-graph values (the BFS reads SRCIds(value))
+  std::vector<int32_t> bgSrc(g->bg_val.size(), -1);
+  if (!g->bg_val.empty()) {
+    std::vector<int32_t> ids(g->bg_val.size());
+    for (size_t i = 0; i < ids.size(); ++i)
+      ids[i] = (g->bg_val[i] >= 0 && g->bg_val[i] < g->E) ? (int32_t)g->bg_val[i] : 0;
+    int32_t *d_ids = nullptr, *d_out = nullptr;
+    HIPCHK(dalloc(&d_ids, ids.size()));
+    HIPCHK(dalloc(&d_out, ids.size()));
+    HIPCHK(hipMemcpyAsync(d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_gather_rows(g->d_src, 1, d_ids, (int64_t)ids.size(), d_out, st));
+    HIPCHK(hipMemcpyAsync(bgSrc.data(), d_out, ids.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(d_ids);
+    (void)hipFree(d_out);
+    for (size_t i = 0; i < ids.size(); ++i)
+      if (!(g->bg_val[i] >= 0 && g->bg_val[i] < g->E)) bgSrc[i] = -1;
+  }
+
+  double kms = 0;
+  int64_t sweeps = 0, bfc = 0, branches = 0;
+  const double INF = std::numeric_limits<double>::infinity();
+  double upperBound = INF;
+  std::vector<int32_t> branch(npl, 0);
+  BnbHeap heap(npl);
+  heap.push(0.0, 0);
+  int maxBranchId = 0;
+  std::vector<char> valid(npl, 0);
+  std::map<int, std::vector<int32_t>> validCycles;
+  std::vector<int32_t> cycle;  // best path, layer L-1 ... source (original ids)
+  std::vector<double> keys(npl);
+  std::vector<int32_t> back(npl), clen(npl);
+  std::vector<std::pair<double, int64_t>> lbs;
+  SolverJob job = make_job(g, K, MODE_MIN_COST, flags);
+  bool branchDirty = true;
+  while (!heap.empty()) {
+    const auto top = heap.pop();
+    const int bid = (int)top.second;
+    if (valid[bid]) {
+      upperBound = top.first;
+      cycle = validCycles.at(bid);
+      break;
+    }
+    branches++;
+    if (branchDirty) {
+      HIPCHK(hipMemcpyAsync(s.branch, branch.data(), npl * 4, hipMemcpyHostToDevice, st));
+      branchDirty = false;
+    }
+    job.branchId = bid;
+    job.upper_bound = upperBound;
+    JobResult r;
+    if ((rc = run_job(g, job, true, r, kms))) return rc;
+    if (r.error) return fail(PGC_ERR_SOLVER, "min-cost sweep failed (error " + std::to_string(r.error) + ")");
+    sweeps += r.sweeps;
+    bfc += r.bf_calls;
+    if (r.status != ST_CAN_CONTAIN) continue;
+    HIPCHK(hipMemcpyAsync(keys.data(), s.lb_key, npl * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(back.data(), s.back, npl * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(clen.data(), s.wlen, npl * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // lower bounds in ascending (key, vertex) order (dijkstra_bnb_subroutine.cu:391-393)
+    lbs.clear();
+    bool anyFinite = false;
+    for (int64_t v = 0; v < npl; ++v) {
+      if (branch[v] != bid) continue;
+      lbs.push_back({keys[v], v});
+      if (!std::isinf(keys[v])) anyFinite = true;
+    }
+    if (!anyFinite) continue;
+    std::sort(lbs.begin(), lbs.end());
+    bool first = true, foundFirst = false;
+    double lbCur = 0;
+    int bSrc = -1, bTrgt = -1;
+    bool pruned = false;
+    for (auto& lb : lbs) {
+      const float lowerBound = (float)lb.first;
+      if (std::isinf(lowerBound)) {
+        first = false;
+        continue;
+      }
+      const int64_t v = lb.second;
+      if (first) lbCur = lowerBound;
+      if (v == back[v]) {
+        if (first) foundFirst = true;
+        const float newUB = lowerBound;
+        if (newUB > upperBound && first) {
+          pruned = true;
+          break;
+        }
+        if (newUB < upperBound) {
+          upperBound = newUB;
+          cycle.resize(clen[v]);
+          if (clen[v])
+            HIPCHK(hipMemcpy(cycle.data(), s.chain + v * (L * K + 2), clen[v] * 4, hipMemcpyDeviceToHost));
+        }
+        break;
+      } else if (first) {
+        if (cpuRules) {
+          int32_t front = -1;
+          if (clen[v] > 0) HIPCHK(hipMemcpy(&front, s.chain + v * (L * K + 2), 4, hipMemcpyDeviceToHost));
+          bSrc = clen[v] > 0 ? (int)(front - g->numNodes) : -1;
+        } else {
+          bSrc = (int)v;
+        }
+        bTrgt = back[v];
+      }
+      first = false;
+    }
+    if (pruned) continue;
+    if (upperBound < heap.peak() && foundFirst) break;
+    if (foundFirst) {
+      heap.push(lbCur, bid);
+      valid[bid] = 1;
+      validCycles.emplace(bid, cycle);
+      continue;
+    }
+    // branch(): split by a multi-source unit-weight BFS (dijkstra_solver_cuda.cpp:30-117)
+    maxBranchId++;
+    {
+      BnbHeap bh(npl);
+      std::vector<double> tbl(npl, -1);
+      for (int64_t i = 0; i < npl; ++i) {
+        tbl[i] = -1;
+        if (i == bSrc || i == bTrgt) {
+          tbl[i] = 1;
+          continue;
+        }
+        if (branch[i] == bid) {
+          bh.push(INF, i);
+          tbl[i] = -1;
+        }
+      }
+      bh.push(0.0, bSrc);
+      bh.push(0.0, bTrgt);
+      if (bTrgt >= 0 && bTrgt < npl) branch[bTrgt] = maxBranchId;
+      int counter = 0;
+      int numInBranch = r.num_in_branch;
+      for (auto& lb : lbs) {
+        const int64_t v = lb.second;
+        if (v == bSrc || v == bTrgt) continue;
+        if (branch[v] != bid) continue;
+        if (counter > std::max(1, numInBranch / 4)) {
+          bh.push(0.1, v);
+          branch[v] = maxBranchId;
+        }
+        counter++;
+      }
+      while (!bh.empty()) {
+        const auto cur = bh.pop();
+        const int64_t node = cur.second;
+        const float curCost = (float)cur.first;
+        tbl[node] = 1.0;
+        for (int64_t p = g->bg_off[node]; p < g->bg_off[node + 1]; ++p) {
+          const int64_t e = g->bg_val[p];
+          if (e == -1) continue;
+          const int32_t sId = bgSrc[p];
+          if (sId < 0 || sId >= npl) continue;
+          if (tbl[sId] < 0) {
+            const float oldV = (float)bh.key_value(sId);
+            const float newV = curCost + 1.0;
+            if (newV < oldV) {
+              bh.decrease(sId, newV);
+              branch[sId] = branch[node];
+            }
+          }
+        }
+      }
+    }
+    branchDirty = true;
+    heap.push(lbCur, bid);
+    heap.push(lbCur, maxBranchId);
+  }
+  // path output (dijkstra_solver_cuda.cpp:261-305)
+  const int64_t cycleLength = (int64_t)cycle.size();
+  std::vector<int32_t> cyc(cycle.rbegin(), cycle.rend());
+  if (!cyc.empty()) cyc.push_back(cyc.front() + (int32_t)g->numNodes);
+  std::vector<int32_t> eids(cyc.size() > 1 ? cyc.size() - 1 : 0, -2);
+  int loops = 0;
+  if (cyc.size() > 1) {
+    if ((int64_t)cyc.size() > s.cyc_cap) return fail(PGC_ERR_CAPACITY, "path longer than the cycle buffer");
+    SolverJob lj = make_job(g, K, MODE_MIN_COST, flags);
+    HIPCHK(hipMemcpyAsync(s.djob, &lj, sizeof(lj), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(s.nodes_io, cyc.data(), cyc.size() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(launch_lookup(s.djob, s.nodes_io, (int)cyc.size(), s.eids_io, false, true, st));
+    HIPCHK(hipMemcpyAsync(eids.data(), s.eids_io, eids.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  std::vector<int32_t> outE;
+  const int64_t loopmax = cycleLength + 1;
+  for (int64_t i = 0; i < loopmax && !cyc.empty(); ++i) {
+    const int64_t a = cyc[i % loopmax], b = cyc[(i + 1) % loopmax];
+    if (b >= g->numNodes) loops++;
+    if (b == a - g->numNodes) continue;  // closing step through the dummy sentinel
+    if (i + 1 >= (int64_t)cyc.size()) continue;
+    const int32_t e = eids[i];
+    if (e < 0) continue;
+    outE.push_back(e);
+  }
+  const int64_t n = (int64_t)outE.size();
+  if (n > capacity) return fail(PGC_ERR_CAPACITY, "output capacity too small: need " + std::to_string(n));
+  if (edge_ids) std::copy(outE.begin(), outE.end(), edge_ids);
+  if (rows && n) {
+    HIPCHK(hipMemcpyAsync(s.eids_io, outE.data(), n * 4, hipMemcpyHostToDevice, st));
+    int32_t* d_out = nullptr;
+    HIPCHK(dalloc(&d_out, n * g->ps_cols));
+    HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, s.eids_io, n, d_out, st));
+    HIPCHK(hipMemcpyAsync(rows, d_out, n * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(d_out);
+  }
+  res->passes_layers_once = loops < 2 ? 1 : 0;
+  res->objective = (float)upperBound;
+  res->n_edges = n;
+  res->sweeps = sweeps;
+  res->bf_calls = bfc;
+  res->lawler_iters = 0;
+  res->branches = branches;
+  res->relaxations = relax_per_sweep(g, K) * (double)sweeps;
+  res->kernel_ms = kms;
+  res->total_ms = wall.ms();
+  return PGC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Internal host/device structures of the MI355X product-graph cycle solvers.
+//
+// Device layout of a layered product graph (one "job" = one graph):
+//   * node (layer l, vertex v) has id l*npl + v; layer L (= numNodes/npl) is
+//     the copy of layer 0 that closes the cylinder (the reference's dummy
+//     nodes, cost_time_ratio_solver.cpp:42-66);
+//   * the in-edges of every node are split into two classes, intralayer
+//     (source on the same layer) and interlayer (source on the previous
+//     layer), and stored per layer as sliced ELL with 64-vertex slices (one
+//     wave per slice): slice c of layer l occupies slots
+//     [off[l*nChunks+c], off[l*nChunks+c+1]) laid out slot-major, so slot j
+//     of lane i sits at off + 64*j + i and a wave's loads are contiguous;
+//   * inside a vertex the in-edges are ordered by (source id, edge id), the
+//     documented deterministic replacement for the reference's hash-set
+//     iteration order; padding slots carry src = -1.
+#pragma once
+#include <stdint.h>
+
+#define PGC_WAVE 64
+#define PGC_BLOCK 1024
+
+namespace pgc {
+
+struct SlotClass {
+  const int32_t* off;  // [(L+1)*nChunks + 1]
+  const int32_t* src;  // source vertex index inside its layer, -1 = padding
+  const void* cost;    // float (q-units, exact) or double (raw)
+  const float* time;   // nullptr: constant time
+  const int32_t* eid;  // product-graph edge id
+};
+
+enum JobMode : int32_t { MODE_RATIO = 0, MODE_UPPER_BOUND = 1, MODE_MIN_COST = 2 };
+
+enum Status : int32_t {
+  ST_CAN_CONTAIN = 0,
+  ST_CANNOT_CONTAIN = 1,
+  ST_NO_NEGATIVE_CYCLE = 2,
+  ST_FOUND_NEGATIVE_CYCLE = 3,
+  ST_ZERO_CYCLE = 4,
+};
+
+enum JobError : int32_t {
+  ERR_NONE = 0,
+  ERR_EMPTY_BRANCH = 1,
+  ERR_CYCLE_TOO_LONG = 2,
+  ERR_EDGE_LOOKUP = 3,
+  ERR_CHAIN = 4,
+  ERR_SWEEP_CAP = 5,
+};
+
+// flags
+enum : uint32_t {
+  FLAG_PICK_LAST_VERTEX = 1u,  // candidate pick of the reference's GPU path (last reporting vertex)
+  FLAG_MINCOST_CPU_RULES = 2u, // min-cost: double lower-bound keys
+};
+
+struct JobResult {
+  int32_t status;
+  int32_t error;
+  double ratio;         // lambda at the end of the Lawler loop (lowerBoundOfCurrentBranch)
+  double ratio_ub;      // upper-bound mode: lambda_UB (float-rounded)
+  int32_t num_wraps;
+  int32_t num_in_branch;
+  int32_t n_hits;
+  int32_t neg_len;      // nodes in the current cycle
+  int32_t best_v;       // upper-bound mode: start vertex
+  int32_t pad;
+  int64_t sweeps;
+  int64_t bf_calls;
+  int64_t lawler_iters;
+  int64_t relax;        // candidate relaxations (finite source value)
+};
+
+struct SolverJob {
+  int32_t npl, L, K, nChunks;
+  int64_t numNodes;
+  int32_t mode;
+  uint32_t flags;
+  int32_t branchId;
+  int32_t chain_stride;  // L*K + 2
+  int32_t cyc_cap;       // capacity of cycle buffers (nodes)
+  int32_t pad0;
+  double ratio_in;
+  double upper_bound;
+  double time_const;
+  double tol;
+  int64_t max_sweeps;     // safety cap on sweeps per launch (error when reached)
+  SlotClass intra, inter;
+  const int32_t* branch;  // [npl] branch id per layer-0 vertex
+  int32_t* pred;          // [(L+1)*K*npl] predecessor (expanded node id)
+  // per-sweep scratch
+  int32_t* wnext;         // [npl]
+  int32_t* wlen;          // [npl]
+  int32_t* chain;         // [npl*chain_stride] original node ids along each wrap
+  int32_t* tmp_nodes;     // [cyc_cap]
+  int32_t* tmp_eid;       // [cyc_cap]
+  double* tmp_c;          // [cyc_cap]
+  double* tmp_t;          // [cyc_cap]
+  // persistent current cycle (the reference's negativeCycle)
+  int32_t* neg_nodes;     // [cyc_cap]
+  int32_t* neg_eid;       // [cyc_cap]
+  double* neg_c;          // [cyc_cap]
+  double* neg_t;          // [cyc_cap]
+  int32_t* neg_len;       // [1]
+  int32_t* hits;          // [npl] zero-layer hits
+  // min-cost outputs
+  double* lb_key;         // [npl]
+  int32_t* back;          // [npl] start vertex of the path ending at (L, v)
+  JobResult* res;
+};
+
+}  // namespace pgc

@@ -1,0 +1,229 @@
+"""MI355X-native product-graph cycle solvers with the reference's Python API.
+
+Mirrors the pybind11 module `cyclic_product_graphs` of the reference
+(pythonbinding/bindings.cpp:22-50) and the state semantics of
+ProductGraphGenerators (product_graph_generators.cpp:93-333):
+
+  * the graph is built lazily on the first solve / getter and then cached, so
+    set_cost_time_ratio_mode after that has no effect (:289-291, :309-312);
+  * a Lawler solve quantizes the generator's energy in place to q-units
+    (cost_time_ratio_solver.cpp:875-882); later get_cost_vector / min-cost
+    solves see the quantized values;
+  * max_depth defaults to 4 (:109);
+  * "export" writes VX/VY/EX/EY/FeatDiffMatrix(/NX/NY).csv to the working
+    directory (:272-282).
+
+Solvers run on the GPU through libpgc.so (C-ABI, include/pgc.h):
+  lawlergpu   - Lawler ratio cycle, candidate pick of the reference's GPU path
+  lawlercpu   - same device solver with the reference CPU path's candidate order
+  dijkstragpu - minimum-cost once-wrapping cycle, rules of the GPU path
+  dijkstracpu - same device solver with the CPU path's rules
+An unknown Lawler name raises ValueError (the reference exits the process);
+an unknown min-cost name falls back to dijkstracpu rules, as the reference does
+(dijkstra_solver.cpp:576-578).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import PgcError  # noqa: F401
+
+__all__ = ["product_graph_generator", "compute_normals_3d", "compute_normals_2d", "compute_local_thickness_3d",
+           "compute_local_thickness_2d", "PgcError"]
+
+_LAWLER = {"lawlergpu": _lib.PGC_LAWLER_PICK_LAST, "lawlercpu": 0}
+_MINCOST = {"dijkstragpu": 0, "dijkstracpu": _lib.PGC_MINCOST_CPU_RULES}
+
+
+def _write_csv(mat, name):
+    # Eigen::IOFormat(FullPrecision, DontAlignCols, ", ", "\n") (helper/utils.hpp:24-32)
+    mat = np.asarray(mat)
+    with open(name + ".csv", "w") as f:
+        lines = []
+        for row in np.atleast_2d(mat):
+            if np.issubdtype(mat.dtype, np.integer):
+                lines.append(", ".join(str(int(x)) for x in row))
+            else:  # std::ostream with precision max_digits10 (17), general format
+                lines.append(", ".join("%.17g" % float(x) for x in row))
+        f.write("\n".join(lines))
+
+
+class product_graph_generator:
+    """product_graph_generator(VX, EX, VY, EY, feat_diff, use_conjugate_graph,
+    use_regularising_cost_term[, prune_intralayer_edges])"""
+
+    def __init__(self, VX, EX, VY, EY, feat_diff, use_conjugate_graph, use_regularising_cost_term,
+                 prune_intralayer_edges=False, device=0):
+        self.VX = np.array(VX, dtype=np.float64)
+        self.EX = np.array(EX, dtype=np.int32)
+        self.VY = np.array(VY, dtype=np.float64)
+        self.EY = np.array(EY, dtype=np.int32)
+        self.feat_diff = np.array(feat_diff, dtype=np.float64)
+        self.conjugate = bool(use_conjugate_graph)
+        self.regularising = bool(use_regularising_cost_term)
+        self.prune = bool(prune_intralayer_edges)
+        self.rl = (0.7, 0.6, 4.0)
+        self.max_depth = 4
+        self.cost_name = "vanilla"
+        self.time_name = ""
+        self.NX = None
+        self.NY = None
+        self.device = device
+        self._graph = None
+        self.last_result = None
+        if self.VX.ndim != 2 or self.VX.shape[1] != 3:
+            raise ValueError("VX must be |VX| x 3")
+        if self.VY.ndim != 2 or self.VY.shape[1] != 3:
+            raise ValueError("VY must be |VY| x 3 (z = 0 for planar curves)")
+        if self.EX.ndim != 2 or self.EX.shape[1] != 2 or self.EY.ndim != 2 or self.EY.shape[1] != 2:
+            raise ValueError("EX and EY must have two columns")
+        if self.feat_diff.shape != (len(self.VX), len(self.VY)):
+            raise ValueError("feat_diff must be |VX| x |VY|")
+
+    # ------------------------------------------------------------- setters
+    def set_normals(self, NX, NY):
+        NX = np.array(NX, dtype=np.float64)
+        NY = np.array(NY, dtype=np.float64)
+        if NX.shape != (len(self.VX), 3):
+            print("[ShapeMM] error: normals for shape X do not contain as many entries as expected. "
+                  "Assumed shape = |VX| x 3")
+            return
+        if NY.shape != (len(self.VY), 3):
+            print("[ShapeMM] error: normals for shape Y do not contain as many entries as expected. "
+                  "Assumed shape = |VY| x 3")
+            return
+        self.NX, self.NY = NX, NY
+
+    def set_max_depth_cost_time_ratio(self, max_depth):
+        self.max_depth = int(max_depth)
+
+    def set_cost_time_ratio_mode(self, cost_mode, time_mode):
+        self.cost_name = str(cost_mode)
+        self.time_name = str(time_mode)
+
+    def update_robust_loss_params(self, alpha, c, pwr):
+        if not self.conjugate:
+            return
+        if self._graph is not None:
+            print("[ShapeMM] error: cannot update robust loss params after model has been generated")
+            return
+        self.rl = (float(alpha), float(c), float(pwr))
+
+    # ------------------------------------------------------------- graph
+    def _ctx(self):
+        return _lib.default_context(self.device)
+
+    def generate(self):
+        if self.conjugate and (self.NX is None or self.NY is None):
+            raise ValueError("conjugate product graph needs normals: call set_normals(NX, NY) first")
+        self._graph = _lib.Graph.generate(self._ctx(), self.VX, self.EX, self.VY, self.EY, self.feat_diff,
+                                          self.conjugate, self.regularising, self.prune, self.cost_name,
+                                          self.time_name, self.NX, self.NY, self.rl)
+
+    def _ensure(self):
+        if self._graph is None:
+            self.generate()
+        return self._graph
+
+    def get_cost_vector(self):
+        return self._ensure().cost()
+
+    def get_product_space(self):
+        return self._ensure().product_space()
+
+    def _export(self):
+        _write_csv(self.VX, "VX")
+        _write_csv(self.VY, "VY")
+        _write_csv(self.EX, "EX")
+        _write_csv(self.EY, "EY")
+        _write_csv(self.feat_diff, "FeatDiffMatrix")
+        if self.conjugate and self.NX is not None:
+            _write_csv(self.NX, "NX")
+            _write_csv(self.NY, "NY")
+        return False, -1.0, np.zeros((4, 1), np.int32)
+
+    # ------------------------------------------------------------- solvers
+    def solve_with_cost_time_ratio(self, solvername, max_depth=None):
+        if solvername == "export":
+            return self._export()
+        if solvername not in _LAWLER:
+            raise ValueError("Solver >>%s<< is not implemented" % solvername)
+        g = self._ensure()
+        K = self.max_depth if max_depth is None else int(max_depth)
+        _, rows, res = g.solve_ratio(K, _LAWLER[solvername])
+        self.last_result = res.as_dict()
+        return bool(res.passes_layers_once), float(np.float32(res.objective)), rows
+
+    def solve_with_minimum_cost(self, solvername, max_depth=None):
+        if solvername == "export":
+            return self._export()
+        if solvername not in _MINCOST:
+            print("Solver >>%s<< is not implemented\nFalling back to >> dijkstracpu <<" % solvername)
+            flags = _MINCOST["dijkstracpu"]
+        else:
+            flags = _MINCOST[solvername]
+        g = self._ensure()
+        K = self.max_depth if max_depth is None else int(max_depth)
+        _, rows, res = g.solve_min_cost(K, flags)
+        self.last_result = res.as_dict()
+        return bool(res.passes_layers_once), float(np.float32(res.objective)), rows
+
+
+# ------------------------------------------------------------- helpers
+def compute_normals_3d(V, F):
+    """Area-weighted per-vertex normals (utils::normals3d, helper/utils.cpp:86-90)."""
+    V = np.asarray(V, np.float64)
+    F = np.asarray(F, np.int64)
+    fn = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    n = np.zeros_like(V)
+    for k in range(3):
+        np.add.at(n, F[:, k], fn)
+    nrm = np.linalg.norm(n, axis=1, keepdims=True)
+    nrm[nrm == 0] = 1
+    return n / nrm
+
+
+def compute_normals_2d(V, E):
+    """Outward vertex normals of a closed planar curve (utils::normals2d, helper/utils.cpp:93-157)."""
+    V = np.array(V, np.float64)
+    E = np.asarray(E, np.int64)
+    if len(V) != len(E):
+        print("error: V.rows() != E.rows()")
+        return np.zeros((1, 1))
+    zero = -1
+    for i in range(3):
+        if abs(V[:, i].sum()) < 1e-4:
+            zero = i
+            break
+    if zero == -1:
+        print("error: points dont lie on xy, xz or yz plane")
+        return np.zeros((1, 1))
+    if zero != 2:
+        V = V[:, [0, 2, 1]] if zero == 1 else V[:, [1, 2, 0]]
+    ev = V[E[:, 0]] - V[E[:, 1]]
+    el = np.linalg.norm(ev, axis=1)
+    R = np.array([[0, 1, 0], [-1, 0, 0], [0, 0, 1]], np.float64)
+    en = ev @ R.T
+    en = en / np.linalg.norm(en, axis=1, keepdims=True)
+    maxX, minX = np.float32(V[:, 0].max()), np.float32(V[:, 0].min())
+    maxY, minY = np.float32(V[:, 1].max()), np.float32(V[:, 1].min())
+    mult = np.float32(0.1 * min(abs(maxX - minX), abs(maxY - minY)))
+    if np.float32((V + float(mult) * en)[:, 0].max()) < maxX:
+        en = -en
+    n = len(V)
+    vn = np.zeros((n, 3))
+    for i in range(1, n + 1):
+        j = i % n
+        vn[j] = el[i - 1] * en[i - 1] + el[j] * en[j]
+    return vn / np.linalg.norm(vn, axis=1, keepdims=True)
+
+
+def compute_local_thickness_3d():
+    print("TODO: IMPL")
+
+
+def compute_local_thickness_2d():
+    print("TODO: IMPL")

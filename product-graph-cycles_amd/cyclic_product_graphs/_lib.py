@@ -1,0 +1,228 @@
+"""ctypes binding of libpgc.so (include/pgc.h).
+
+The library is built in-tree (product-graph-cycles_amd/Makefile). Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpgc.so")
+
+PGC_LAWLER_PICK_LAST = 1
+PGC_MINCOST_CPU_RULES = 2
+
+
+class PgcError(RuntimeError):
+    pass
+
+
+class pgc_result(C.Structure):
+    _fields_ = [
+        ("passes_layers_once", C.c_int32),
+        ("objective", C.c_float),
+        ("n_edges", C.c_int64),
+        ("sweeps", C.c_int64),
+        ("bf_calls", C.c_int64),
+        ("lawler_iters", C.c_int64),
+        ("branches", C.c_int64),
+        ("relaxations", C.c_double),
+        ("kernel_ms", C.c_double),
+        ("total_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTED = [
+    "pgc_last_error", "pgc_abi_version", "pgc_context_create", "pgc_context_destroy", "pgc_graph_generate",
+    "pgc_graph_from_arrays", "pgc_graph_set_branch_graph", "pgc_graph_info", "pgc_graph_get_product_space",
+    "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
+]
+
+_lib = None
+
+
+def load():
+    """Load libpgc.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PgcError("libpgc.so is not built (%s); run `make -C product-graph-cycles_amd`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    P, I64, I32, D = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+    L.pgc_last_error.restype = C.c_char_p
+    L.pgc_abi_version.restype = C.c_int
+    L.pgc_context_create.argtypes = [C.c_int, C.POINTER(P)]
+    L.pgc_context_destroy.argtypes = [P]
+    L.pgc_context_destroy.restype = None
+    L.pgc_graph_generate.argtypes = [P, P, I64, P, I64, P, I64, P, I64, P, C.c_int, C.c_int, C.c_int,
+                                     C.c_char_p, C.c_char_p, P, P, D, D, D, C.POINTER(P)]
+    L.pgc_graph_from_arrays.argtypes = [P, P, C.c_int, P, P, I64, P, C.c_int, I64, C.POINTER(P)]
+    L.pgc_graph_set_branch_graph.argtypes = [P, P, P]
+    L.pgc_graph_info.argtypes = [P, C.POINTER(I64), C.POINTER(I32), C.POINTER(I32), C.POINTER(I64),
+                                 C.POINTER(I64)]
+    L.pgc_graph_get_product_space.argtypes = [P, P]
+    L.pgc_graph_get_cost.argtypes = [P, P]
+    L.pgc_graph_set_cost.argtypes = [P, P]
+    L.pgc_graph_destroy.argtypes = [P]
+    L.pgc_graph_destroy.restype = None
+    L.pgc_solve_ratio.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
+    L.pgc_solve_min_cost.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
+    for name in EXPORTED:
+        getattr(L, name)
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise PgcError("libpgc error %d: %s" % (rc, load().pgc_last_error().decode()))
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """A device + HIP stream owned by libpgc."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        self._h = C.c_void_p()
+        check(L.pgc_context_create(int(device), C.byref(self._h)))
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            load().pgc_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+class Graph:
+    """A device-resident product graph (rows, SRC/TRGT, per-edge cost/time)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._h = handle
+
+    @classmethod
+    def generate(cls, ctx, VX, EX, VY, EY, FD, conjugate, regularising, prune, cost_mode, time_mode,
+                 NX=None, NY=None, rl=(0.7, 0.6, 4.0)):
+        L = load()
+        VX = np.ascontiguousarray(VX, np.float64)
+        VY = np.ascontiguousarray(VY, np.float64)
+        EX = np.ascontiguousarray(EX, np.int32)
+        EY = np.ascontiguousarray(EY, np.int32)
+        FD = np.ascontiguousarray(FD, np.float64)
+        NX = None if NX is None else np.ascontiguousarray(NX, np.float64)
+        NY = None if NY is None else np.ascontiguousarray(NY, np.float64)
+        h = C.c_void_p()
+        check(L.pgc_graph_generate(ctx.handle, ptr(VX), len(VX), ptr(EX), len(EX), ptr(VY), len(VY), ptr(EY),
+                                   len(EY), ptr(FD), int(conjugate), int(regularising), int(prune),
+                                   cost_mode.encode(), time_mode.encode(), ptr(NX), ptr(NY), float(rl[0]),
+                                   float(rl[1]), float(rl[2]), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_arrays(cls, ctx, productspace, src, trgt, cost, nodes_per_layer):
+        L = load()
+        src = np.ascontiguousarray(src, np.int32).reshape(-1)
+        trgt = np.ascontiguousarray(trgt, np.int32).reshape(-1)
+        cost = np.asarray(cost, np.float64)
+        if cost.ndim == 1:
+            cost = cost[:, None]
+        cm = np.ascontiguousarray(cost.T)
+        ps = None if productspace is None else np.ascontiguousarray(productspace, np.int32)
+        cols = 4 if ps is None else ps.shape[1]
+        h = C.c_void_p()
+        check(L.pgc_graph_from_arrays(ctx.handle, ptr(ps), cols, ptr(src), ptr(trgt), len(src), ptr(cm),
+                                      cm.shape[0], int(nodes_per_layer), C.byref(h)))
+        return cls(ctx, h)
+
+    def info(self):
+        e, npl, nl = C.c_int64(), C.c_int64(), C.c_int64()
+        pc, cc = C.c_int32(), C.c_int32()
+        check(load().pgc_graph_info(self._h, C.byref(e), C.byref(pc), C.byref(cc), C.byref(npl), C.byref(nl)))
+        return dict(n_edges=e.value, ps_cols=pc.value, n_cost_cols=cc.value, nodes_per_layer=npl.value,
+                    n_layers=nl.value)
+
+    def product_space(self):
+        i = self.info()
+        out = np.empty((i["n_edges"], i["ps_cols"]), np.int32)
+        check(load().pgc_graph_get_product_space(self._h, ptr(out)))
+        return out
+
+    def cost(self):
+        i = self.info()
+        out = np.empty((i["n_cost_cols"], i["n_edges"]), np.float64)
+        check(load().pgc_graph_get_cost(self._h, ptr(out)))
+        return out.T.copy()
+
+    def set_cost(self, cost):
+        cost = np.asarray(cost, np.float64)
+        if cost.ndim == 1:
+            cost = cost[:, None]
+        cm = np.ascontiguousarray(cost.T)
+        check(load().pgc_graph_set_cost(self._h, ptr(cm)))
+
+    def set_branch_graph(self, lists):
+        off = np.zeros(len(lists) + 1, np.int64)
+        vals = []
+        for i, l in enumerate(lists):
+            vals += sorted(int(x) for x in l)
+            off[i + 1] = len(vals)
+        v = np.array(vals if vals else [0], np.int64)
+        check(load().pgc_graph_set_branch_graph(self._h, ptr(off), ptr(v)))
+
+    def _solve(self, fn, max_depth, flags):
+        i = self.info()
+        cap = max(64, 101 * i["n_layers"] * max_depth + 8)
+        eids = np.empty(cap, np.int32)
+        rows = np.empty((cap, i["ps_cols"]), np.int32)
+        res = pgc_result()
+        check(fn(self.ctx.handle, self._h, int(max_depth), int(flags), ptr(eids), ptr(rows), cap, C.byref(res)))
+        n = res.n_edges
+        return eids[:n].copy(), rows[:n].copy(), res
+
+    def solve_ratio(self, max_depth, flags=PGC_LAWLER_PICK_LAST):
+        return self._solve(load().pgc_solve_ratio, max_depth, flags)
+
+    def solve_min_cost(self, max_depth, flags=0):
+        return self._solve(load().pgc_solve_min_cost, max_depth, flags)
+
+    def close(self):
+        if self._h:
+            load().pgc_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

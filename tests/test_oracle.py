@@ -1,0 +1,166 @@
+"""Pins the CPU oracle (oracle/pgc_oracle.cpp) before it is trusted as the checker.
+
+The reference ships no tests or fixtures (its only executable check is the
+unseeded main.cpp example), so the oracle is pinned against:
+  * the reference's own consistency check of the product space: every row is
+    reconstructed from its SRC/TRGT ids (product_space.cpp:103-120);
+  * closed forms of the energy recomputed with numpy expressions;
+  * numpy's median and an independent round-half-away-from-zero;
+  * exact optima of once-wrapping cycles on tiny product graphs (per-start DP
+    over the expanded DAG, tests/bruteforce.py), for both solvers.
+"""
+import numpy as np
+import pytest
+
+import bruteforce as B
+import pgc_oracle as O
+from cyclic_product_graphs import synthetic as S
+
+
+def test_product_space_reconstructs_rows():
+    p = S.make_pair(1, 12, seed=3)
+    EX, EY = p["EX"], p["EY"]
+    rows, src, trgt = O.product_space(EX, EY)
+    nVX = int(EX.max()) + 1
+    L = len(EY)
+    assert len(rows) == L * (2 * len(EX) + nVX)
+    ls, xs = src // nVX, src % nVX
+    lt, xt = trgt // nVX, trgt % nVX
+    assert np.array_equal(xs, rows[:, 2]) and np.array_equal(xt, rows[:, 3])
+    assert np.array_equal(EY[ls, 0], rows[:, 0])
+    intra = lt == ls
+    assert np.array_equal(rows[intra, 0], rows[intra, 1])
+    assert np.array_equal(lt[~intra], (ls[~intra] + 1) % L)
+    assert np.array_equal(EY[ls[~intra], 1], rows[~intra, 1])
+    # per layer: intra = EX, inter = EX + degenerate (x, x)
+    for i in [0, L - 1]:
+        m = ls == i
+        got_intra = sorted(map(tuple, rows[m & intra][:, 2:]))
+        got_inter = sorted(map(tuple, rows[m & ~intra][:, 2:]))
+        assert got_intra == sorted(map(tuple, EX))
+        assert got_inter == sorted(list(map(tuple, EX)) + [(x, x) for x in range(nVX)])
+
+
+def test_product_space_pruned():
+    p = S.make_pair(0, 5, seed=1)
+    rows, src, trgt = O.product_space(p["EX"], p["EY"], prune=True)
+    nVX = len(p["VX"])
+    assert len(rows) == 5 * (len(p["EX"]) + nVX)
+    assert np.all(trgt // nVX != src // nVX)
+
+
+@pytest.mark.parametrize("reg,mode", [(False, ("vanilla", "")), (True, ("vanilla", "nomode")),
+                                      (False, ("feature", "lengthnormalisation"))])
+def test_energy_closed_form(reg, mode):
+    p = S.make_pair(1, 10, seed=5)
+    rows, _, _ = O.product_space(p["EX"], p["EY"])
+    FD = p["feat_diff"].astype(np.float64)
+    en = O.energy(p["VX"], p["VY"], rows, FD, reg=reg, cost_mode=mode[0], time_mode=mode[1])
+    y0, y1, x0, x1 = rows.T
+    fd = FD[x0, y0] + FD[x1, y1]
+    seg = np.concatenate([p["VY"][y0] - p["VY"][y1], p["VX"][x0] - p["VX"][x1]], axis=1)
+    ln = np.linalg.norm(seg, axis=1)
+    if mode[0] == "feature":
+        assert en.shape[1] == 2
+        assert np.array_equal(en[:, 0], fd)
+        np.testing.assert_allclose(en[:, 1], ln, rtol=1e-15, atol=0)
+    elif reg:
+        np.testing.assert_allclose(en[:, 0], 0.5 * fd * ln, rtol=1e-15, atol=0)
+    else:
+        assert np.array_equal(en[:, 0], 0.5 * fd)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 10, 1001])
+def test_median_and_quantize(n):
+    rng = np.random.default_rng(n)
+    x = rng.gamma(2.0, 1.5, n)
+    x[rng.integers(0, n, max(1, n // 5))] *= 1e4  # values beyond the clip
+    assert O.median(x) == np.median(x)
+    q = O.quantize(x[:, None])[:, 0]
+    m = max(np.median(x), 0.001)
+    s = 1e5 / (2 * m)
+    y = s * x
+    expect = np.minimum(np.sign(y) * np.floor(np.abs(y) + 0.5), 1e7)
+    assert np.array_equal(q, expect)
+
+
+def test_quantize_tiny_median_floor():
+    x = np.array([0.0, 0.0, 0.0, 1e-4, 5.0])
+    q = O.quantize(x[:, None])[:, 0]
+    assert np.array_equal(q, np.minimum(np.floor(1e5 / 0.002 * x + 0.5), 1e7))
+
+
+def _tiny(seed, n_x, n_y, ring):
+    p = S.tiny_pair(n_x, n_y, seed=seed, ring=ring)
+    rows, src, trgt = O.product_space(p["EX"], p["EY"])
+    en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"])
+    return p, rows, src, trgt, en
+
+
+CASES = [(s, nx, ny, ring, K) for s, (nx, ny, ring) in enumerate([(4, 4, True), (4, 4, False), (5, 6, True),
+                                                                   (3, 4, False), (6, 4, True), (4, 6, False)])
+         for K in (1, 2, 3)]
+
+
+@pytest.mark.parametrize("seed,nx,ny,ring,K", CASES)
+@pytest.mark.parametrize("gpu_pick", [False, True])
+def test_ratio_solver_matches_bruteforce(seed, nx, ny, ring, K, gpu_pick):
+    p, rows, src, trgt, en = _tiny(seed, nx, ny, ring)
+    edges, obj, passes, stats, q = O.ctr_solve(src, trgt, en, nx, K, gpu_pick=gpu_pick)
+    cq, tq = q[:, 0], np.full(len(q), 5e4)
+    best = B.best_ratio(src, trgt, nx, K, cq, tq)
+    c, t = int(cq[edges].sum()), int(tq[edges].sum())
+    # the returned edges form a once-wrapping cycle of the product graph
+    assert passes
+    assert [tuple(x) for x in rows[edges]] == [tuple(rows[e]) for e in edges]
+    assert int(trgt[edges[-1]]) % nx == int(src[edges[0]]) % nx
+    for a, b in zip(edges[:-1], edges[1:]):
+        assert trgt[a] == src[b]
+    # optimal up to the reference's zero-cost tolerance (|w| < 0.5 q-units)
+    r = c / t
+    assert (r - float(best[0])) * best[2] < 0.5 + 1e-9
+    assert obj == np.float32(r)
+
+
+@pytest.mark.parametrize("seed,nx,ny,ring,K", CASES)
+@pytest.mark.parametrize("cpu_rules", [False, True])
+def test_mincost_solver_matches_bruteforce(seed, nx, ny, ring, K, cpu_rules):
+    p, rows, src, trgt, en = _tiny(seed, nx, ny, ring)
+    try:
+        edges, obj, passes, stats = O.dijkstra_solve(src, trgt, en[:, 0], nx, K, cpu_rules=cpu_rules)
+    except RuntimeError as ex:
+        # dijkstracpu seeds its branching BFS with a negative vertex id
+        # (dijkstra_solver.cpp:381) that the heap ignores; the BFS can then move
+        # every vertex out of the branch, and the reference exits on the empty
+        # branch (dijkstra_solver.cpp:236-238). The oracle reports that error.
+        assert cpu_rules and "no nodes in branch" in str(ex)
+        return
+    best = B.best_cost(src, trgt, nx, K, en[:, 0])
+    assert passes
+    cost = 0.0
+    for e in edges:
+        cost += float(en[e, 0])
+    assert abs(cost - best[0]) <= 1e-12 * max(1.0, abs(best[0]))
+    assert obj == np.float32(cost)
+    for a, b in zip(edges[:-1], edges[1:]):
+        assert trgt[a] == src[b]
+
+
+def test_ratio_time_column_bruteforce():
+    # feature / lengthnormalisation gives a real time column
+    p = S.tiny_pair(4, 4, seed=11, ring=False)
+    rows, src, trgt = O.product_space(p["EX"], p["EY"])
+    en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"], cost_mode="feature", time_mode="lengthnormalisation")
+    edges, obj, passes, stats, q = O.ctr_solve(src, trgt, en, 4, 2)
+    best = B.best_ratio(src, trgt, 4, 2, q[:, 0], q[:, 1])
+    c, t = int(q[edges, 0].sum()), int(q[edges, 1].sum())
+    assert (c / t - float(best[0])) * best[2] < 0.5 + 1e-9
+
+
+def test_quantize_in_place_changes_second_solve():
+    # a second Lawler solve re-quantizes already quantized costs (in-place energy)
+    p, rows, src, trgt, en = _tiny(2, 5, 6, True)
+    _, obj1, _, _, q1 = O.ctr_solve(src, trgt, en, 5, 2)
+    _, obj2, _, _, q2 = O.ctr_solve(src, trgt, q1, 5, 2)
+    assert np.all(q1 == np.round(q1))
+    assert abs(float(obj1) - float(obj2)) / float(obj1) < 1e-3
