@@ -1,0 +1,99 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry point
+declared in include/pgc.h, and reports errors through status codes."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from cyclic_product_graphs import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "pgc.h")).read()
+    return sorted(set(re.findall(r"\b(pgc_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_matches_exports():
+    L = _lib.load()
+    syms = declared_symbols()
+    assert "pgc_solve_ratio" in syms and "pgc_solve_min_cost" in syms
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(_lib.EXPORTED)
+
+
+def test_abi_version():
+    assert _lib.load().pgc_abi_version() == 1
+
+
+def test_errors_are_status_codes():
+    L = _lib.load()
+    assert L.pgc_context_create(0, None) == -1
+    assert b"null" in L.pgc_last_error()
+    assert L.pgc_solve_ratio(None, None, 2, 0, None, None, 0, None) == -1
+    assert L.pgc_graph_info(None, None, None, None, None, None) == -1
+
+
+def test_unknown_solver_names():
+    import numpy as np
+    import cyclic_product_graphs as cpg
+    from cyclic_product_graphs import synthetic as S
+    p = S.tiny_pair(4, 4)
+    g = cpg.product_graph_generator(p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"], False, False)
+    with pytest.raises(ValueError):
+        g.solve_with_cost_time_ratio("lawlerfoo", 2)
+
+
+def test_generator_validates_shapes():
+    import numpy as np
+    import cyclic_product_graphs as cpg
+    from cyclic_product_graphs import synthetic as S
+    p = S.tiny_pair(4, 4)
+    with pytest.raises(ValueError):
+        cpg.product_graph_generator(p["VX"][:, :2], p["EX"], p["VY"], p["EY"], p["feat_diff"], False, False)
+    with pytest.raises(ValueError):
+        cpg.product_graph_generator(p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"][:, :3], False, False)
+    g = cpg.product_graph_generator(p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"], True, False)
+    with pytest.raises(ValueError):
+        g.generate()  # conjugate graph without normals
+
+
+def test_export_writes_csv(tmp_path, monkeypatch):
+    import numpy as np
+    import cyclic_product_graphs as cpg
+    from cyclic_product_graphs import synthetic as S
+    p = S.tiny_pair(4, 4)
+    monkeypatch.chdir(tmp_path)
+    g = cpg.product_graph_generator(p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"], False, False)
+    passes, obj, m = g.solve_with_cost_time_ratio("export")
+    assert passes is False and obj == -1.0 and m.shape == (4, 1)
+    for name in ["VX", "VY", "EX", "EY", "FeatDiffMatrix"]:
+        assert (tmp_path / (name + ".csv")).exists()
+    back = np.loadtxt(tmp_path / "VX.csv", delimiter=",")
+    assert np.array_equal(back, p["VX"])
+    ex = np.loadtxt(tmp_path / "EX.csv", delimiter=",", dtype=np.int64)
+    assert np.array_equal(ex, p["EX"])
+
+
+def test_normals_helpers():
+    import numpy as np
+    import cyclic_product_graphs as cpg
+    from cyclic_product_graphs import synthetic as S
+    V, F = S.icosphere(1)
+    N = cpg.compute_normals_3d(V, F)
+    # unit sphere: vertex normals point radially outward
+    assert np.allclose(np.einsum("ij,ij->i", N, V), 1.0, atol=2e-2)
+    n = 16
+    a = 2 * np.pi * np.arange(n) / n
+    VY = np.stack([np.cos(a) + 3.0, np.sin(a) + 2.0, np.zeros(n)], 1)
+    EY = np.stack([np.arange(n), (np.arange(n) + 1) % n], 1)
+    NY = cpg.compute_normals_2d(VY, EY)
+    radial = VY - np.array([3.0, 2.0, 0.0])
+    assert np.allclose(np.einsum("ij,ij->i", NY, radial), 1.0, atol=1e-9)
+    # the plane is found by a near-zero column sum (helper/utils.cpp:101-107):
+    # a curve centred on the origin is taken for a yz/xz curve
+    centred = cpg.compute_normals_2d(radial, EY)
+    assert not np.allclose(np.einsum("ij,ij->i", centred, radial), 1.0)

@@ -1,0 +1,160 @@
+"""Parity of the HIP solvers (through libpgc.so's C-ABI) with the CPU oracle.
+
+Bar: product space and quantized costs bit-exact; the matching (product-space
+rows of the returned cycle, walk order) bit-exact; passes_layers_just_once
+equal; the fp32 objective equal (tolerance written below: 1e-5 relative, the
+observed difference is 0). Raw fp64 energies: bit-exact without the length
+term, within 1e-15 relative with it (the reference squares with std::pow,
+the device with a multiply).
+"""
+import numpy as np
+import pytest
+
+import pgc_oracle as O
+import cyclic_product_graphs as cpg
+from cyclic_product_graphs import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+OBJ_RTOL = 1e-5
+
+LAWLER = {"lawlergpu": True, "lawlercpu": False}
+MINCOST = {"dijkstragpu": False, "dijkstracpu": True}
+
+
+def _gen(p, reg=False, cost="vanilla", time="", prune=False):
+    g = cpg.product_graph_generator(p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"], False, reg, prune)
+    g.set_cost_time_ratio_mode(cost, time)
+    return g
+
+
+def _oracle_graph(p, reg=False, cost="vanilla", time="", prune=False):
+    rows, src, trgt = O.product_space(p["EX"], p["EY"], prune=prune)
+    en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"].astype(np.float64), reg=reg, cost_mode=cost,
+                  time_mode=time)
+    return rows, src, trgt, en
+
+
+def _check_ratio(p, K, name, reg=False, cost="vanilla", time="", prune=False):
+    print("case ratio", name, "nx", len(p["VX"]), "ny", len(p["VY"]), "K", K, flush=True)
+    g = _gen(p, reg, cost, time, prune)
+    passes, obj, rows_gpu = g.solve_with_cost_time_ratio(name, K)
+    rows, src, trgt, en = _oracle_graph(p, reg, cost, time, prune)
+    e, oobj, opasses, stats, q = O.ctr_solve(src, trgt, en, len(p["VX"]), K, gpu_pick=LAWLER[name])
+    assert passes == opasses
+    assert np.array_equal(rows_gpu, rows[e]), "matching differs"
+    assert abs(obj - float(oobj)) <= OBJ_RTOL * abs(float(oobj))
+    assert obj == float(oobj)
+    assert np.array_equal(g.get_cost_vector(), q), "quantized cost differs"
+    st = g.last_result
+    assert st["sweeps"] == stats[1] and st["bf_calls"] == stats[0] and st["branches"] == stats[3]
+    return g
+
+
+def _check_mincost(p, K, name, prune=False):
+    print("case mincost", name, "nx", len(p["VX"]), "ny", len(p["VY"]), "K", K, flush=True)
+    g = _gen(p, prune=prune)
+    rows, src, trgt, en = _oracle_graph(p, prune=prune)
+    bg = None
+    if prune:
+        bg = [[] for _ in range(len(p["VX"]))]
+        for a, b in p["EX"]:
+            bg[a].append(b)
+    try:
+        e, oobj, opasses, stats = O.dijkstra_solve(src, trgt, en[:, 0], len(p["VX"]), K, branch_graph=bg,
+                                                   cpu_rules=MINCOST[name])
+    except RuntimeError as ex:
+        assert "no nodes in branch" in str(ex)
+        with pytest.raises(cpg.PgcError):
+            g.solve_with_minimum_cost(name, K)
+        return
+    passes, obj, rows_gpu = g.solve_with_minimum_cost(name, K)
+    assert passes == opasses
+    assert np.array_equal(rows_gpu, rows[e]), "matching differs"
+    assert obj == float(oobj)
+    st = g.last_result
+    assert st["sweeps"] == stats[1] and st["branches"] == stats[3]
+
+
+TINY = [(s, nx, ny, ring, K) for s, (nx, ny, ring) in enumerate([(4, 4, True), (4, 4, False), (5, 6, True),
+                                                                  (3, 4, False), (6, 5, True), (4, 7, False)])
+        for K in (1, 2, 3)]
+
+
+@pytest.mark.parametrize("seed,nx,ny,ring,K", TINY)
+@pytest.mark.parametrize("name", list(LAWLER))
+def test_ratio_tiny(seed, nx, ny, ring, K, name):
+    _check_ratio(S.tiny_pair(nx, ny, seed=seed, ring=ring), K, name)
+
+
+@pytest.mark.parametrize("seed,nx,ny,ring,K", TINY)
+@pytest.mark.parametrize("name", list(MINCOST))
+def test_mincost_tiny(seed, nx, ny, ring, K, name):
+    _check_mincost(S.tiny_pair(nx, ny, seed=seed, ring=ring), K, name)
+
+
+@pytest.mark.parametrize("sub,ny,K,seed", [(1, 8, 2, 0), (2, 33, 2, 1), (2, 33, 3, 2), (2, 20, 1, 3),
+                                           (2, 41, 3, 4), (3, 64, 2, 0)])
+@pytest.mark.parametrize("name", list(LAWLER))
+def test_ratio_meshes(sub, ny, K, seed, name):
+    # odd |VY| exercises the reference's fold row numLayers % 2 for K >= 2
+    _check_ratio(S.make_pair(sub, ny, seed=seed), K, name)
+
+
+@pytest.mark.parametrize("sub,ny,K,seed", [(1, 8, 2, 0), (2, 33, 2, 1), (2, 20, 1, 3), (3, 64, 2, 0)])
+@pytest.mark.parametrize("name", list(MINCOST))
+def test_mincost_meshes(sub, ny, K, seed, name):
+    _check_mincost(S.make_pair(sub, ny, seed=seed), K, name)
+
+
+def test_ratio_modes():
+    p = S.make_pair(2, 24, seed=7)
+    _check_ratio(p, 2, "lawlergpu", reg=True)
+    _check_ratio(p, 2, "lawlergpu", cost="feature", time="lengthnormalisation")
+    _check_ratio(p, 2, "lawlergpu", prune=True)
+
+
+def test_mincost_pruned():
+    _check_mincost(S.make_pair(2, 24, seed=8), 2, "dijkstragpu", prune=True)
+
+
+@pytest.mark.parametrize("reg,cost,time", [(False, "vanilla", ""), (True, "vanilla", ""),
+                                           (False, "feature", "lengthnormalisation")])
+def test_graph_construction(reg, cost, time):
+    p = S.make_pair(3, 48, seed=4)
+    g = _gen(p, reg, cost, time)
+    rows, src, trgt, en = _oracle_graph(p, reg, cost, time)
+    assert np.array_equal(g.get_product_space(), rows)
+    got = g.get_cost_vector()
+    assert got.shape == en.shape
+    if reg or cost == "feature":
+        np.testing.assert_allclose(got, en, rtol=1e-15, atol=0)
+    else:
+        assert np.array_equal(got, en)
+
+
+def test_state_semantics():
+    p = S.make_pair(2, 16, seed=9)
+    g = _gen(p)
+    e0 = g.get_cost_vector()
+    g.set_cost_time_ratio_mode("feature", "lengthnormalisation")  # no effect after generation
+    g.solve_with_cost_time_ratio("lawlergpu", 2)
+    q1 = g.get_cost_vector()
+    assert q1.shape == e0.shape and np.all(q1 == np.round(q1))
+    # a second Lawler solve re-quantizes the quantized costs, as the reference does
+    rows, src, trgt, en = _oracle_graph(p)
+    _, _, _, _, qa = O.ctr_solve(src, trgt, en, len(p["VX"]), 2)
+    _, obj_b, _, _, qb = O.ctr_solve(src, trgt, qa, len(p["VX"]), 2, gpu_pick=True)
+    _, obj2, _ = g.solve_with_cost_time_ratio("lawlergpu", 2)
+    assert np.array_equal(g.get_cost_vector(), qb)
+    assert obj2 == float(obj_b)
+
+
+@pytest.mark.parametrize("name", ["lawlergpu", "dijkstragpu"])
+def test_c1_full_size(name):
+    # icosphere subdivision 4 (2562 vertices), 128 curve points, max_depth 3
+    p = S.make_pair(4, 128, seed=0)
+    if name == "lawlergpu":
+        _check_ratio(p, 3, name)
+    else:
+        _check_mincost(p, 3, name)
