@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X product-graph cycle solvers.
+
+Workload (one step = one complete `lawlergpu` solve of one shape pair, graph
+construction and quantization included): icosphere subdivision 4 (2562
+vertices, 15360 directed edges) against a 128-point closed curve, max_depth 3,
+cost vanilla / time nomode (mean problem), non-conjugate, no regulariser, no
+pruning; seeded synthetic features. The inputs (VX, EX, VY, EY, feat_diff) are
+device tensors before the timed region starts.
+
+Reported: solves/s (value), ms per solve, min-cost (`dijkstragpu`) solves/s on
+the same pair, Bellman-Ford GTEPS (edge relaxations / solver-kernel time), the
+roofline of the dominant kernel (the persistent solver kernel, HBM-bound:
+algorithmic bytes per sweep = E*(4 src + 4 cost [+ 4 time]) + V_x*4 parent,
+summed over the sweeps of each launch, divided by the launch's HIP-event
+duration), and the CPU oracle (restated reference CPU path) timed on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "product-graph-cycles_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _pair():
+    from cyclic_product_graphs import synthetic as S
+    return S.make_pair(4, 128, seed=0)
+
+
+def cpu_baseline(p, K, seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pgc_oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    O.set_threads(threads)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        rows, src, trgt = O.product_space(p["EX"], p["EY"])
+        en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"].astype(np.float64))
+        O.ctr_solve(src, trgt, en, len(p["VX"]), K, gpu_pick=True)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and n >= 2:
+            break
+    return {"value": n / el, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": "%d complete lawler solves (product space + energy + quantization + BnB) of the same pair, "
+                      "%.1f s, oracle/pgc_oracle.cpp with OpenMP" % (n, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the solver has no multi-GPU path in this build "
+              "(independent pairs would shard across GPUs with no collective). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from cyclic_product_graphs import _lib
+
+    K = 3
+    p = _pair()
+    dev = torch.device("cuda:0")
+    VX = torch.tensor(p["VX"], dtype=torch.float64, device=dev)
+    VY = torch.tensor(p["VY"], dtype=torch.float64, device=dev)
+    EX = torch.tensor(p["EX"], dtype=torch.int32, device=dev)
+    EY = torch.tensor(p["EY"], dtype=torch.int32, device=dev)
+    FD = torch.tensor(p["feat_diff"].astype(np.float64), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx = _lib.default_context(0)
+
+    def step(mincost=False):
+        g = _lib.Graph.generate_device(ctx, VX, EX, VY, EY, FD)
+        if mincost:
+            out = g.solve_min_cost(K, 0)
+        else:
+            out = g.solve_ratio(K, _lib.PGC_LAWLER_PICK_LAST)
+        info = g.info()
+        g.close()
+        return out, info
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    results = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    el = t1 - t0
+
+    # min-cost solver on the same pair (fewer steps)
+    nd = max(3, args.steps // 2)
+    step(mincost=True)
+    torch.cuda.synchronize()
+    d0 = time.perf_counter()
+    dres = [step(mincost=True) for _ in range(nd)]
+    torch.cuda.synchronize()
+    d_el = time.perf_counter() - d0
+
+    info = results[-1][1]
+    E = info["n_edges"]
+    npl = info["nodes_per_layer"]
+    L = info["n_layers"]
+    Vx = npl * (L + 1) * K
+    bytes_per_sweep = E * 8 + Vx * 4  # no time column: cost 4 B + src 4 B per edge, parent 4 B per node
+    sweeps = sum(r[0][2].sweeps for r in results)
+    kms = sum(r[0][2].kernel_ms for r in results)
+    relax = sum(r[0][2].relaxations for r in results)
+    launches = sum(r[0][2].branches + 1 for r in results)  # upper-bound launch + one per branch
+    phases = {}
+    for r in results:
+        for k, v in r[0][2].as_dict()["phase_ms"].items():
+            phases[k] = phases.get(k, 0.0) + v / args.steps
+    achieved = bytes_per_sweep * sweeps / (kms * 1e-3) / 1e9
+    gteps = relax / (kms * 1e-3) / 1e9
+
+    # parity of the last step with the CPU oracle (outside the timed region)
+    parity = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pgc_oracle as O
+        rows, src, trgt = O.product_space(p["EX"], p["EY"])
+        en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"].astype(np.float64))
+        e, oobj, _, _, _ = O.ctr_solve(src, trgt, en, npl, K, gpu_pick=True)
+        (eids, rws, res), _ = results[-1]
+        parity = bool(np.array_equal(rws, rows[e]) and np.float32(res.objective) == oobj)
+    except Exception as ex:  # the checker is optional at run time
+        print("parity check skipped: %s" % ex, file=sys.stderr)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, K, args.cpu_seconds)
+
+    line = {
+        "metric": "solves/sec & Bellman-Ford GTEPS; batched fwd+bwd vs plain loop, same cycle",
+        "value": args.steps / el,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded icosphere/curve pair, planted descriptors)",
+        "config": {"workload": "lawlergpu single pair, icosphere s=4 (2562 V, 15360 directed E) x 128-point curve, "
+                               "max_depth=3, vanilla/nomode, inputs resident in HBM",
+                   "n_edges": E, "nodes_per_layer": npl, "layers": L, "expanded_nodes": Vx},
+        "gteps": gteps,
+        "sweeps_per_solve": sweeps / args.steps,
+        "solver_launches_per_solve": launches / args.steps,
+        "solver_kernel_ms_per_solve": kms / args.steps,
+        "solver_phase_ms_per_solve": phases,
+        "mincost_solves_per_s": nd / d_el,
+        "mincost_ms_per_solve": d_el / nd * 1e3,
+        "parity_with_oracle": parity,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "pgc::solve_kernel (ratio mode)",
+                     "bytes_per_sweep": bytes_per_sweep},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -49,6 +49,8 @@ typedef struct pgc_result {
   double relaxations;         /* edge relaxations attempted (E_x per sweep, summed) */
   double kernel_ms;           /* device time of the solver kernels (HIP events) */
   double total_ms;            /* wall time of the call */
+  double phase_ms[5];         /* device time inside the solver kernels by phase: layer sweeps,
+                                 closing-layer fold, wrap chase, cycle walks, cycle extraction */
 } pgc_result;
 
 const char* pgc_last_error(void);
@@ -71,6 +73,15 @@ int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const in
                        int conjugate, int regularising, int prune, const char* cost_mode, const char* time_mode,
                        const double* NX, const double* NY, double rl_alpha, double rl_c, double rl_pwr,
                        pgc_graph** out);
+
+/* Same as pgc_graph_generate with every array already in device memory of the
+ * context's device (e.g. torch tensors); the caller orders its own writes to
+ * these buffers before the call. */
+int pgc_graph_generate_device(pgc_context* ctx, const double* VX, int64_t nVX, const int32_t* EX, int64_t nEX,
+                              const double* VY, int64_t nVY, const int32_t* EY, int64_t nEY,
+                              const double* feat_diff, int conjugate, int regularising, int prune,
+                              const char* cost_mode, const char* time_mode, const double* NX, const double* NY,
+                              double rl_alpha, double rl_c, double rl_pwr, pgc_graph** out);
 
 /* Takes the solver inputs of the reference's seam as host arrays: the
  * arguments of CostTimeRatioSolver / DijkstraSolver

@@ -202,62 +202,36 @@ __global__ void k_count(const int32_t* __restrict__ src, const int32_t* __restri
   atomicAdd(&cnt[cls * nTot + node], 1);
 }
 
-// width (in slots per lane) of every (class, layer, chunk); chunks = 64 vertices
-__global__ void k_chunk_width(const int32_t* __restrict__ cnt, int64_t npl, int64_t nLayers, int nChunks,
-                              int32_t* __restrict__ width) {
-  const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x & 63;
-  const int64_t per = nLayers * nChunks;
-  if (g >= 2 * per) return;
-  const int cls = (int)(g / per);
-  const int64_t r = g - cls * per;
-  const int64_t layer = r / nChunks;
-  const int c = (int)(r - layer * nChunks);
-  const int64_t v = (int64_t)c * 64 + lane;
-  int w = 0;
-  if (v < npl) w = cnt[cls * nLayers * npl + layer * npl + v];
-  for (int o = 32; o > 0; o >>= 1) w = max(w, __shfl_xor(w, o));
-  if (lane == 0) width[g] = w;
-}
-
-// exclusive scan of width*64 for both classes (single workgroup, n small)
-__global__ void k_scan_offsets(const int32_t* __restrict__ width, int64_t per, int32_t* __restrict__ off0,
-                               int32_t* __restrict__ off1, int64_t* __restrict__ totals) {
-  __shared__ int64_t part[1024];
-  for (int cls = 0; cls < 2; ++cls) {
-    const int32_t* w = width + cls * per;
-    int32_t* off = cls == 0 ? off0 : off1;
-    const int64_t chunk = (per + blockDim.x - 1) / blockDim.x;
-    const int64_t a = threadIdx.x * chunk, b = min(per, a + chunk);
-    int64_t s = 0;
-    for (int64_t i = a; i < b; ++i) s += (int64_t)w[i] * 64;
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t run = 0;
-      for (int i = 0; i < (int)blockDim.x; ++i) {
-        const int64_t x = part[i];
-        part[i] = run;
-        run += x;
-      }
-      totals[cls] = run;
-    }
-    __syncthreads();
-    int64_t run = part[threadIdx.x];
-    for (int64_t i = a; i < b; ++i) {
-      off[i] = (int32_t)run;
-      run += (int64_t)w[i] * 64;
-    }
-    if (threadIdx.x == blockDim.x - 1) off[per] = (int32_t)totals[cls];
-    __syncthreads();
+// per class: slots per lane (the maximum in-degree over all nodes) in
+// out[0..1] and the number of edges (the sum of in-degrees) in out[2..3]
+__global__ void k_class_width(const int32_t* __restrict__ cnt, int64_t nTot, int32_t* __restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int w0 = 0, w1 = 0, n0 = 0, n1 = 0;
+  for (int64_t i = g; i < nTot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = cnt[i], b = cnt[nTot + i];
+    w0 = max(w0, a);
+    w1 = max(w1, b);
+    n0 += a;
+    n1 += b;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    w0 = max(w0, __shfl_xor(w0, o));
+    w1 = max(w1, __shfl_xor(w1, o));
+    n0 += __shfl_xor(n0, o);
+    n1 += __shfl_xor(n1, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (w0) atomicMax(&out[0], w0);
+    if (w1) atomicMax(&out[1], w1);
+    if (n0) atomicAdd(&out[2], n0);
+    if (n1) atomicAdd(&out[3], n1);
   }
 }
 
 __global__ void k_fill_slots(const int32_t* __restrict__ src, const int32_t* __restrict__ trgt, int64_t E,
                              int64_t npl, int64_t numNodes, int nChunks, int32_t* __restrict__ fill,
-                             const int32_t* __restrict__ off0, const int32_t* __restrict__ off1,
-                             int32_t* __restrict__ s0, int32_t* __restrict__ s1, int32_t* __restrict__ e0,
-                             int32_t* __restrict__ e1) {
+                             const int32_t* __restrict__ width, int32_t* __restrict__ s0, int32_t* __restrict__ s1,
+                             int32_t* __restrict__ e0, int32_t* __restrict__ e1) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   int64_t node;
@@ -270,8 +244,8 @@ __global__ void k_fill_slots(const int32_t* __restrict__ src, const int32_t* __r
   const int64_t layer = node / npl;
   const int64_t v = node - layer * npl;
   const int c = (int)(v >> 6), lane = (int)(v & 63);
-  const int32_t* off = cls == 0 ? off0 : off1;
-  const int64_t slot = (int64_t)off[layer * nChunks + c] + (int64_t)pos * 64 + lane;
+  const int64_t w = width[cls];
+  const int64_t slot = (layer * nChunks + c) * w * 64 + (int64_t)pos * 64 + lane;
   const int64_t sl = s - (cls == 0 ? layer : layer - 1) * npl;
   (cls == 0 ? s0 : s1)[slot] = (int32_t)sl;
   (cls == 0 ? e0 : e1)[slot] = (int32_t)e;
@@ -279,9 +253,8 @@ __global__ void k_fill_slots(const int32_t* __restrict__ src, const int32_t* __r
 
 // order the slots of every node by (source, edge id)
 __global__ void k_sort_slots(const int32_t* __restrict__ cnt, int64_t npl, int64_t nLayers, int nChunks,
-                             const int32_t* __restrict__ off0, const int32_t* __restrict__ off1,
-                             int32_t* __restrict__ s0, int32_t* __restrict__ s1, int32_t* __restrict__ e0,
-                             int32_t* __restrict__ e1) {
+                             const int32_t* __restrict__ width, int32_t* __restrict__ s0, int32_t* __restrict__ s1,
+                             int32_t* __restrict__ e0, int32_t* __restrict__ e1) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t nTot = nLayers * npl;
   if (g >= 2 * nTot) return;
@@ -292,10 +265,9 @@ __global__ void k_sort_slots(const int32_t* __restrict__ cnt, int64_t npl, int64
   const int64_t layer = node / npl;
   const int64_t v = node - layer * npl;
   const int c = (int)(v >> 6), lane = (int)(v & 63);
-  const int32_t* off = cls == 0 ? off0 : off1;
   int32_t* S = cls == 0 ? s0 : s1;
   int32_t* Ee = cls == 0 ? e0 : e1;
-  const int64_t base = (int64_t)off[layer * nChunks + c] + lane;
+  const int64_t base = (layer * nChunks + c) * (int64_t)width[cls] * 64 + lane;
   for (int i = 1; i < n; ++i) {
     const int32_t ks = S[base + (int64_t)i * 64], ke = Ee[base + (int64_t)i * 64];
     int j = i - 1;
@@ -311,6 +283,13 @@ __global__ void k_sort_slots(const int32_t* __restrict__ cnt, int64_t npl, int64
   }
 }
 
+// padding slots (src < 0) point at the sentinel vertex npl of the solver's
+// distance rows (always +inf), so the solver needs no per-slot validity test
+__global__ void k_pad_src(int32_t* __restrict__ s, int64_t n, int32_t npl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && s[i] < 0) s[i] = npl;
+}
+
 // per-slot cost (float q-units or raw double) and time gathered by edge id
 template <typename CT>
 __global__ void k_fill_costs(const int32_t* __restrict__ eid, const int32_t* __restrict__ s, int64_t nSlots,
@@ -318,7 +297,7 @@ __global__ void k_fill_costs(const int32_t* __restrict__ eid, const int32_t* __r
                              float* __restrict__ time) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nSlots) return;
-  if (s[i] < 0) {
+  if (s[i] < 0 || eid[i] < 0) {
     cost[i] = (CT)0;
     if (time) time[i] = 0.f;
     return;
@@ -385,23 +364,28 @@ hipError_t launch_layout_count(const int32_t* src, const int32_t* trgt, int64_t 
   return hipGetLastError();
 }
 
-hipError_t launch_layout_offsets(const int32_t* cnt, int64_t npl, int64_t nLayers, int nChunks, int32_t* width,
-                                 int32_t* off0, int32_t* off1, int64_t* totals, hipStream_t st) {
-  const int64_t groups = 2 * nLayers * nChunks;
-  hipLaunchKernelGGL(k_chunk_width, dim3(nb(groups, 4)), dim3(256), 0, st, cnt, npl, nLayers, nChunks, width);
-  hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, st, width, nLayers * nChunks, off0, off1, totals);
+hipError_t launch_layout_width(const int32_t* cnt, int64_t nTot, int32_t* width, hipStream_t st) {
+  hipError_t e = hipMemsetAsync(width, 0, 4 * 4, st);
+  if (e != hipSuccess) return e;
+  const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (nTot + 255) / 256));
+  hipLaunchKernelGGL(k_class_width, dim3(g), dim3(256), 0, st, cnt, nTot, width);
   return hipGetLastError();
 }
 
 hipError_t launch_layout_fill(const int32_t* src, const int32_t* trgt, int64_t E, int64_t npl, int64_t numNodes,
-                              int nChunks, int32_t* fill, const int32_t* cnt, const int32_t* off0,
-                              const int32_t* off1, int32_t* s0, int32_t* s1, int32_t* e0, int32_t* e1,
-                              hipStream_t st) {
+                              int nChunks, int32_t* fill, const int32_t* cnt, const int32_t* width, int32_t* s0,
+                              int32_t* s1, int32_t* e0, int32_t* e1, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_slots, dim3(nb(E, 256)), dim3(256), 0, st, src, trgt, E, npl, numNodes, nChunks, fill,
-                     off0, off1, s0, s1, e0, e1);
+                     width, s0, s1, e0, e1);
   const int64_t nLayers = numNodes / npl + 1;
   hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nLayers * npl, 256)), dim3(256), 0, st, cnt, npl, nLayers, nChunks,
-                     off0, off1, s0, s1, e0, e1);
+                     width, s0, s1, e0, e1);
+  return hipGetLastError();
+}
+
+hipError_t launch_pad_src(int32_t* s, int64_t n, int32_t npl, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pad_src, dim3(nb(n, 256)), dim3(256), 0, st, s, n, npl);
   return hipGetLastError();
 }
 

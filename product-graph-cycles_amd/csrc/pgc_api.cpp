@@ -35,18 +35,17 @@ hipError_t launch_quantize(double*, int64_t, void*, hipStream_t);
 size_t select_state_bytes();
 hipError_t launch_layout_count(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int32_t*, int32_t*,
                                hipStream_t);
-hipError_t launch_layout_offsets(const int32_t*, int64_t, int64_t, int, int32_t*, int32_t*, int32_t*, int64_t*,
-                                 hipStream_t);
+hipError_t launch_layout_width(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_layout_fill(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int, int32_t*,
-                              const int32_t*, const int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*, int32_t*,
-                              hipStream_t);
+                              const int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t);
 hipError_t launch_fill_costs(const int32_t*, const int32_t*, int64_t, const double*, int64_t, int, bool, void*,
                              float*, hipStream_t);
 hipError_t launch_gather_rows(const int32_t*, int, const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
+hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
 // solver.hip
 size_t solver_lds_bytes(int npl, int K);
-hipError_t launch_solver(const SolverJob*, int, int, bool, bool, bool, size_t, hipStream_t);
+hipError_t launch_solver(const SolverJob*, int, int, int, bool, bool, bool, size_t, hipStream_t);
 hipError_t launch_lookup(const SolverJob*, const int32_t*, int, int32_t*, bool, bool, hipStream_t);
 }  // namespace pgc
 
@@ -91,11 +90,12 @@ struct JobScratch {
   double *tmp_c = nullptr, *tmp_t = nullptr, *neg_c = nullptr, *neg_t = nullptr;
   int32_t *hits = nullptr, *back = nullptr, *branch = nullptr, *nodes_io = nullptr, *eids_io = nullptr;
   double* lb_key = nullptr;
+  double* backup = nullptr;
   JobResult* res = nullptr;
   SolverJob* djob = nullptr;
   void release() {
     void* ps[] = {pred, wnext, wlen, chain, tmp_nodes, tmp_eid, neg_nodes, neg_eid, neg_len, tmp_c, tmp_t,
-                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, res, djob};
+                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, res, djob};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = JobScratch();
@@ -115,7 +115,11 @@ struct pgc_graph {
   int nChunks = 0;
   int64_t nLayers = 0;
   int64_t nslots[2] = {0, 0};
-  int32_t *d_off[2] = {nullptr, nullptr}, *d_s[2] = {nullptr, nullptr}, *d_e[2] = {nullptr, nullptr};
+  int32_t* d_width = nullptr;  // per class: slots per lane, then edge count (4 values)
+  int32_t *d_s[2] = {nullptr, nullptr}, *d_e[2] = {nullptr, nullptr};
+  int32_t width[2] = {0, 0};
+  int32_t lstride[2] = {0, 0};
+  int32_t maxw = 0;
   int32_t* d_cnt = nullptr;
   void* d_cost[2] = {nullptr, nullptr};
   float* d_time[2] = {nullptr, nullptr};
@@ -128,7 +132,7 @@ struct pgc_graph {
   JobScratch js;
   ~pgc_graph() {
     js.release();
-    void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_off[0], d_off[1], d_s[0], d_s[1], d_e[0], d_e[1],
+    void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_width, d_s[0], d_s[1], d_e[0], d_e[1],
                   d_cnt, d_cost[0], d_cost[1], d_time[0], d_time[1]};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -193,50 +197,58 @@ int finish_graph_meta(pgc_graph* g) {
   return PGC_OK;
 }
 
+constexpr int64_t kSlotPad = 32 * 64;
+
 int build_layout(pgc_graph* g) {
   if (g->layout) return PGC_OK;
   hipStream_t st = g->ctx->stream;
   const int64_t npl = g->npl;
   g->nLayers = g->L + 1;
   g->nChunks = (int)((npl + 63) / 64);
+  const int nC = g->nChunks;
   const int64_t nTot = g->nLayers * npl;
-  const int64_t per = g->nLayers * g->nChunks;
-  int32_t *d_dropped = nullptr, *d_width = nullptr, *d_fill = nullptr;
-  int64_t* d_tot = nullptr;
+  int32_t *d_dropped = nullptr, *d_fill = nullptr;
   HIPCHK(dalloc(&g->d_cnt, 2 * nTot));
   HIPCHK(dalloc(&d_fill, 2 * nTot));
   HIPCHK(dalloc(&d_dropped, 1));
-  HIPCHK(dalloc(&d_width, 2 * per));
-  HIPCHK(dalloc(&d_tot, 2));
-  HIPCHK(dalloc(&g->d_off[0], per + 1));
-  HIPCHK(dalloc(&g->d_off[1], per + 1));
+  HIPCHK(dalloc(&g->d_width, 4));
   HIPCHK(hipMemsetAsync(g->d_cnt, 0, 2 * nTot * 4, st));
   HIPCHK(hipMemsetAsync(d_fill, 0, 2 * nTot * 4, st));
   HIPCHK(hipMemsetAsync(d_dropped, 0, 4, st));
   HIPCHK(launch_layout_count(g->d_src, g->d_trgt, g->E, npl, g->numNodes, g->d_cnt, d_dropped, st));
-  HIPCHK(launch_layout_offsets(g->d_cnt, npl, g->nLayers, g->nChunks, d_width, g->d_off[0], g->d_off[1], d_tot, st));
-  int64_t tot[2];
+  HIPCHK(launch_layout_width(g->d_cnt, nTot, g->d_width, st));
   int32_t dropped = 0;
-  HIPCHK(hipMemcpyAsync(tot, d_tot, sizeof(tot), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&dropped, d_dropped, 4, hipMemcpyDeviceToHost, st));
+  int32_t wn[4];
+  HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (tot[0] >= (int64_t)1 << 31 || tot[1] >= (int64_t)1 << 31)
-    return fail(PGC_ERR_UNSUPPORTED, "in-edge layout exceeds 2^31 slots");
+  g->width[0] = wn[0];
+  g->width[1] = wn[1];
+  g->intra_slots_valid = wn[2];
+  g->inter_slots_valid = wn[3];
+  g->maxw = std::max(g->width[0], g->width[1]);
   g->dropped = dropped;
   for (int c = 0; c < 2; ++c) {
-    g->nslots[c] = tot[c];
-    HIPCHK(dalloc(&g->d_s[c], tot[c]));
-    HIPCHK(dalloc(&g->d_e[c], tot[c]));
-    HIPCHK(hipMemsetAsync(g->d_s[c], 0xff, std::max<int64_t>(tot[c], 1) * 4, st));
-    HIPCHK(hipMemsetAsync(g->d_e[c], 0xff, std::max<int64_t>(tot[c], 1) * 4, st));
+    const int64_t per_layer = (int64_t)nC * g->width[c] * 64;
+    const int64_t total = per_layer * g->nLayers;
+    if (total >= ((int64_t)1 << 31) - 1 - kSlotPad)
+      return fail(PGC_ERR_UNSUPPORTED, "in-edge layout exceeds 2^31 slots");
+    g->lstride[c] = (int32_t)per_layer;
+    g->nslots[c] = total;
+    // the solver may read up to 32 slots per lane past a chunk start; the
+    // padding keeps those reads inside the allocation
+    const int64_t alloc = total + kSlotPad;
+    HIPCHK(dalloc(&g->d_s[c], alloc));
+    HIPCHK(dalloc(&g->d_e[c], alloc));
+    HIPCHK(hipMemsetAsync(g->d_s[c], 0xff, alloc * 4, st));
+    HIPCHK(hipMemsetAsync(g->d_e[c], 0xff, alloc * 4, st));
   }
-  HIPCHK(launch_layout_fill(g->d_src, g->d_trgt, g->E, npl, g->numNodes, g->nChunks, d_fill, g->d_cnt, g->d_off[0],
-                            g->d_off[1], g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
+  HIPCHK(launch_layout_fill(g->d_src, g->d_trgt, g->E, npl, g->numNodes, nC, d_fill, g->d_cnt, g->d_width,
+                            g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
+  for (int c = 0; c < 2; ++c) HIPCHK(launch_pad_src(g->d_s[c], g->nslots[c] + kSlotPad, (int32_t)npl, st));
   HIPCHK(hipStreamSynchronize(st));
   (void)hipFree(d_dropped);
-  (void)hipFree(d_width);
   (void)hipFree(d_fill);
-  (void)hipFree(d_tot);
   g->layout = true;
   return PGC_OK;
 }
@@ -250,9 +262,13 @@ int fill_costs(pgc_graph* g, int kind) {
     g->d_cost[c] = nullptr;
     g->d_time[c] = nullptr;
     const int64_t n = g->nslots[c];
-    HIPCHK(hipMalloc(&g->d_cost[c], std::max<int64_t>(n, 1) * (kind == 2 ? 8 : 4)));
+    HIPCHK(hipMalloc(&g->d_cost[c], (n + kSlotPad) * (kind == 2 ? 8 : 4)));
+    HIPCHK(hipMemsetAsync(g->d_cost[c], 0, (n + kSlotPad) * (kind == 2 ? 8 : 4), st));
     const bool withTime = kind == 1 && g->ncols > 1;
-    if (withTime) HIPCHK(dalloc(&g->d_time[c], n));
+    if (withTime) {
+      HIPCHK(dalloc(&g->d_time[c], n + kSlotPad));
+      HIPCHK(hipMemsetAsync(g->d_time[c], 0, (n + kSlotPad) * 4, st));
+    }
     HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], n, g->d_energy, g->E, g->ncols, kind == 2, g->d_cost[c],
                              withTime ? g->d_time[c] : nullptr, st));
   }
@@ -292,6 +308,7 @@ int ensure_scratch(pgc_graph* g, int K) {
   HIPCHK(dalloc(&s.nodes_io, cap));
   HIPCHK(dalloc(&s.eids_io, cap));
   HIPCHK(dalloc(&s.lb_key, npl));
+  HIPCHK(dalloc(&s.backup, npl));
   HIPCHK(dalloc(&s.res, 1));
   HIPCHK(dalloc(&s.djob, 1));
   return PGC_OK;
@@ -319,7 +336,8 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   if (const char* cap = std::getenv("PGC_SWEEP_CAP")) j.max_sweeps = std::max<int64_t>(1, std::atoll(cap));
   for (int c = 0; c < 2; ++c) {
     SlotClass& S = c == 0 ? j.intra : j.inter;
-    S.off = g->d_off[c];
+    S.w = g->width[c];
+    S.lstride = g->lstride[c];
     S.src = g->d_s[c];
     S.cost = g->d_cost[c];
     S.time = g->d_time[c];
@@ -327,6 +345,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   }
   j.branch = s.branch;
   j.pred = s.pred;
+  j.backup = s.backup;
   j.wnext = s.wnext;
   j.wlen = s.wlen;
   j.chain = s.chain;
@@ -354,6 +373,8 @@ struct Timer {
 };
 
 // one solver launch: upload job, run, read result (kernel time accumulated)
+static thread_local double g_phase_ms[5];
+
 int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, double& kms) {
   pgc_context* c = g->ctx;
   hipStream_t st = c->stream;
@@ -361,13 +382,16 @@ int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, doub
   const size_t lds = solver_lds_bytes((int)g->npl, j.K);
   const int maxc = (g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE);
   HIPCHK(hipEventRecord(c->ev0, st));
-  HIPCHK(launch_solver(g->js.djob, 1, maxc, mincost, g->d_time[0] != nullptr, g->cost_kind == 2, lds, st));
+  HIPCHK(launch_solver(g->js.djob, 1, maxc, g->maxw, mincost, g->d_time[0] != nullptr, g->cost_kind == 2, lds,
+                       st));
   HIPCHK(hipEventRecord(c->ev1, st));
   HIPCHK(hipMemcpyAsync(&out, g->js.res, sizeof(out), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   kms += ms;
+  const int64_t ticks[5] = {out.t_sweep, out.t_fold, out.t_chase, out.t_walk, out.t_extract};
+  for (int i = 0; i < 5; ++i) g_phase_ms[i] += (double)ticks[i] * 1e-5;  // 100 MHz wall clock
   return PGC_OK;
 }
 
@@ -381,24 +405,14 @@ int check_solvable(pgc_graph* g, int K) {
   return PGC_OK;
 }
 
+int check_width(pgc_graph* g) {
+  if (g->maxw > 32) return fail(PGC_ERR_UNSUPPORTED, "a node has more than 32 in-edges of one class");
+  return PGC_OK;
+}
+
 // E_x: relaxations attempted per sweep = intra slots*(K-1) + inter slots*K
 double relax_per_sweep(pgc_graph* g, int K) {
   return (double)g->intra_slots_valid * (K - 1) + (double)g->inter_slots_valid * K;
-}
-
-int count_valid_slots(pgc_graph* g) {
-  // number of edges in each class (sum of in-degrees)
-  const int64_t nTot = g->nLayers * g->npl;
-  std::vector<int32_t> cnt(2 * nTot);
-  HIPCHK(hipMemcpy(cnt.data(), g->d_cnt, cnt.size() * 4, hipMemcpyDeviceToHost));
-  int64_t a = 0, b = 0;
-  for (int64_t i = 0; i < nTot; ++i) {
-    a += cnt[i];
-    b += cnt[nTot + i];
-  }
-  g->intra_slots_valid = a;
-  g->inter_slots_valid = b;
-  return PGC_OK;
 }
 
 }  // namespace
@@ -437,33 +451,35 @@ int pgc_graph_from_arrays(pgc_context* ctx, const int32_t* productspace, int ps_
   return PGC_OK;
 }
 
-int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const int32_t* EX, int64_t nEX,
-                       const double* VY, int64_t nVY, const int32_t* EY, int64_t nEY, const double* feat_diff,
-                       int conjugate, int regularising, int prune, const char* cost_mode, const char* time_mode,
-                       const double* NX, const double* NY, double rl_alpha, double rl_c, double rl_pwr,
-                       pgc_graph** out) {
-  (void)NX;
-  (void)NY;
-  (void)rl_alpha;
-  (void)rl_c;
-  (void)rl_pwr;
-  if (!ctx || !out || !VX || !EX || !VY || !EY || !feat_diff || nVX <= 0 || nEX <= 0 || nVY <= 0 || nEY <= 0)
-    return fail(PGC_ERR_ARG, "invalid arguments");
+}  // extern "C"
+
+namespace {
+
+// Shared body of the two generate entry points; all array arguments are
+// device pointers, EX_host (optional) is the host copy used for validation.
+int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_t* dEX, int64_t nEX,
+                  const double* dVY, int64_t nVY, const int32_t* dEY, int64_t nEY, const double* dFD, int conjugate,
+                  int regularising, int prune, const char* cost_mode, const char* time_mode, const int32_t* EX_host,
+                  pgc_graph** out) {
   if (conjugate) return fail(PGC_ERR_UNSUPPORTED, "conjugate product graph is not available in this build");
+  hipStream_t st = ctx->stream;
+  std::vector<int32_t> exh;
+  if (!EX_host) {
+    exh.resize(2 * nEX);
+    HIPCHK(hipMemcpyAsync(exh.data(), dEX, 2 * nEX * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    EX_host = exh.data();
+  }
   int32_t mx = -1;
   for (int64_t i = 0; i < 2 * nEX; ++i) {
-    if (EX[i] < 0) return fail(PGC_ERR_ARG, "EX holds a negative vertex id");
-    mx = std::max(mx, EX[i]);
+    if (EX_host[i] < 0) return fail(PGC_ERR_ARG, "EX holds a negative vertex id");
+    mx = std::max(mx, EX_host[i]);
   }
   const int64_t nVXe = (int64_t)mx + 1;  // product_space.cpp:20
   if (nVXe != nVX)
     return fail(PGC_ERR_ARG, "max(EX)+1 differs from |VX|: the product space and the solver would disagree");
-  for (int64_t i = 0; i < 2 * nEY; ++i)
-    if (EY[i] < 0 || EY[i] >= nVY) return fail(PGC_ERR_ARG, "EY holds an id outside [0, |VY|)");
-  HIPCHK(hipSetDevice(ctx->device));
   std::unique_ptr<pgc_graph> g(new pgc_graph());
   g->ctx = ctx;
-  hipStream_t st = ctx->stream;
   const int64_t E = nEY * ((prune ? 1 : 2) * nEX + nVXe);
   g->E = E;
   g->npl = nVX;
@@ -471,38 +487,20 @@ int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const in
   const bool lengthMode = cost_mode && time_mode && std::strcmp(cost_mode, "feature") == 0 &&
                           std::strcmp(time_mode, "lengthnormalisation") == 0;
   g->ncols = lengthMode ? 2 : 1;
-  int32_t *dEX = nullptr, *dEY = nullptr;
-  double *dVX = nullptr, *dVY = nullptr, *dFD = nullptr;
-  HIPCHK(dalloc(&dEX, 2 * nEX));
-  HIPCHK(dalloc(&dEY, 2 * nEY));
-  HIPCHK(dalloc(&dVX, 3 * nVX));
-  HIPCHK(dalloc(&dVY, 3 * nVY));
-  HIPCHK(dalloc(&dFD, nVX * nVY));
   HIPCHK(dalloc(&g->d_rows, E * 4));
   HIPCHK(dalloc(&g->d_src, E));
   HIPCHK(dalloc(&g->d_trgt, E));
   HIPCHK(dalloc(&g->d_energy, E * g->ncols));
-  HIPCHK(hipMemcpyAsync(dEX, EX, 2 * nEX * 4, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dEY, EY, 2 * nEY * 4, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dVX, VX, 3 * nVX * 8, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dVY, VY, 3 * nVY * 8, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(dFD, feat_diff, nVX * nVY * 8, hipMemcpyHostToDevice, st));
   HIPCHK(launch_product_space(dEX, nEX, dEY, nEY, nVXe, prune ? 1 : 0, g->d_rows, g->d_src, g->d_trgt, E, st));
   HIPCHK(launch_energy(dVX, dVY, g->d_rows, E, dFD, nVY, (regularising || lengthMode) ? 1 : 0, lengthMode ? 1 : 0,
                        g->d_energy, st));
-  HIPCHK(hipStreamSynchronize(st));
-  (void)hipFree(dEX);
-  (void)hipFree(dEY);
-  (void)hipFree(dVX);
-  (void)hipFree(dVY);
-  (void)hipFree(dFD);
-  // numNodes / numLayers as the solver derives them
+  // numNodes / numLayers as the solver derives them (run(), :870-871)
   g->numNodes = nEY * nVXe;
   g->L = nEY;
   if (prune) {
     // branchGraph[x] = out-neighbour vertex ids (product_space.cpp:122-131)
     std::vector<std::vector<int64_t>> bg(nVX);
-    for (int64_t j = 0; j < nEX; ++j) bg[EX[2 * j]].push_back(EX[2 * j + 1]);
+    for (int64_t j = 0; j < nEX; ++j) bg[EX_host[2 * j]].push_back(EX_host[2 * j + 1]);
     g->bg_off.assign(nVX + 1, 0);
     g->bg_val.clear();
     for (int64_t v = 0; v < nVX; ++v) {
@@ -515,6 +513,66 @@ int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const in
   }
   *out = g.release();
   return PGC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const int32_t* EX, int64_t nEX,
+                       const double* VY, int64_t nVY, const int32_t* EY, int64_t nEY, const double* feat_diff,
+                       int conjugate, int regularising, int prune, const char* cost_mode, const char* time_mode,
+                       const double* NX, const double* NY, double rl_alpha, double rl_c, double rl_pwr,
+                       pgc_graph** out) {
+  (void)NX;
+  (void)NY;
+  (void)rl_alpha;
+  (void)rl_c;
+  (void)rl_pwr;
+  if (!ctx || !out || !VX || !EX || !VY || !EY || !feat_diff || nVX <= 0 || nEX <= 0 || nVY <= 0 || nEY <= 0)
+    return fail(PGC_ERR_ARG, "invalid arguments");
+  for (int64_t i = 0; i < 2 * nEY; ++i)
+    if (EY[i] < 0 || EY[i] >= nVY) return fail(PGC_ERR_ARG, "EY holds an id outside [0, |VY|)");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int32_t *dEX = nullptr, *dEY = nullptr;
+  double *dVX = nullptr, *dVY = nullptr, *dFD = nullptr;
+  HIPCHK(dalloc(&dEX, 2 * nEX));
+  HIPCHK(dalloc(&dEY, 2 * nEY));
+  HIPCHK(dalloc(&dVX, 3 * nVX));
+  HIPCHK(dalloc(&dVY, 3 * nVY));
+  HIPCHK(dalloc(&dFD, nVX * nVY));
+  HIPCHK(hipMemcpyAsync(dEX, EX, 2 * nEX * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dEY, EY, 2 * nEY * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dVX, VX, 3 * nVX * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dVY, VY, 3 * nVY * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dFD, feat_diff, nVX * nVY * 8, hipMemcpyHostToDevice, st));
+  const int rc = generate_impl(ctx, dVX, nVX, dEX, nEX, dVY, nVY, dEY, nEY, dFD, conjugate, regularising, prune,
+                               cost_mode, time_mode, EX, out);
+  HIPCHK(hipStreamSynchronize(st));
+  (void)hipFree(dEX);
+  (void)hipFree(dEY);
+  (void)hipFree(dVX);
+  (void)hipFree(dVY);
+  (void)hipFree(dFD);
+  return rc;
+}
+
+int pgc_graph_generate_device(pgc_context* ctx, const double* VX, int64_t nVX, const int32_t* EX, int64_t nEX,
+                              const double* VY, int64_t nVY, const int32_t* EY, int64_t nEY,
+                              const double* feat_diff, int conjugate, int regularising, int prune,
+                              const char* cost_mode, const char* time_mode, const double* NX, const double* NY,
+                              double rl_alpha, double rl_c, double rl_pwr, pgc_graph** out) {
+  (void)NX;
+  (void)NY;
+  (void)rl_alpha;
+  (void)rl_c;
+  (void)rl_pwr;
+  if (!ctx || !out || !VX || !EX || !VY || !EY || !feat_diff || nVX <= 0 || nEX <= 0 || nVY <= 0 || nEY <= 0)
+    return fail(PGC_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(ctx->device));
+  return generate_impl(ctx, VX, nVX, EX, nEX, VY, nVY, EY, nEY, feat_diff, conjugate, regularising, prune,
+                       cost_mode, time_mode, nullptr, out);
 }
 
 int pgc_graph_set_branch_graph(pgc_graph* g, const int64_t* offsets, const int64_t* values) {
@@ -571,6 +629,7 @@ void pgc_graph_destroy(pgc_graph* g) {
 int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
                     int64_t capacity, pgc_result* res) {
   Timer wall;
+  for (double& x : g_phase_ms) x = 0;
   if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
   if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
   int rc = check_solvable(g, K);
@@ -581,7 +640,7 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   // robustFloatToIntConversion of every column, in place (:875-882)
   for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
   if ((rc = build_layout(g))) return rc;
-  if ((rc = count_valid_slots(g))) return rc;
+  if ((rc = check_width(g))) return rc;
   if ((rc = fill_costs(g, 1))) return rc;
   if ((rc = ensure_scratch(g, K))) return rc;
   JobScratch& s = g->js;
@@ -719,6 +778,7 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   res->relaxations = relax_per_sweep(g, K) * (double)sweeps;
   res->kernel_ms = kms;
   res->total_ms = wall.ms();
+  for (int i = 0; i < 5; ++i) res->phase_ms[i] = g_phase_ms[i];
   return PGC_OK;
 }
 
@@ -726,6 +786,7 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
 int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
                        int64_t capacity, pgc_result* res) {
   Timer wall;
+  for (double& x : g_phase_ms) x = 0;
   if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
   if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
   int rc = check_solvable(g, K);
@@ -735,19 +796,15 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   const int64_t npl = g->npl, L = g->L;
   const bool cpuRules = (flags & PGC_MINCOST_CPU_RULES) != 0;
   if ((rc = build_layout(g))) return rc;
-  if ((rc = count_valid_slots(g))) return rc;
+  if ((rc = check_width(g))) return rc;
   if ((rc = fill_costs(g, 2))) return rc;
   if ((rc = ensure_scratch(g, K))) return rc;
   JobScratch& s = g->js;
   HIPCHK(hipMemsetAsync(s.pred, 0xff, s.predN * 4, st));
   // branch graph: the given one, else the in-edges of layer-0 nodes (ascending edge id)
   if (!g->has_bg) {
-    const int nC = g->nChunks;
-    std::vector<int32_t> off(nC + 1), sl, el;
-    HIPCHK(hipMemcpy(off.data(), g->d_off[0], (nC + 1) * 4, hipMemcpyDeviceToHost));
-    const int64_t n0 = off[nC];
-    sl.resize(n0);
-    el.resize(n0);
+    const int64_t n0 = g->lstride[0];  // layer-0 block of the intralayer class
+    std::vector<int32_t> sl(n0), el(n0);
     if (n0) {
       HIPCHK(hipMemcpy(sl.data(), g->d_s[0], n0 * 4, hipMemcpyDeviceToHost));
       HIPCHK(hipMemcpy(el.data(), g->d_e[0], n0 * 4, hipMemcpyDeviceToHost));
@@ -756,11 +813,11 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
     g->bg_val.clear();
     for (int64_t v = 0; v < npl; ++v) {
       const int c = (int)(v >> 6), lane = (int)(v & 63);
-      const int w = (off[c + 1] - off[c]) / 64;
+      const int w = g->width[0];
       std::vector<int64_t> ids;
       for (int j = 0; j < w; ++j) {
-        const int64_t idx = off[c] + (int64_t)j * 64 + lane;
-        if (sl[idx] < 0) break;
+        const int64_t idx = ((int64_t)c * w + j) * 64 + lane;
+        if (sl[idx] < 0 || sl[idx] >= npl) break;
         ids.push_back(el[idx]);
       }
       std::sort(ids.begin(), ids.end());
@@ -991,6 +1048,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   res->relaxations = relax_per_sweep(g, K) * (double)sweeps;
   res->kernel_ms = kms;
   res->total_ms = wall.ms();
+  for (int i = 0; i < 5; ++i) res->phase_ms[i] = g_phase_ms[i];
   return PGC_OK;
 }
 

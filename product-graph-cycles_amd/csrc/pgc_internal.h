@@ -7,12 +7,14 @@
 //   * the in-edges of every node are split into two classes, intralayer
 //     (source on the same layer) and interlayer (source on the previous
 //     layer), and stored per layer as sliced ELL with 64-vertex slices (one
-//     wave per slice): slice c of layer l occupies slots
-//     [off[l*nChunks+c], off[l*nChunks+c+1]) laid out slot-major, so slot j
-//     of lane i sits at off + 64*j + i and a wave's loads are contiguous;
+//     wave per slice): every slice of a class has w slots per lane (the
+//     class's maximum in-degree) and slot j of lane i of slice c on layer l
+//     sits at l*stride + (c*w + j)*64 + i, so a wave's loads are contiguous
+//     and all addressing is arithmetic on wave-uniform values;
 //   * inside a vertex the in-edges are ordered by (source id, edge id), the
 //     documented deterministic replacement for the reference's hash-set
-//     iteration order; padding slots carry src = -1.
+//     iteration order; padding slots carry src = npl (a sentinel whose
+//     distance is +inf in the solver) and eid = -1.
 #pragma once
 #include <stdint.h>
 
@@ -22,11 +24,12 @@
 namespace pgc {
 
 struct SlotClass {
-  const int32_t* off;  // [(L+1)*nChunks + 1]
-  const int32_t* src;  // source vertex index inside its layer, -1 = padding
-  const void* cost;    // float (q-units, exact) or double (raw)
-  const float* time;   // nullptr: constant time
-  const int32_t* eid;  // product-graph edge id
+  const int32_t* src;    // source vertex index inside its layer; padding = npl (sentinel)
+  const void* cost;      // float (q-units, exact) or double (raw)
+  const float* time;     // nullptr: constant time
+  const int32_t* eid;    // product-graph edge id (-1 for padding)
+  int32_t w;             // slots per lane of every chunk (the class's maximum in-degree)
+  int32_t lstride;       // slots per layer = nChunks * w * 64
 };
 
 enum JobMode : int32_t { MODE_RATIO = 0, MODE_UPPER_BOUND = 1, MODE_MIN_COST = 2 };
@@ -69,6 +72,8 @@ struct JobResult {
   int64_t bf_calls;
   int64_t lawler_iters;
   int64_t relax;        // candidate relaxations (finite source value)
+  // device wall-clock ticks (wall_clock64, 100 MHz) spent per phase
+  int64_t t_sweep, t_fold, t_chase, t_walk, t_extract;
 };
 
 struct SolverJob {
@@ -89,6 +94,7 @@ struct SolverJob {
   const int32_t* branch;  // [npl] branch id per layer-0 vertex
   int32_t* pred;          // [(L+1)*K*npl] predecessor (expanded node id)
   // per-sweep scratch
+  double* backup;         // [npl] layer-0 values at the start of a sweep
   int32_t* wnext;         // [npl]
   int32_t* wlen;          // [npl]
   int32_t* chain;         // [npl*chain_stride] original node ids along each wrap

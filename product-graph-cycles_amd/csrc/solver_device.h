@@ -1,0 +1,1031 @@
+// Layer-synchronous Bellman-Ford / Lawler solver kernels for gfx950.
+//
+// One workgroup (1024 threads, 16 wave64s) owns one job (one product graph,
+// one branch) for the whole Lawler subroutine: every sweep, fold, negative
+// cycle walk, cycle extraction and lambda update happens on the device; the
+// host only runs the branch-and-bound bookkeeping between launches.
+//
+// Semantics follow the reference's CPU path so that results are identical:
+//   mooreBellmanFordSubroutineCPU  cost_time_ratio_solver.cpp:251-542
+//   lawlerSoubroutine              cost_time_ratio_solver.cpp:558-635
+//   findInitalUpperBound           cost_time_ratio_solver.cpp:652-707
+//   computeRatioForCycle           cost_time_ratio_solver.cpp:88-133
+//   dijkstra BF + lower bounds     dijkstra_solver.cpp:203-387, dijkstra_bnb_subroutine.cu:366-393
+// The distance table keeps the reference's row layout ((2K) rolling rows:
+// even layers in rows [0,K), odd layers in rows [K,2K)) in LDS, so that the
+// reference's fold row (numLayers % 2) and the stale layer-0 depth rows behave
+// identically; the zero-layer backup row lives in registers of the vertex
+// owners. The per-source minimum over depths of the previous layer is kept
+// in an extra LDS row so that an interlayer relaxation reads one value; the
+// depth of the winning source is then recovered exactly (first depth whose
+// rounded sum equals the minimum, the reference's loop order).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "pgc_internal.h"
+
+namespace pgc {
+
+#define PGC_INF __builtin_huge_val()
+
+// The job's arrays are reached through pointers stored in device memory, so
+// the compiler cannot infer their address space and would emit FLAT
+// accesses, which also count against the LDS counter: every LDS wait and
+// barrier would then wait for all outstanding global loads. Hot paths
+// access them through explicit global-address-space pointers instead.
+#define PGC_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ PGC_GLOBAL T* gp(T* p) {
+  return (PGC_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const PGC_GLOBAL T* gp(const T* p) {
+  return (const PGC_GLOBAL T*)p;
+}
+
+__device__ __forceinline__ int orig_index(int64_t npl, int64_t K, int64_t p) {
+  const int64_t per = npl * K;
+  const int64_t layer = p / per;
+  const int64_t rem = p - layer * per;
+  const int64_t depth = rem / npl;
+  return (int)(rem - depth * npl + layer * npl);
+}
+
+// std::round then double->int as on x86-64 (INT_MIN outside the int range)
+__device__ __forceinline__ int round_to_int(double w) {
+  const double r = round(w);
+  if (!(r >= -2147483648.0 && r < 2147483648.0)) return (int)0x80000000;
+  return (int)r;
+}
+
+struct Shared {
+  int orCount[2];  // barrier_or counters, alternating by parity
+  int i0, i1, i2, i3;
+  double d0, d1, d2;
+  int status;
+  int numInBranch;
+  int indexNode;
+  int pickV;
+  double pickD;
+  int redI[PGC_BLOCK / PGC_WAVE];
+  double redD[PGC_BLOCK / PGC_WAVE];
+};
+
+// Block-wide OR of a predicate with a single barrier: waves with a true
+// predicate bump the counter of the current parity; a thread compares the
+// counter after the barrier with its previous reading. Counters alternate
+// by parity so a fast wave's next increment cannot reach a counter a slow
+// wave has still to read (that wave must first pass the next barrier).
+struct BarrierOr {
+  int prev0 = 0, prev1 = 0;
+  bool odd = false;
+  __device__ __forceinline__ bool operator()(Shared& sh, bool pred) {
+    int* cnt = odd ? &sh.orCount[1] : &sh.orCount[0];
+    const unsigned long long b = __ballot(pred);
+    if (b && (threadIdx.x & 63) == 0) atomicAdd(cnt, 1);
+    __syncthreads();
+    const int c = *cnt;
+    bool any;
+    if (odd) {
+      any = c != prev1;
+      prev1 = c;
+    } else {
+      any = c != prev0;
+      prev0 = c;
+    }
+    odd = !odd;
+    return any;
+  }
+};
+
+__device__ __forceinline__ uint64_t wclock() { return wall_clock64(); }
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ int block_sum(Shared& sh, int x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.redI[w] = x;
+  __syncthreads();
+  int s = 0;
+  for (int i = 0; i < (PGC_BLOCK / PGC_WAVE); ++i) s += sh.redI[i];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ int block_max(Shared& sh, int x) {
+  for (int o = 32; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o));
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh.redI[w] = x;
+  __syncthreads();
+  int s = sh.redI[0];
+  for (int i = 1; i < (PGC_BLOCK / PGC_WAVE); ++i) s = max(s, sh.redI[i]);
+  __syncthreads();
+  return s;
+}
+
+// lexicographic minimum of (key, v); v = INT_MAX means none
+__device__ __forceinline__ void block_min_key(Shared& sh, double& key, int& v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double k2 = __shfl_xor(key, o);
+    const int v2 = __shfl_xor(v, o);
+    if (k2 < key || (k2 == key && v2 < v)) {
+      key = k2;
+      v = v2;
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    sh.redD[w] = key;
+    sh.redI[w] = v;
+  }
+  __syncthreads();
+  key = sh.redD[0];
+  v = sh.redI[0];
+  for (int i = 1; i < (PGC_BLOCK / PGC_WAVE); ++i) {
+    if (sh.redD[i] < key || (sh.redD[i] == key && sh.redI[i] < v)) {
+      key = sh.redD[i];
+      v = sh.redI[i];
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------ edge lookup
+// Edge joining consecutive cycle nodes a -> b (original node ids): the first
+// in-edge of b (remapped to the closing copy when it wraps) whose source is a.
+template <typename CT, bool HAS_TIME>
+__device__ __forceinline__ bool lookup_edge(const SolverJob& J, int a, int b, int& eid, double& c, double& t) {
+  const int64_t npl = J.npl;
+  int64_t bb = b;
+  if (bb < npl && a > npl) bb += J.numNodes;
+  if (a < 0 || bb < 0 || bb >= J.numNodes + npl) return false;
+  const int lb = (int)(bb / npl);
+  const int vi = (int)(bb - (int64_t)lb * npl);
+  const SlotClass* S;
+  int sl;
+  if ((int64_t)a / npl == lb) {
+    S = &J.intra;
+    sl = (int)(a - (int64_t)lb * npl);
+  } else {
+    S = &J.inter;
+    sl = (int)(a - (int64_t)(lb - 1) * npl);
+    if (lb == 0 || sl < 0 || sl >= npl) return false;
+  }
+  const int ch = vi >> 6, lane = vi & 63;
+  const int width = S->w;
+  const int base = lb * S->lstride + ch * width * 64;
+  for (int j = 0; j < width; ++j) {
+    const int idx = base + j * 64 + lane;
+    const int s = gp(S->src)[idx];
+    if (s < 0 || s >= npl) break;
+    if (s == sl) {
+      eid = gp(S->eid)[idx];
+      c = (double)gp((const CT*)S->cost)[idx];
+      t = HAS_TIME ? (double)gp(S->time)[idx] : J.time_const;
+      return true;
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------- kernel
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
+struct Solver {
+  const SolverJob& J;
+  double* D;     // [2K][ds] distance rows; entry npl of every row is +inf (padding slots point there)
+  double* Dmin;  // [ds]
+  Shared& sh;
+  int npl, ds, K, L, nC, wave, lane, nw;
+  BarrierOr bor;
+  uint64_t tSweep = 0, tFold = 0, tChase = 0, tWalk = 0, tExtract = 0;
+
+  __device__ __forceinline__ Solver(const SolverJob& j, double* d, double* dm, Shared& s)
+      : J(j), D(d), Dmin(dm), sh(s) {
+    npl = J.npl;
+    ds = npl + 1;
+    K = J.K;
+    L = J.L;
+    nC = J.nChunks;
+    wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    lane = threadIdx.x & 63;
+    nw = PGC_BLOCK / PGC_WAVE;
+  }
+
+  // geometry of owned chunk i (wave-uniform: scalar loads)
+  __device__ __forceinline__ int chunk_of(int i) const { return wave + i * nw; }
+
+  __device__ __forceinline__ int vert(int i) const {
+    const int c = wave + i * nw;
+    if (c >= nC) return -1;
+    const int v = c * 64 + lane;
+    return v < npl ? v : -1;
+  }
+
+  // Slots of one owned chunk (pass i) of a layer block: the class width w
+  // (wave-uniform, the same for every chunk) gives independent loads issued
+  // back to back with no table lookup in between. Padding slots point at the
+  // sentinel vertex npl (distance +inf); slots j >= w are set to it.
+  __device__ __forceinline__ void load_pass(const SlotClass& S, int b, int w, int (&s)[W], CT (&c)[W],
+                                            float (&t)[W]) const {
+    const PGC_GLOBAL int* __restrict__ src = gp(S.src);
+    const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)S.cost);
+    const PGC_GLOBAL float* __restrict__ tim = gp(S.time);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      if (j < w) {
+        s[j] = src[b + j * 64];
+        c[j] = cost[b + j * 64];
+        if (HAS_TIME) t[j] = tim[b + j * 64];
+      } else {
+        s[j] = npl;
+        c[j] = 0;
+        if (HAS_TIME) t[j] = 0.f;
+      }
+    }
+  }
+
+  // c - lambda*t; rt = lambda*tconst (the same product for every edge when
+  // the time is constant, so hoisting it is exact)
+  __device__ __forceinline__ double weight(CT c, float t, double ratio, double rt) const {
+    if (MINCOST) return (double)c;
+    return __dsub_rn((double)c, HAS_TIME ? __dmul_rn(ratio, (double)t) : rt);
+  }
+
+  // Interlayer step of one owned vertex (depth 0 of layer `layer`).
+  __device__ __forceinline__ bool inter_vertex(int i, int v, int layer, int cur, int prev, const int (&s)[W],
+                                               const CT (&c)[W], const float (&t)[W], double ratio, double rt,
+                                               double ub, double (&dmin_r)[MAXC]) {
+    double best = PGC_INF;
+    int bs = npl;
+    CT bc = 0;
+    float bt = 0;
+    const double* Dp = (K == 1) ? D + prev * ds : Dmin;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double nv = Dp[s[j]] + weight(c[j], t[j], ratio, rt);
+      const bool better = nv < best;
+      best = better ? nv : best;
+      bs = better ? s[j] : bs;
+      bc = better ? c[j] : bc;
+      if (HAS_TIME) bt = better ? t[j] : bt;
+    }
+    double res = PGC_INF;
+    bool imp = false;
+    if (best < PGC_INF && best < ub) {
+      int ks = 0;
+      if (K > 1) {
+        const double bw = weight(bc, bt, ratio, rt);
+        for (int kk = 0; kk < K; ++kk) {
+          if (D[(prev + kk) * ds + bs] + bw == best) {
+            ks = kk;
+            break;
+          }
+        }
+      }
+      gp(J.pred)[(int64_t)layer * npl * K + v] = bs + (layer - 1) * npl * K + ks * npl;
+      res = best;
+      imp = true;
+    }
+    D[cur * ds + v] = res;
+    dmin_r[i] = res;
+    return imp;
+  }
+
+  // Intralayer step (depth k) of one owned vertex.
+  __device__ __forceinline__ bool intra_vertex(int i, int v, int layer, int cur, int k, const int (&s)[W],
+                                               const CT (&c)[W], const float (&t)[W], double ratio, double rt,
+                                               double ub, double (&dmin_r)[MAXC]) {
+    double best = PGC_INF;
+    int bs = npl;
+    const double* Dk = D + (cur + k - 1) * ds;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double nv = Dk[s[j]] + weight(c[j], t[j], ratio, rt);
+      const bool better = nv < best;
+      best = better ? nv : best;
+      bs = better ? s[j] : bs;
+    }
+    const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
+    double res = curv;
+    bool imp = false;
+    if (best < curv && best < ub) {
+      gp(J.pred)[(int64_t)layer * npl * K + (int64_t)k * npl + v] = bs + layer * npl * K + (k - 1) * npl;
+      res = best;
+      imp = true;
+    }
+    D[(cur + k) * ds + v] = res;
+    dmin_r[i] = res < dmin_r[i] ? res : dmin_r[i];
+    if (k == K - 1) Dmin[v] = dmin_r[i];
+    return imp;
+  }
+
+  // One step over the owned chunks: the slots of every pass are loaded up
+  // front (independent loads, one memory latency per step), then computed.
+  template <bool INTER>
+  __device__ __forceinline__ bool step(int layer, int cur, int prev, int k, double ratio, double rt, double ub,
+                                       double (&dmin_r)[MAXC]) {
+    const SlotClass& S = INTER ? J.inter : J.intra;
+    const int w = S.w;
+    const int base = layer * S.lstride + lane;
+    int s[MAXC][W];
+    CT c[MAXC][W];
+    float t[MAXC][W];
+    bool imp = false;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int ci = chunk_of(i);
+      // passes past the last chunk read chunk 0 (in bounds) and are discarded
+      load_pass(S, base + (ci < nC ? ci * w * 64 : 0), w, s[i], c[i], t[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int v = vert(i);
+      if (v >= 0) {
+        if (INTER)
+          imp |= inter_vertex(i, v, layer, cur, prev, s[i], c[i], t[i], ratio, rt, ub, dmin_r);
+        else
+          imp |= intra_vertex(i, v, layer, cur, k, s[i], c[i], t[i], ratio, rt, ub, dmin_r);
+      }
+    }
+    return imp;
+  }
+
+  // One pass over layers 0..L. Returns 0 when a layer 0<l<L had no improvement.
+  __device__ __forceinline__ int run_layers(double ratio, double ub) {
+    const double rt = __dmul_rn(ratio, J.time_const);
+    double dmin_r[MAXC];
+    for (int layer = 0; layer <= L; ++layer) {
+      const int cur = (layer & 1) * K;
+      const int prev = K - cur;
+      bool imp = false;
+      if (layer == 0) {
+#pragma unroll
+        for (int i = 0; i < MAXC; ++i) {
+          const int v = vert(i);
+          if (v >= 0) dmin_r[i] = D[v];
+        }
+      } else {
+        imp = step<true>(layer, cur, prev, 0, ratio, rt, ub, dmin_r);
+        if (K == 1) {
+          const bool any = bor(sh, imp);
+          if (!any && layer < L) return 0;
+          continue;
+        }
+        __syncthreads();
+      }
+      for (int k = 1; k < K; ++k) {
+        imp |= step<false>(layer, cur, prev, k, ratio, rt, ub, dmin_r);
+        if (k == K - 1) {
+          const bool any = bor(sh, imp);
+          if (layer > 0 && layer < L && !any) return 0;
+        } else {
+          __syncthreads();
+        }
+      }
+    }
+    return 1;
+  }
+
+  // D <- inf, branch sources <- 0, pred of sources <- -1; returns #sources.
+  __device__ __forceinline__ int setup(int bid, int onlyV) {
+    for (int i = threadIdx.x; i < 2 * K * ds; i += PGC_BLOCK) D[i] = PGC_INF;
+    for (int i = threadIdx.x; i < ds; i += PGC_BLOCK) Dmin[i] = PGC_INF;
+    __syncthreads();
+    int cnt = 0, last = -1;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int v = vert(i);
+      if (v < 0) continue;
+      const bool in = onlyV >= 0 ? (v == onlyV) : (gp(J.branch)[v] == bid);
+      if (in) {
+        D[v] = 0;
+        gp(J.pred)[v] = -1;
+        cnt++;
+        last = max(last, v);
+      }
+    }
+    const int n = block_sum(sh, cnt);
+    const int lv = block_max(sh, last);
+    if (threadIdx.x == 0) {
+      sh.numInBranch = n;
+      sh.indexNode = lv;
+    }
+    __syncthreads();
+    return n;
+  }
+
+  __device__ __forceinline__ bool in_branch(int v, int bid, int onlyV) const {
+    return onlyV >= 0 ? (v == onlyV) : (gp(J.branch)[v] == bid);
+  }
+
+  // Fold of the closing layer into layer 0; returns whether any value improved.
+  __device__ __forceinline__ bool fold(int bid, int onlyV) {
+    const int lastRow = L % 2;
+    bool imp = false;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int v = vert(i);
+      if (v < 0 || !in_branch(v, bid, onlyV)) continue;
+      const double dl = D[lastRow * ds + v];
+      const double bk = gp(J.backup)[v];
+      if (dl < bk) {
+        imp = true;
+        D[v] = dl;
+        gp(J.pred)[v] = gp(J.pred)[(int64_t)npl * K * L + v];
+      } else {
+        D[v] = bk;
+      }
+    }
+    return bor(sh, imp);
+  }
+
+  // Chase every layer-0 vertex's predecessor chain back to the previous
+  // layer-0 vertex (one wrap), recording the original node ids.
+  __device__ __forceinline__ void chase_wraps() {
+    const int stride = J.chain_stride;
+    int p[MAXC], pos[MAXC];
+    bool act[MAXC];
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int v = vert(i);
+      act[i] = v >= 0;
+      pos[i] = 0;
+      p[i] = act[i] ? gp(J.pred)[v] : -1;
+      if (act[i] && p[i] == -1) {
+        gp(J.wnext)[v] = -1;
+        gp(J.wlen)[v] = 0;
+        act[i] = false;
+      }
+    }
+    bool any = true;
+    while (any) {
+      any = false;
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) {
+        if (!act[i]) continue;
+        const int v = vert(i);
+        const int pp = p[i];
+        if (pp < npl) {
+          gp(J.chain)[(int64_t)v * stride + pos[i]] = pp;
+          pos[i]++;
+          gp(J.wnext)[v] = pp;
+          gp(J.wlen)[v] = pos[i];
+          act[i] = false;
+          continue;
+        }
+        if (pos[i] >= stride - 1) {
+          gp(J.wnext)[v] = -2;
+          gp(J.wlen)[v] = pos[i];
+          act[i] = false;
+          continue;
+        }
+        gp(J.chain)[(int64_t)v * stride + pos[i]] = orig_index(npl, K, pp);
+        pos[i]++;
+        const int np = gp(J.pred)[pp];
+        p[i] = np;
+        if (np == -1) {
+          gp(J.wnext)[v] = -1;
+          gp(J.wlen)[v] = pos[i];
+          act[i] = false;
+          continue;
+        }
+        any = true;
+      }
+    }
+  }
+
+  // Walk of the reference (cost_time_ratio_solver.cpp:443-489) expressed on
+  // the wrap map W: first repeated layer-0 vertex from v and its step count.
+  __device__ __forceinline__ bool walk_from(int v, int64_t capSteps, int& vstar) const {
+    const PGC_GLOBAL int* Wn = gp(J.wnext);
+    const PGC_GLOBAL int* Ln = gp(J.wlen);
+    const int lim = 4 * npl + 8;
+    int t = Wn[v];
+    if (t < 0) return false;
+    int h = Wn[t];
+    int it = 0;
+    while (t != h && h >= 0 && it < lim) {
+      t = Wn[t];
+      h = Wn[h];
+      if (h >= 0) h = Wn[h];
+      ++it;
+    }
+    if (h < 0 || it >= lim) return false;
+    int a = v;
+    it = 0;
+    while (a != h && it < lim) {
+      a = Wn[a];
+      h = Wn[h];
+      ++it;
+    }
+    const int e = a;
+    int64_t P = 0;
+    int x = v;
+    it = 0;
+    while (x != e && it < lim) {
+      P += Ln[x];
+      x = Wn[x];
+      ++it;
+    }
+    x = e;
+    do {
+      P += Ln[x];
+      x = Wn[x];
+      ++it;
+    } while (x != e && it < 2 * lim);
+    if (P > capSteps) return false;
+    vstar = e;
+    return true;
+  }
+
+  // sequential sums over a cycle's pairs (thread 0 only)
+  __device__ __forceinline__ void pair_sums(const double* c_, const double* t_, int m, double ratio, double& cs,
+                                            double& ts, double& ws) const {
+    const PGC_GLOBAL double* c = gp(c_);
+    const PGC_GLOBAL double* t = gp(t_);
+    cs = 0;
+    ts = 0;
+    ws = 0;
+    for (int i = 0; i < m; ++i) {
+      const double ci = c[i], ti = t[i];
+      cs += ci;
+      ts += ti;
+      ws += __dsub_rn(ci, __dmul_rn(ratio, ti));
+    }
+  }
+
+  // edge lookups for tmp_nodes[0..n) -> tmp_eid/c/t[0..n-1); returns ok (block-uniform)
+  __device__ __forceinline__ bool lookup_pairs(int n) {
+    bool bad = false;
+    for (int i = threadIdx.x; i + 1 < n; i += PGC_BLOCK) {
+      int eid = -1;
+      double c = 0, t = 0;
+      if (!lookup_edge<CT, HAS_TIME>(J, gp(J.tmp_nodes)[i], gp(J.tmp_nodes)[i + 1], eid, c, t)) bad = true;
+      gp(J.tmp_eid)[i] = eid;
+      gp(J.tmp_c)[i] = c;
+      gp(J.tmp_t)[i] = t;
+    }
+    return !bor(sh, bad);
+  }
+
+  __device__ __forceinline__ void copy_tmp_to_neg(int n) {
+    for (int i = threadIdx.x; i < n; i += PGC_BLOCK) {
+      gp(J.neg_nodes)[i] = gp(J.tmp_nodes)[i];
+      if (i + 1 < n) {
+        gp(J.neg_eid)[i] = gp(J.tmp_eid)[i];
+        gp(J.neg_c)[i] = gp(J.tmp_c)[i];
+        gp(J.neg_t)[i] = gp(J.tmp_t)[i];
+      }
+    }
+    if (threadIdx.x == 0) *gp(J.neg_len) = n;
+    __syncthreads();
+  }
+
+  // ZERO_CYCLE if the current cycle has (rounded) zero reweighted cost, else NO_NEGATIVE_CYCLE
+  __device__ __forceinline__ int zero_or_none(double ratio) {
+    if (threadIdx.x == 0) {
+      const int n = *gp(J.neg_len);
+      double cs, ts, ws;
+      pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
+      sh.status = round_to_int(ws) == 0 ? ST_ZERO_CYCLE : ST_NO_NEGATIVE_CYCLE;
+    }
+    __syncthreads();
+    return sh.status;
+  }
+
+  // mooreBellmanFordSubroutine (ratio and upper-bound modes).
+  // Returns status; on FOUND the new cycle is in neg_* and its ratio in sh.d0.
+  __device__ __forceinline__ int bellman_ford(double ratio, bool zeroSearch, bool ubSearch, int bid, int onlyV, int64_t& sweeps,
+                              int& err) {
+    const int n = setup(bid, onlyV);
+    if (n == 0) {
+      err = ERR_EMPTY_BRANCH;
+      return ST_NO_NEGATIVE_CYCLE;
+    }
+    const double ub = PGC_INF;
+    int64_t iter = 0;
+    while (true) {
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) {
+        const int v = vert(i);
+        if (v >= 0) gp(J.backup)[v] = D[v];
+      }
+      sweeps++;
+      uint64_t t0 = wclock();
+      const int complete = run_layers(ratio, ub);
+      uint64_t t1 = wclock();
+      tSweep += t1 - t0;
+      if (!complete) return zero_or_none(ratio);
+
+      if (ubSearch) {
+        // path from the closing copy of the start vertex back to it (:370-397)
+        if (threadIdx.x == 0) {
+          const int start = sh.indexNode;
+          const int64_t maxLength = (int64_t)L * K;
+          int cnt = 0;
+          int64_t firstV = start + (int64_t)npl * L * K;
+          gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, firstV);
+          int64_t p = gp(J.pred)[firstV];
+          gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, p);
+          for (int64_t kk = maxLength - 1; kk > 0 && cnt < J.cyc_cap; --kk) {
+            if (p == -1) break;
+            p = gp(J.pred)[p];
+            gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, p);
+            if (p == start) break;
+          }
+          for (int i = 0, j = cnt - 1; i < j; ++i, --j) {
+            const int tmp = gp(J.tmp_nodes)[i];
+            gp(J.tmp_nodes)[i] = gp(J.tmp_nodes)[j];
+            gp(J.tmp_nodes)[j] = tmp;
+          }
+          sh.i0 = cnt;
+        }
+        __syncthreads();
+        const int cnt = sh.i0;
+        if (!lookup_pairs(cnt)) err = ERR_EDGE_LOOKUP;
+        copy_tmp_to_neg(cnt);
+        if (threadIdx.x == 0) {
+          double cs, ts, ws;
+          pair_sums(J.neg_c, J.neg_t, cnt - 1, ratio, cs, ts, ws);
+          const int w = round_to_int(ws);
+          sh.status = w == 0 ? ST_ZERO_CYCLE : (w < 0 ? ST_FOUND_NEGATIVE_CYCLE : ST_NO_NEGATIVE_CYCLE);
+        }
+        __syncthreads();
+        return sh.status;
+      }
+
+      t0 = wclock();
+      const bool folded = fold(bid, onlyV);
+      t1 = wclock();
+      tFold += t1 - t0;
+      if (!folded) return zero_or_none(ratio);
+
+      chase_wraps();
+      __syncthreads();
+      t0 = wclock();
+      tChase += t0 - t1;
+
+      // candidates: branch vertices with a negative (or, searching zero
+      // cycles, |D| < tol) layer-0 value
+      const bool pickLast = (J.flags & FLAG_PICK_LAST_VERTEX) != 0;
+      const int64_t capSteps = J.numNodes - 1;
+      double bestKey = PGC_INF;
+      int bestV = 0x7fffffff;
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) {
+        const int v = vert(i);
+        if (v < 0 || !in_branch(v, bid, onlyV)) continue;
+        const double lb = D[v];
+        if (!zeroSearch && lb >= 0) continue;
+        if (zeroSearch && (lb <= -J.tol || lb >= J.tol)) continue;
+        int vs;
+        if (!walk_from(v, capSteps, vs)) continue;
+        const double key = pickLast ? -(double)v : lb;
+        if (key < bestKey || (key == bestKey && v < bestV)) {
+          bestKey = key;
+          bestV = v;
+        }
+      }
+      block_min_key(sh, bestKey, bestV);
+      t1 = wclock();
+      tWalk += t1 - t0;
+      if (bestV != 0x7fffffff) {
+        // extract the cycle through v* (first repeated vertex of bestV's walk)
+        if (threadIdx.x == 0) {
+          int vs = -1;
+          walk_from(bestV, capSteps, vs);
+          int64_t m = 0;
+          int x = vs, it = 0;
+          do {
+            m += gp(J.wlen)[x];
+            x = gp(J.wnext)[x];
+          } while (x != vs && ++it < 4 * npl + 8);
+          sh.i0 = vs;
+          sh.i1 = (m + 1 <= J.cyc_cap) ? (int)m : -1;
+        }
+        __syncthreads();
+        const int vs = sh.i0;
+        const int m = sh.i1;
+        if (m < 0) {
+          err = ERR_CYCLE_TOO_LONG;
+          return ST_NO_NEGATIVE_CYCLE;
+        }
+        if (wave == 0) {
+          int x = vs, offp = 0, guard = 0;
+          do {
+            const int len = gp(J.wlen)[x];
+            for (int j = lane; j < len && offp + j < m; j += 64)
+              gp(J.tmp_nodes)[m - 1 - (offp + j)] = gp(J.chain)[(int64_t)x * J.chain_stride + j];
+            offp += len;
+            x = gp(J.wnext)[x];
+          } while (x != vs && x >= 0 && offp < m && ++guard < 4 * npl + 8);
+          if (lane == 0) gp(J.tmp_nodes)[m] = vs + (int)J.numNodes;
+        }
+        __syncthreads();
+        const int nn = m + 1;
+        if (!lookup_pairs(nn)) {
+          err = ERR_EDGE_LOOKUP;
+          return ST_NO_NEGATIVE_CYCLE;
+        }
+        if (threadIdx.x == 0) {
+          double cs, ts, ws;
+          pair_sums(J.tmp_c, J.tmp_t, nn - 1, ratio, cs, ts, ws);
+          sh.i2 = round_to_int(ws);
+          sh.d0 = cs / ts;
+        }
+        __syncthreads();
+        tExtract += wclock() - t1;
+        const int w = sh.i2;
+        if (w < 0) {
+          copy_tmp_to_neg(nn);
+          return ST_FOUND_NEGATIVE_CYCLE;
+        }
+        // a zero-cost cycle ends the sweep loop through its condition and
+        // reaches the final NO_NEGATIVE_CYCLE return (:506-518, :282)
+        if (w == 0) return ST_NO_NEGATIVE_CYCLE;
+        return zero_or_none(ratio);
+      }
+      iter++;
+      if (iter >= J.numNodes - 1) return ST_NO_NEGATIVE_CYCLE;
+      if (sweeps >= J.max_sweeps) {
+        err = ERR_SWEEP_CAP;
+        return ST_NO_NEGATIVE_CYCLE;
+      }
+    }
+  }
+
+  // wraps and zero-layer hits of the current cycle (:606-634); thread 0
+  __device__ __forceinline__ void wraps_and_hits(int& numWraps, int& nHits) const {
+    const int n = *gp(J.neg_len);
+    numWraps = 0;
+    nHits = 0;
+    bool entered = true, left = false;
+    for (int i = 0; i < n; ++i) {
+      const int v = gp(J.neg_nodes)[i];
+      if (left && v < npl) {
+        entered = true;
+        left = false;
+      }
+      if (entered) {
+        if (nHits < npl) gp(J.hits)[nHits] = v;
+        nHits++;
+        entered = false;
+      }
+      if (v >= npl && !left) {
+        numWraps++;
+        left = true;
+      }
+    }
+  }
+
+  // ------------------------------------------------------- modes
+  __device__ __forceinline__ void lawler() {
+    int64_t sweeps = 0, bfc = 0, its = 0;
+    int err = ERR_NONE;
+    double ratio = J.ratio_in;
+    double oldRatio = ratio;
+    const double tol = J.tol;
+    int i = 0;
+    int lastStatus = ST_NO_NEGATIVE_CYCLE;
+    while (true) {
+      its++;
+      bfc++;
+      const int st = bellman_ford(ratio, false, false, J.branchId, -1, sweeps, err);
+      lastStatus = st;
+      if (err != ERR_NONE) break;
+      if (st == ST_ZERO_CYCLE || st == ST_NO_NEGATIVE_CYCLE) break;
+      ratio = sh.d0;
+      if (fabs(oldRatio - ratio) < tol) break;
+      if (ratio - tol > oldRatio) break;
+      oldRatio = ratio;
+      if (i > 1000) break;
+      i++;
+    }
+    if (threadIdx.x == 0) {
+      int nw_ = 0, nh = 0;
+      wraps_and_hits(nw_, nh);
+      PGC_GLOBAL JobResult& r = *gp(J.res);
+      r.status = lastStatus;
+      r.error = err;
+      r.ratio = ratio;
+      r.ratio_ub = 0;
+      r.num_wraps = nw_;
+      r.num_in_branch = sh.numInBranch;
+      r.n_hits = nh;
+      r.neg_len = *gp(J.neg_len);
+      r.best_v = -1;
+      r.sweeps = sweeps;
+      r.bf_calls = bfc;
+      r.lawler_iters = its;
+      r.relax = 0;
+      r.t_sweep = tSweep;
+      r.t_fold = tFold;
+      r.t_chase = tChase;
+      r.t_walk = tWalk;
+      r.t_extract = tExtract;
+    }
+  }
+
+  // findInitalUpperBound (:652-707)
+  __device__ __forceinline__ void upper_bound() {
+    // start vertex: minimal float mean of the in-edge costs of its layer-1
+    // copy (accumulated in ascending edge id), among vertices with > 3 in-edges
+    double bestKey = PGC_INF;
+    int bestV = 0x7fffffff;
+    for (int v = threadIdx.x; v < npl; v += PGC_BLOCK) {
+      const int node_layer = 1;
+      float mean = 0.0f;
+      int n = 0;
+      int lastE = -1;
+      while (true) {
+        int nextE = 0x7fffffff;
+        double nextC = 0;
+        for (int cls = 0; cls < 2; ++cls) {
+          const SlotClass& S = cls == 0 ? J.intra : J.inter;
+          const int ch = v >> 6, ln = v & 63;
+          const int width = S.w;
+          const int base = node_layer * S.lstride + ch * width * 64;
+          for (int j = 0; j < width; ++j) {
+            const int idx = base + j * 64 + ln;
+            const int sv = gp(S.src)[idx];
+            if (sv < 0 || sv >= npl) break;
+            const int e = gp(S.eid)[idx];
+            if (e > lastE && e < nextE) {
+              nextE = e;
+              nextC = (double)gp((const CT*)S.cost)[idx];
+            }
+          }
+        }
+        if (nextE == 0x7fffffff) break;
+        mean = (float)((double)mean + nextC);
+        n++;
+        lastE = nextE;
+      }
+      mean = mean / (float)n;
+      if (n > 3 && mean < (float)bestKey) {
+        // strict '<' in ascending v: keep the first minimiser
+        bestKey = (double)mean;
+        bestV = v;
+      }
+    }
+    block_min_key(sh, bestKey, bestV);
+    const int start = (bestV == 0x7fffffff) ? 0 : bestV;
+    int64_t sweeps = 0;
+    int err = ERR_NONE;
+    double ratio = 10.0;
+    bellman_ford(ratio, true, true, 0, start, sweeps, err);
+    if (threadIdx.x == 0) {
+      const int n = *gp(J.neg_len);
+      double cs, ts, ws;
+      pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
+      double cycleCost = (double)(float)ws;
+      int guard = 0;
+      while ((cycleCost > 0 || isnan(cycleCost)) && guard++ < 4096) {
+        ratio += ratio;
+        pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
+        cycleCost = (double)(float)ws;
+      }
+      const double r_ub = (double)(float)(cs / ts);
+      PGC_GLOBAL JobResult& r = *gp(J.res);
+      r.status = ST_NO_NEGATIVE_CYCLE;
+      r.error = err;
+      r.ratio = r_ub;
+      r.ratio_ub = r_ub;
+      r.num_wraps = 0;
+      r.num_in_branch = 1;
+      r.n_hits = 0;
+      r.neg_len = n;
+      r.best_v = start;
+      r.sweeps = sweeps;
+      r.bf_calls = 1;
+      r.lawler_iters = 0;
+      r.relax = 0;
+      r.t_sweep = tSweep;
+      r.t_fold = tFold;
+      r.t_chase = tChase;
+      r.t_walk = tWalk;
+      r.t_extract = tExtract;
+    }
+  }
+
+  // one sweep of the minimum-cost solver + lower bounds and paths
+  __device__ __forceinline__ void min_cost() {
+    int64_t sweeps = 0;
+    const int bid = J.branchId;
+    const int n = setup(bid, -1);
+    int status = ST_CAN_CONTAIN;
+    int err = ERR_NONE;
+    if (n == 0) {
+      err = ERR_EMPTY_BRANCH;
+      status = ST_CANNOT_CONTAIN;
+    } else {
+      sweeps++;
+      const uint64_t t0 = wclock();
+      if (!run_layers(0.0, J.upper_bound)) status = ST_CANNOT_CONTAIN;
+      tSweep += wclock() - t0;
+    }
+    const uint64_t tc0 = wclock();
+    if (status == ST_CAN_CONTAIN) {
+      const int lastRow = L % 2;
+      const bool cpuRules = (J.flags & FLAG_MINCOST_CPU_RULES) != 0;
+      const int stride = J.chain_stride;
+      const int64_t maxLen = (int64_t)L * K;
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) {
+        const int v = vert(i);
+        if (v < 0) continue;
+        if (gp(J.branch)[v] != bid) {
+          gp(J.lb_key)[v] = PGC_INF;
+          gp(J.back)[v] = -1;
+          gp(J.wlen)[v] = 0;
+          continue;
+        }
+        double key = D[lastRow * ds + v];
+        if (!cpuRules) key = (double)(float)key;
+        gp(J.lb_key)[v] = key;
+        int back = -1, cnt = 0;
+        if (!isinf((float)key)) {
+          int64_t p = gp(J.pred)[(int64_t)v + (int64_t)npl * L * K];
+          if (p != -1) {
+            gp(J.chain)[(int64_t)v * stride + cnt++] = orig_index(npl, K, p);
+            for (int64_t kk = maxLen; kk > 0; --kk) {
+              const int64_t q = gp(J.pred)[p];
+              if (q == -1) break;
+              p = q;
+              if (cnt < stride) gp(J.chain)[(int64_t)v * stride + cnt] = orig_index(npl, K, p);
+              cnt++;
+            }
+            back = orig_index(npl, K, p);
+          }
+        }
+        gp(J.back)[v] = back;
+        gp(J.wlen)[v] = cnt < stride ? cnt : stride;
+      }
+    }
+    __syncthreads();
+    tChase += wclock() - tc0;
+    if (threadIdx.x == 0) {
+      PGC_GLOBAL JobResult& r = *gp(J.res);
+      r.status = status;
+      r.error = err;
+      r.ratio = 0;
+      r.ratio_ub = 0;
+      r.num_wraps = 0;
+      r.num_in_branch = sh.numInBranch;
+      r.n_hits = 0;
+      r.neg_len = 0;
+      r.best_v = -1;
+      r.sweeps = sweeps;
+      r.bf_calls = 1;
+      r.lawler_iters = 0;
+      r.relax = 0;
+      r.t_sweep = tSweep;
+      r.t_fold = tFold;
+      r.t_chase = tChase;
+      r.t_walk = tWalk;
+      r.t_extract = tExtract;
+    }
+  }
+};
+
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
+__global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __restrict__ jobs) {
+  extern __shared__ double lds[];
+  __shared__ Shared sh;
+  if (threadIdx.x == 0) {
+    sh.orCount[0] = 0;
+    sh.orCount[1] = 0;
+  }
+  __syncthreads();
+  const SolverJob& J = jobs[blockIdx.x];
+  double* D = lds;
+  double* Dmin = lds + 2 * J.K * (J.npl + 1);
+  Solver<CT, HAS_TIME, MINCOST, MAXC, W> s(J, D, Dmin, sh);
+  if (MINCOST) {
+    s.min_cost();
+  } else if (J.mode == MODE_UPPER_BOUND) {
+    s.upper_bound();
+  } else {
+    s.lawler();
+  }
+}
+
+// Edge lookups for an arbitrary node list (min-cost path output).
+template <typename CT, bool HAS_TIME>
+__global__ void lookup_kernel(const SolverJob* __restrict__ jobs, const int32_t* nodes, int n, int32_t* eids) {
+  const SolverJob& J = jobs[0];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= n) return;
+  int eid = -2;
+  double c, t;
+  if (!lookup_edge<CT, HAS_TIME>(J, nodes[i], nodes[i + 1], eid, c, t)) eid = -2;
+  eids[i] = eid;
+}
+
+}  // namespace pgc
+

@@ -33,14 +33,20 @@ class pgc_result(C.Structure):
         ("relaxations", C.c_double),
         ("kernel_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("phase_ms", C.c_double * 5),
     ]
 
+    PHASES = ("sweep", "fold", "chase", "walk", "extract")
+
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "phase_ms"}
+        d["phase_ms"] = dict(zip(self.PHASES, list(self.phase_ms)))
+        return d
 
 
 EXPORTED = [
     "pgc_last_error", "pgc_abi_version", "pgc_context_create", "pgc_context_destroy", "pgc_graph_generate",
+    "pgc_graph_generate_device",
     "pgc_graph_from_arrays", "pgc_graph_set_branch_graph", "pgc_graph_info", "pgc_graph_get_product_space",
     "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
 ]
@@ -64,6 +70,7 @@ def load():
     L.pgc_context_destroy.restype = None
     L.pgc_graph_generate.argtypes = [P, P, I64, P, I64, P, I64, P, I64, P, C.c_int, C.c_int, C.c_int,
                                      C.c_char_p, C.c_char_p, P, P, D, D, D, C.POINTER(P)]
+    L.pgc_graph_generate_device.argtypes = L.pgc_graph_generate.argtypes
     L.pgc_graph_from_arrays.argtypes = [P, P, C.c_int, P, P, I64, P, C.c_int, I64, C.POINTER(P)]
     L.pgc_graph_set_branch_graph.argtypes = [P, P, P]
     L.pgc_graph_info.argtypes = [P, C.POINTER(I64), C.POINTER(I32), C.POINTER(I32), C.POINTER(I64),
@@ -147,6 +154,23 @@ class Graph:
                                    len(EY), ptr(FD), int(conjugate), int(regularising), int(prune),
                                    cost_mode.encode(), time_mode.encode(), ptr(NX), ptr(NY), float(rl[0]),
                                    float(rl[1]), float(rl[2]), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def generate_device(cls, ctx, VX, EX, VY, EY, FD, regularising=False, prune=False, cost_mode="vanilla",
+                        time_mode=""):
+        """Inputs are device tensors (torch, on this context's device): VX/VY/FD float64, EX/EY int32."""
+        L = load()
+        for t, dt in ((VX, "float64"), (VY, "float64"), (FD, "float64"), (EX, "int32"), (EY, "int32")):
+            if not t.is_cuda or str(t.dtype) != "torch." + dt or not t.is_contiguous():
+                raise ValueError("generate_device expects contiguous device tensors (float64 / int32)")
+        h = C.c_void_p()
+        check(L.pgc_graph_generate_device(ctx.handle, C.c_void_p(VX.data_ptr()), VX.shape[0],
+                                          C.c_void_p(EX.data_ptr()), EX.shape[0], C.c_void_p(VY.data_ptr()),
+                                          VY.shape[0], C.c_void_p(EY.data_ptr()), EY.shape[0],
+                                          C.c_void_p(FD.data_ptr()), 0, int(regularising), int(prune),
+                                          cost_mode.encode(), time_mode.encode(), None, None, 0.7, 0.6, 4.0,
+                                          C.byref(h)))
         return cls(ctx, h)
 
     @classmethod
