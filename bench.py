@@ -53,6 +53,20 @@ def cpu_baseline(p, K, seconds):
                       "%.1f s, oracle/pgc_oracle.cpp with OpenMP" % (n, el)}
 
 
+def copy_calibration(mib=256, reps=4):
+    """A plain device-to-device copy of `mib` MiB, `reps` times: the per-dispatch
+    FETCH_SIZE / WRITE_SIZE of this kernel against its known byte count gives the
+    counters' scale on this GPU."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,7 +74,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a device copy of known size (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path in this build "
               "(independent pairs would shard across GPUs with no collective). Use --gpus 1.", file=sys.stderr)
@@ -124,6 +143,13 @@ def main():
         for k, v in r[0][2].as_dict()["phase_ms"].items():
             phases[k] = phases.get(k, 0.0) + v / args.steps
     achieved = bytes_per_sweep * sweeps / (kms * 1e-3) / 1e9
+    # HBM bytes per solver launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE summary of this workload (scaled by the copy calibration)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "solve_kernel_c1_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["traffic_bytes_per_launch"]
     gteps = relax / (kms * 1e-3) / 1e9
 
     # parity of the last step with the CPU oracle (outside the timed region)
@@ -168,9 +194,11 @@ def main():
         "mincost_ms_per_solve": d_el / nd * 1e3,
         "parity_with_oracle": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "pgc::solve_kernel (ratio mode)",
-                     "bytes_per_sweep": bytes_per_sweep},
+                     "bytes_per_sweep": bytes_per_sweep,
+                     "algorithmic_bytes_per_launch": bytes_per_sweep * sweeps / launches,
+                     "avg_launch_ms": kms / launches},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
