@@ -113,6 +113,20 @@ void pgc_graph_destroy(pgc_graph* g);
 int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int max_depth, uint32_t flags, int32_t* edge_ids,
                     int32_t* rows, int64_t capacity, pgc_result* res);
 
+/* Complete "lawlergpu" solves of n graphs, each exactly as pgc_solve_ratio
+ * (same quantization in place, same cycle, objective and counters), with the
+ * whole solve of a graph (upper bound, branch and bound, Lawler iterations)
+ * inside one workgroup and all graphs in persistent launches (one per kernel
+ * variant) that take graphs from a device queue. Batched form of the solve loop
+ * over CostTimeRatioSolver::run (cost_time_ratio_solver.cpp:868-900); the
+ * reference has no batched entry point. max_depth[i] is graph i's depth; graph
+ * i's cycle goes to edge_ids[offsets[i] .. offsets[i+1]) and, if rows != NULL,
+ * its product-space rows to rows + (offsets[i] - offsets[0]) * ps_cols (all
+ * graphs must then have the same ps_cols). A graph may appear once per batch. */
+int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int32_t* max_depth, int64_t n,
+                          uint32_t flags, int32_t* edge_ids, int32_t* rows, const int64_t* offsets,
+                          pgc_result* res);
+
 /* Minimum-cost once-wrapping cycle with branch and bound ("dijkstragpu").
  * Replaces DijkstraSolver::run -> solveWithBnBCuda
  * (dijkstra_solver.cpp:563-579, dijkstra_solver_cuda.cpp:136-322). */

@@ -19,6 +19,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pgc.h"
@@ -45,7 +46,7 @@ hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
 // solver.hip
 size_t solver_lds_bytes(int npl, int K);
-hipError_t launch_solver(const SolverJob*, int, int, int, bool, bool, bool, size_t, hipStream_t);
+hipError_t launch_solver(const SolverJob*, int, int, int*, int, int, bool, bool, bool, size_t, hipStream_t);
 hipError_t launch_lookup(const SolverJob*, const int32_t*, int, int32_t*, bool, bool, hipStream_t);
 }  // namespace pgc
 
@@ -73,9 +74,20 @@ static hipError_t dalloc(T** p, size_t n) {
 
 struct pgc_context {
   int device = 0;
+  int num_cu = 0;
   hipStream_t stream = nullptr;
   void* select_state = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // batched solves: job table, results, output cycles (grow-only)
+  pgc::SolverJob* d_jobs = nullptr;
+  int64_t jobs_cap = 0;
+  pgc::JobResult* d_res = nullptr;
+  int64_t res_cap = 0;
+  int32_t* d_out = nullptr;
+  int64_t out_cap = 0;
+  int32_t* d_rows = nullptr;
+  int64_t rows_cap = 0;
+  int32_t* d_queue = nullptr;  // one counter per launch group
 };
 
 namespace {
@@ -93,9 +105,15 @@ struct JobScratch {
   double* backup = nullptr;
   JobResult* res = nullptr;
   SolverJob* djob = nullptr;
+  // device-resident branch and bound (batched solves)
+  double* hkey = nullptr;
+  int32_t *hval = nullptr, *hwhere = nullptr, *vslot = nullptr, *vc_eid = nullptr, *vc_len = nullptr;
+  double *vc_c = nullptr, *vc_t = nullptr;
+  int vc_cap = 0, vc_stride = 0;
   void release() {
     void* ps[] = {pred, wnext, wlen, chain, tmp_nodes, tmp_eid, neg_nodes, neg_eid, neg_len, tmp_c, tmp_t,
-                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, res, djob};
+                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, res, djob,
+                  hkey, hval, hwhere, vslot, vc_eid, vc_len, vc_c, vc_t};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     *this = JobScratch();
@@ -156,6 +174,8 @@ int pgc_context_create(int device, pgc_context** out) {
   if (e == hipSuccess) e = hipMalloc(&c->select_state, select_state_bytes());
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_queue, 64 * sizeof(int32_t));
   if (e != hipSuccess) {
     delete c;
     return fail(PGC_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
@@ -170,6 +190,9 @@ void pgc_context_destroy(pgc_context* c) {
   if (c->select_state) (void)hipFree(c->select_state);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -314,6 +337,26 @@ int ensure_scratch(pgc_graph* g, int K) {
   return PGC_OK;
 }
 
+// Queue and valid-branch cycle store of the device-resident branch and bound.
+constexpr int kValidCycles = 64;
+
+int ensure_bnb(pgc_graph* g) {
+  JobScratch& s = g->js;
+  if (s.hkey) return PGC_OK;
+  const int64_t npl = g->npl;
+  s.vc_cap = kValidCycles;
+  s.vc_stride = (int)(g->L * s.K + 2);  // a once-wrapping cycle has at most L*K+1 edges
+  HIPCHK(dalloc(&s.hkey, npl));
+  HIPCHK(dalloc(&s.hval, npl));
+  HIPCHK(dalloc(&s.hwhere, npl));
+  HIPCHK(dalloc(&s.vslot, npl));
+  HIPCHK(dalloc(&s.vc_eid, (int64_t)s.vc_cap * s.vc_stride));
+  HIPCHK(dalloc(&s.vc_c, (int64_t)s.vc_cap * s.vc_stride));
+  HIPCHK(dalloc(&s.vc_t, (int64_t)s.vc_cap * s.vc_stride));
+  HIPCHK(dalloc(&s.vc_len, s.vc_cap));
+  return PGC_OK;
+}
+
 SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   JobScratch& s = g->js;
   SolverJob j;
@@ -362,6 +405,16 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.lb_key = s.lb_key;
   j.back = s.back;
   j.res = s.res;
+  j.hkey = s.hkey;
+  j.hval = s.hval;
+  j.hwhere = s.hwhere;
+  j.vslot = s.vslot;
+  j.vc_eid = s.vc_eid;
+  j.vc_c = s.vc_c;
+  j.vc_t = s.vc_t;
+  j.vc_len = s.vc_len;
+  j.vc_cap = s.vc_cap;
+  j.vc_stride = s.vc_stride;
   return j;
 }
 
@@ -382,8 +435,8 @@ int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, doub
   const size_t lds = solver_lds_bytes((int)g->npl, j.K);
   const int maxc = (g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE);
   HIPCHK(hipEventRecord(c->ev0, st));
-  HIPCHK(launch_solver(g->js.djob, 1, maxc, g->maxw, mincost, g->d_time[0] != nullptr, g->cost_kind == 2, lds,
-                       st));
+  HIPCHK(launch_solver(g->js.djob, 1, 1, nullptr, maxc, g->maxw, mincost, g->d_time[0] != nullptr,
+                       g->cost_kind == 2, lds, st));
   HIPCHK(hipEventRecord(c->ev1, st));
   HIPCHK(hipMemcpyAsync(&out, g->js.res, sizeof(out), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -779,6 +832,175 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   res->kernel_ms = kms;
   res->total_ms = wall.ms();
   for (int i = 0; i < 5; ++i) res->phase_ms[i] = g_phase_ms[i];
+  return PGC_OK;
+}
+
+// ------------------------------------------------- batched complete solves
+extern "C++" {
+template <typename T>
+static int grow(T** p, int64_t& cap, int64_t need) {
+  if (need <= cap && *p) return PGC_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  const int64_t n = std::max<int64_t>(need, 64) * 3 / 2;
+  HIPCHK(dalloc(p, n));
+  cap = n;
+  return PGC_OK;
+}
+}  // extern "C++"
+
+int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int32_t* max_depth, int64_t n,
+                          uint32_t flags, int32_t* edge_ids, int32_t* rows, const int64_t* offsets,
+                          pgc_result* res) {
+  Timer wall;
+  if (!ctx || n < 0 || (n > 0 && (!graphs || !max_depth || !offsets || !res)))
+    return fail(PGC_ERR_ARG, "invalid arguments");
+  if (n == 0) return PGC_OK;
+  if (n > (int64_t)1 << 30) return fail(PGC_ERR_ARG, "too many graphs in one batch");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int rc;
+  for (int64_t i = 0; i < n; ++i) {
+    pgc_graph* g = graphs[i];
+    if (!g) return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " is null");
+    if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " belongs to another context");
+    if (offsets[i + 1] < offsets[i]) return fail(PGC_ERR_ARG, "offsets must be non-decreasing");
+    if (rows && g->ps_cols != graphs[0]->ps_cols)
+      return fail(PGC_ERR_ARG, "rows output needs one product-space width for the whole batch");
+    for (int64_t k = 0; k < i && k < 64; ++k)
+      if (graphs[k] == g) return fail(PGC_ERR_ARG, "a graph appears twice in one batch");
+  }
+  // per graph: in-place quantization, layout, fp32 slot costs, scratch
+  std::vector<SolverJob> jobs(n);
+  for (int64_t i = 0; i < n; ++i) {
+    pgc_graph* g = graphs[i];
+    const int K = max_depth[i];
+    if ((rc = check_solvable(g, K))) return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
+    for (int c = 0; c < g->ncols; ++c)
+      HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
+    if ((rc = build_layout(g))) return rc;
+    if ((rc = check_width(g))) return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
+    if ((rc = fill_costs(g, 1))) return rc;
+    if ((rc = ensure_scratch(g, K))) return rc;
+    if ((rc = ensure_bnb(g))) return rc;
+    HIPCHK(hipMemsetAsync(g->js.pred, 0xff, g->js.predN * 4, st));
+    HIPCHK(hipMemsetAsync(g->js.neg_len, 0, 4, st));
+    jobs[i] = make_job(g, K, MODE_FULL_RATIO, flags);
+  }
+  // batch tables: results and output cycles in one arena each
+  if ((rc = grow(&ctx->d_jobs, ctx->jobs_cap, n))) return rc;
+  if ((rc = grow(&ctx->d_res, ctx->res_cap, n))) return rc;
+  const int64_t outN = std::max<int64_t>(offsets[n] - offsets[0], 1);
+  if ((rc = grow(&ctx->d_out, ctx->out_cap, outN))) return rc;
+  for (int64_t i = 0; i < n; ++i) {
+    jobs[i].res = ctx->d_res + i;
+    jobs[i].out_eid = ctx->d_out + (offsets[i] - offsets[0]);
+    jobs[i].out_cap = (int32_t)std::min<int64_t>(offsets[i + 1] - offsets[i], INT32_MAX);
+  }
+  // launch groups: jobs sharing one kernel instantiation run in one
+  // persistent launch, ordered largest first inside the group
+  struct Key {
+    int maxc, maxw;
+    bool time;
+    bool operator<(const Key& o) const {
+      return std::tie(maxc, maxw, time) < std::tie(o.maxc, o.maxw, o.time);
+    }
+  };
+  std::map<Key, std::vector<int64_t>> groups;
+  for (int64_t i = 0; i < n; ++i) {
+    pgc_graph* g = graphs[i];
+    const int maxc = (g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE);
+    const int mc = maxc <= 4 ? maxc : 8;
+    const int mw = g->maxw <= 8 ? 8 : (g->maxw <= 16 ? 16 : 32);
+    groups[{mc, mw, g->d_time[0] != nullptr}].push_back(i);
+  }
+  if (groups.size() > 64) return fail(PGC_ERR_UNSUPPORTED, "too many launch groups");
+  std::vector<SolverJob> ordered;
+  ordered.reserve(n);
+  struct Launch {
+    Key k;
+    int64_t first, count;
+    size_t lds;
+  };
+  std::vector<Launch> launches;
+  for (auto& kv : groups) {
+    auto& idx = kv.second;
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+      return graphs[a]->E * max_depth[a] > graphs[b]->E * max_depth[b];
+    });
+    size_t lds = 0;
+    for (int64_t i : idx) lds = std::max(lds, solver_lds_bytes((int)graphs[i]->npl, max_depth[i]));
+    launches.push_back({kv.first, (int64_t)ordered.size(), (int64_t)idx.size(), lds});
+    for (int64_t i : idx) ordered.push_back(jobs[i]);
+  }
+  HIPCHK(hipMemcpyAsync(ctx->d_jobs, ordered.data(), n * sizeof(SolverJob), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(ctx->d_queue, 0, 64 * sizeof(int32_t), st));
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  for (size_t li = 0; li < launches.size(); ++li) {
+    const Launch& L = launches[li];
+    const int grid = (int)std::min<int64_t>(L.count, std::max(1, ctx->num_cu));
+    HIPCHK(launch_solver(ctx->d_jobs + L.first, (int)L.count, grid, ctx->d_queue + li, L.k.maxc, L.k.maxw, false,
+                         L.k.time, false, L.lds, st));
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, st));
+  std::vector<JobResult> out(n);
+  HIPCHK(hipMemcpyAsync(out.data(), ctx->d_res, n * sizeof(JobResult), hipMemcpyDeviceToHost, st));
+  if (edge_ids)
+    HIPCHK(hipMemcpyAsync(edge_ids + offsets[0], ctx->d_out, (offsets[n] - offsets[0]) * 4, hipMemcpyDeviceToHost,
+                          st));
+  HIPCHK(hipStreamSynchronize(st));
+  float kms = 0;
+  HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+  for (int64_t i = 0; i < n; ++i) {
+    const JobResult& r = out[i];
+    if (r.error)
+      return fail(r.error == ERR_CYCLE_TOO_LONG && r.out_len > jobs[i].out_cap ? PGC_ERR_CAPACITY : PGC_ERR_SOLVER,
+                  "graph " + std::to_string(i) + ": solver error " + std::to_string(r.error) +
+                      (r.error == ERR_CYCLE_TOO_LONG ? " (cycle of " + std::to_string(r.out_len) +
+                                                           " edges, capacity " + std::to_string(jobs[i].out_cap) + ")"
+                                                     : ""));
+  }
+  if (rows) {
+    int64_t tot = 0;
+    for (int64_t i = 0; i < n; ++i) tot += out[i].out_len * (int64_t)graphs[i]->ps_cols;
+    if ((rc = grow(&ctx->d_rows, ctx->rows_cap, std::max<int64_t>(tot, 1)))) return rc;
+    int64_t at = 0;
+    std::vector<int64_t> rowAt(n);
+    for (int64_t i = 0; i < n; ++i) {
+      rowAt[i] = at;
+      if (out[i].out_len)
+        HIPCHK(launch_gather_rows(graphs[i]->d_rows, graphs[i]->ps_cols, jobs[i].out_eid, out[i].out_len,
+                                  ctx->d_rows + at, st));
+      at += out[i].out_len * (int64_t)graphs[i]->ps_cols;
+    }
+    std::vector<int32_t> h(std::max<int64_t>(tot, 1));
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->d_rows, tot * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // rows of graph i at rows + (offsets[i] - offsets[0]) * ps_cols
+    const int64_t pc = graphs[0]->ps_cols;
+    for (int64_t i = 0; i < n; ++i)
+      std::memcpy(rows + (offsets[i] - offsets[0]) * pc, h.data() + rowAt[i],
+                  out[i].out_len * (int64_t)graphs[i]->ps_cols * 4);
+  }
+  const double total = wall.ms();
+  for (int64_t i = 0; i < n; ++i) {
+    const JobResult& r = out[i];
+    pgc_result& o = res[i];
+    std::memset(&o, 0, sizeof(o));
+    o.passes_layers_once = 1;
+    o.objective = (float)(r.out_cs / r.out_ts);
+    o.n_edges = r.out_len;
+    o.sweeps = r.sweeps;
+    o.bf_calls = r.bf_calls;
+    o.lawler_iters = r.lawler_iters;
+    o.branches = r.branches;
+    o.relaxations = relax_per_sweep(graphs[i], max_depth[i]) * (double)r.sweeps;
+    o.kernel_ms = kms;
+    o.total_ms = total;
+    const int64_t ticks[5] = {r.t_sweep, r.t_fold, r.t_chase, r.t_walk, r.t_extract};
+    for (int k = 0; k < 5; ++k) o.phase_ms[k] = (double)ticks[k] * 1e-5;
+  }
   return PGC_OK;
 }
 

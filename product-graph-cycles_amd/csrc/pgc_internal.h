@@ -32,7 +32,7 @@ struct SlotClass {
   int32_t lstride;       // slots per layer = nChunks * w * 64
 };
 
-enum JobMode : int32_t { MODE_RATIO = 0, MODE_UPPER_BOUND = 1, MODE_MIN_COST = 2 };
+enum JobMode : int32_t { MODE_RATIO = 0, MODE_UPPER_BOUND = 1, MODE_MIN_COST = 2, MODE_FULL_RATIO = 3 };
 
 enum Status : int32_t {
   ST_CAN_CONTAIN = 0,
@@ -49,6 +49,8 @@ enum JobError : int32_t {
   ERR_EDGE_LOOKUP = 3,
   ERR_CHAIN = 4,
   ERR_SWEEP_CAP = 5,
+  ERR_VALID_STORE = 6,  // more valid branches than the job's cycle store holds
+  ERR_BRANCH_CAP = 7,   // safety cap on branches of one solve (4*npl + 64)
 };
 
 // flags
@@ -74,6 +76,11 @@ struct JobResult {
   int64_t relax;        // candidate relaxations (finite source value)
   // device wall-clock ticks (wall_clock64, 100 MHz) spent per phase
   int64_t t_sweep, t_fold, t_chase, t_walk, t_extract;
+  // full solve (MODE_FULL_RATIO)
+  int64_t branches;
+  double out_cs, out_ts;  // fp64 sums of the output cycle's cost and time (walk order)
+  int32_t out_len;        // edges of the output cycle (in out_eid)
+  int32_t pad2;
 };
 
 struct SolverJob {
@@ -91,7 +98,7 @@ struct SolverJob {
   double tol;
   int64_t max_sweeps;     // safety cap on sweeps per launch (error when reached)
   SlotClass intra, inter;
-  const int32_t* branch;  // [npl] branch id per layer-0 vertex
+  int32_t* branch;        // [npl] branch id per layer-0 vertex
   int32_t* pred;          // [(L+1)*K*npl] predecessor (expanded node id)
   // per-sweep scratch
   double* backup;         // [npl] layer-0 values at the start of a sweep
@@ -113,6 +120,17 @@ struct SolverJob {
   double* lb_key;         // [npl]
   int32_t* back;          // [npl] start vertex of the path ending at (L, v)
   JobResult* res;
+  // device-resident branch and bound (MODE_FULL_RATIO)
+  double* hkey;           // [npl] queue slots
+  int32_t* hval;          // [npl]
+  int32_t* hwhere;        // [npl]
+  int32_t* vslot;         // [npl] cycle-store slot of a valid branch (-1: none)
+  int32_t* vc_eid;        // [vc_cap * vc_stride] cycles of valid branches
+  double* vc_c;
+  double* vc_t;
+  int32_t* vc_len;        // [vc_cap]
+  int32_t* out_eid;       // [out_cap] output cycle
+  int32_t vc_cap, vc_stride, out_cap, pad3;
 };
 
 }  // namespace pgc

@@ -5,24 +5,26 @@
 
 namespace pgc {
 
-hipError_t launch_solver_ratio(int, int, const SolverJob*, int, size_t, hipStream_t);
-hipError_t launch_solver_ratio_time(int, int, const SolverJob*, int, size_t, hipStream_t);
-hipError_t launch_solver_mincost(int, int, const SolverJob*, int, size_t, hipStream_t);
+hipError_t launch_solver_ratio(int, int, const SolverJob*, int, int, int*, size_t, hipStream_t);
+hipError_t launch_solver_ratio_time(int, int, const SolverJob*, int, int, int*, size_t, hipStream_t);
+hipError_t launch_solver_mincost(int, int, const SolverJob*, int, int, int*, size_t, hipStream_t);
 
 size_t solver_lds_bytes(int npl, int K) { return (size_t)(2 * K + 1) * (npl + 1) * sizeof(double); }
 
-// Persistent solver launch: one workgroup per job. maxw = widest chunk of the
-// layout (<= 32), maxChunksPerWave = ceil(ceil(npl/64) / 16) (<= 8).
-hipError_t launch_solver(const SolverJob* d_jobs, int njobs, int maxChunksPerWave, int maxw, bool mincost,
-                         bool hasTime, bool doubleCost, size_t lds, hipStream_t st) {
+// Solver launch: grid workgroups over njobs jobs (one job per workgroup when
+// queue is null, else a persistent launch on the queue counter). maxw = the
+// layout's slot width (<= 32), maxChunksPerWave = ceil(ceil(npl/64) / 16) (<= 8)
+// for the largest job.
+hipError_t launch_solver(const SolverJob* d_jobs, int njobs, int grid, int* queue, int maxChunksPerWave, int maxw,
+                         bool mincost, bool hasTime, bool doubleCost, size_t lds, hipStream_t st) {
   if (lds > 160 * 1024 - 1024) return hipErrorInvalidValue;
   if (maxChunksPerWave > 8 || maxw > 32) return hipErrorInvalidValue;
   if (mincost) {
     if (!doubleCost) return hipErrorInvalidValue;
-    return launch_solver_mincost(maxw, maxChunksPerWave, d_jobs, njobs, lds, st);
+    return launch_solver_mincost(maxw, maxChunksPerWave, d_jobs, njobs, grid, queue, lds, st);
   }
-  if (hasTime) return launch_solver_ratio_time(maxw, maxChunksPerWave, d_jobs, njobs, lds, st);
-  return launch_solver_ratio(maxw, maxChunksPerWave, d_jobs, njobs, lds, st);
+  if (hasTime) return launch_solver_ratio_time(maxw, maxChunksPerWave, d_jobs, njobs, grid, queue, lds, st);
+  return launch_solver_ratio(maxw, maxChunksPerWave, d_jobs, njobs, grid, queue, lds, st);
 }
 
 hipError_t launch_lookup(const SolverJob* d_job, const int32_t* d_nodes, int n, int32_t* d_eids, bool hasTime,

@@ -24,6 +24,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "bnb_device.h"
 #include "pgc_internal.h"
 
 namespace pgc {
@@ -784,10 +785,11 @@ struct Solver {
   }
 
   // ------------------------------------------------------- modes
-  __device__ __forceinline__ void lawler() {
-    int64_t sweeps = 0, bfc = 0, its = 0;
-    int err = ERR_NONE;
-    double ratio = J.ratio_in;
+  // lawlerSoubroutine (:558-635) on branch bid starting at lambda = ratio_in;
+  // returns the last Bellman-Ford status, the final lambda in `ratio`.
+  __device__ __forceinline__ int lawler_core(int bid, double ratio_in, double& ratio, int64_t& sweeps, int64_t& bfc,
+                                             int64_t& its, int& err) {
+    ratio = ratio_in;
     double oldRatio = ratio;
     const double tol = J.tol;
     int i = 0;
@@ -795,7 +797,7 @@ struct Solver {
     while (true) {
       its++;
       bfc++;
-      const int st = bellman_ford(ratio, false, false, J.branchId, -1, sweeps, err);
+      const int st = bellman_ford(ratio, false, false, bid, -1, sweeps, err);
       lastStatus = st;
       if (err != ERR_NONE) break;
       if (st == ST_ZERO_CYCLE || st == ST_NO_NEGATIVE_CYCLE) break;
@@ -806,33 +808,49 @@ struct Solver {
       if (i > 1000) break;
       i++;
     }
+    return lastStatus;
+  }
+
+  __device__ __forceinline__ void write_result(int status, int err, double ratio, double ratio_ub, int numWraps,
+                                               int nHits, int bestV, int64_t sweeps, int64_t bfc, int64_t its) {
+    PGC_GLOBAL JobResult& r = *gp(J.res);
+    r.status = status;
+    r.error = err;
+    r.ratio = ratio;
+    r.ratio_ub = ratio_ub;
+    r.num_wraps = numWraps;
+    r.num_in_branch = sh.numInBranch;
+    r.n_hits = nHits;
+    r.neg_len = *gp(J.neg_len);
+    r.best_v = bestV;
+    r.sweeps = sweeps;
+    r.bf_calls = bfc;
+    r.lawler_iters = its;
+    r.relax = 0;
+    r.t_sweep = tSweep;
+    r.t_fold = tFold;
+    r.t_chase = tChase;
+    r.t_walk = tWalk;
+    r.t_extract = tExtract;
+  }
+
+  __device__ __forceinline__ void lawler() {
+    int64_t sweeps = 0, bfc = 0, its = 0;
+    int err = ERR_NONE;
+    double ratio;
+    const int st = lawler_core(J.branchId, J.ratio_in, ratio, sweeps, bfc, its, err);
     if (threadIdx.x == 0) {
       int nw_ = 0, nh = 0;
       wraps_and_hits(nw_, nh);
+      write_result(st, err, ratio, 0, nw_, nh, -1, sweeps, bfc, its);
       PGC_GLOBAL JobResult& r = *gp(J.res);
-      r.status = lastStatus;
-      r.error = err;
-      r.ratio = ratio;
-      r.ratio_ub = 0;
-      r.num_wraps = nw_;
-      r.num_in_branch = sh.numInBranch;
-      r.n_hits = nh;
-      r.neg_len = *gp(J.neg_len);
-      r.best_v = -1;
-      r.sweeps = sweeps;
-      r.bf_calls = bfc;
-      r.lawler_iters = its;
-      r.relax = 0;
-      r.t_sweep = tSweep;
-      r.t_fold = tFold;
-      r.t_chase = tChase;
-      r.t_walk = tWalk;
-      r.t_extract = tExtract;
+      r.branches = 0;
     }
   }
 
   // findInitalUpperBound (:652-707)
-  __device__ __forceinline__ void upper_bound() {
+  // Returns lambda_UB (float-rounded) in every thread.
+  __device__ __forceinline__ double upper_bound_core(int64_t& sweeps, int& err, int& startV) {
     // start vertex: minimal float mean of the in-edge costs of its layer-1
     // copy (accumulated in ascending edge id), among vertices with > 3 in-edges
     double bestKey = PGC_INF;
@@ -875,8 +893,7 @@ struct Solver {
     }
     block_min_key(sh, bestKey, bestV);
     const int start = (bestV == 0x7fffffff) ? 0 : bestV;
-    int64_t sweeps = 0;
-    int err = ERR_NONE;
+    startV = start;
     double ratio = 10.0;
     bellman_ford(ratio, true, true, 0, start, sweeps, err);
     if (threadIdx.x == 0) {
@@ -890,27 +907,174 @@ struct Solver {
         pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
         cycleCost = (double)(float)ws;
       }
-      const double r_ub = (double)(float)(cs / ts);
-      PGC_GLOBAL JobResult& r = *gp(J.res);
-      r.status = ST_NO_NEGATIVE_CYCLE;
-      r.error = err;
-      r.ratio = r_ub;
-      r.ratio_ub = r_ub;
-      r.num_wraps = 0;
-      r.num_in_branch = 1;
-      r.n_hits = 0;
-      r.neg_len = n;
-      r.best_v = start;
-      r.sweeps = sweeps;
-      r.bf_calls = 1;
-      r.lawler_iters = 0;
-      r.relax = 0;
-      r.t_sweep = tSweep;
-      r.t_fold = tFold;
-      r.t_chase = tChase;
-      r.t_walk = tWalk;
-      r.t_extract = tExtract;
+      sh.d1 = (double)(float)(cs / ts);
     }
+    __syncthreads();
+    return sh.d1;
+  }
+
+  __device__ __forceinline__ void upper_bound() {
+    int64_t sweeps = 0;
+    int err = ERR_NONE, start = 0;
+    const double r_ub = upper_bound_core(sweeps, err, start);
+    if (threadIdx.x == 0) {
+      write_result(ST_NO_NEGATIVE_CYCLE, err, r_ub, r_ub, 0, 0, start, sweeps, 1, 0);
+      PGC_GLOBAL JobResult& r = *gp(J.res);
+      r.num_in_branch = 1;
+      r.branches = 0;
+    }
+  }
+
+  // Complete lawlergpu solve on the device: upper bound, then the branch and
+  // bound of solveWithBnbGPULawler (cost_time_ratio_solver_cuda.cpp:175-331,
+  // the CPU twin cost_time_ratio_solver.cpp:794-862) with its queue in global
+  // memory, then the output cycle (edge ids, fp64 cost and time sums in walk
+  // order) in out_eid / res.
+  __device__ __forceinline__ void full_ratio() {
+    int64_t sweeps = 0, bfc = 0, its = 0, branches = 0;
+    int err = ERR_NONE, start = 0;
+    const double rUB = upper_bound_core(sweeps, err, start);
+    bfc++;
+    // branch ids, queue and valid-branch slots
+    for (int v = threadIdx.x; v < npl; v += PGC_BLOCK) {
+      gp(J.branch)[v] = 0;
+      gp(J.vslot)[v] = -1;
+      gp(J.hkey)[v] = -1.0;
+      gp(J.hval)[v] = -1;
+      gp(J.hwhere)[v] = -1;
+    }
+    __syncthreads();
+    DevHeap heap;
+    int maxBranchId = 0, nValid = 0;
+    if (threadIdx.x == 0) {
+      heap.init(J.hkey, J.hval, J.hwhere, npl);
+      heap.push(0.0, 0);
+    }
+    int finalSlot = -1;  // -1: the current cycle; >= 0: a stored valid branch
+    int lastStatus = ST_NO_NEGATIVE_CYCLE, numWraps = 0, nHits = 0;
+    double lbBranch = 0;
+    while (err == ERR_NONE) {
+      // 0 = solve branch i1, 1 = finish with the current cycle, 2 = finish with slot i1
+      if (threadIdx.x == 0) {
+        if (heap.empty()) {
+          sh.i0 = 1;
+        } else {
+          double k;
+          const int bid = heap.pop(&k);
+          const int slot = gp(J.vslot)[bid];
+          sh.i0 = slot >= 0 ? 2 : 0;
+          sh.i1 = slot >= 0 ? slot : bid;
+        }
+      }
+      __syncthreads();
+      const int cmd = sh.i0, arg = sh.i1;
+      __syncthreads();
+      if (cmd == 1) break;
+      if (cmd == 2) {
+        finalSlot = arg;
+        break;
+      }
+      const int bid = arg;
+      if (++branches > 4 * (int64_t)npl + 64) {
+        err = ERR_BRANCH_CAP;
+        break;
+      }
+      lastStatus = lawler_core(bid, rUB, lbBranch, sweeps, bfc, its, err);
+      if (err != ERR_NONE) break;
+      // 0 = continue, 1 = stop with the current cycle, 2 = store the current cycle in slot i1,
+      // 3 = relabel branch bid to i1 (peel), 4 = error
+      if (threadIdx.x == 0) {
+        wraps_and_hits(numWraps, nHits);
+        const bool wraps = numWraps > 1;
+        if (rUB < heap.peak() && !wraps) {
+          sh.i0 = 1;
+        } else if (!wraps) {
+          heap.push(lbBranch, bid);
+          if (nValid >= J.vc_cap) {
+            sh.i0 = 4;
+          } else {
+            gp(J.vslot)[bid] = nValid;
+            sh.i0 = 2;
+            sh.i1 = nValid++;
+          }
+        } else if (sh.numInBranch == 1) {
+          sh.i0 = 0;
+        } else {
+          // branch() (:142-179): hits after the first get their own branch
+          const int nh = nHits < npl ? nHits : npl;
+          int offset = 0;
+          for (int h = 0; h < nh; ++h) {
+            const int x = gp(J.hits)[h];
+            if (offset > 0 && x >= 0 && x < npl) gp(J.branch)[x] = maxBranchId + offset;
+            offset++;
+          }
+          sh.i0 = 3;
+          sh.i1 = maxBranchId;
+          sh.i2 = offset;
+        }
+      }
+      __syncthreads();
+      const int act = sh.i0, a1 = sh.i1, a2 = sh.i2;
+      __syncthreads();
+      if (act == 1) break;
+      if (act == 4) {
+        err = ERR_VALID_STORE;
+        break;
+      }
+      if (act == 2) {
+        const int m = max(0, *gp(J.neg_len) - 1);
+        const int64_t base = (int64_t)a1 * J.vc_stride;
+        if (m > J.vc_stride) {
+          err = ERR_CYCLE_TOO_LONG;
+          break;
+        }
+        for (int i = threadIdx.x; i < m; i += PGC_BLOCK) {
+          gp(J.vc_eid)[base + i] = gp(J.neg_eid)[i];
+          gp(J.vc_c)[base + i] = gp(J.neg_c)[i];
+          gp(J.vc_t)[base + i] = gp(J.neg_t)[i];
+        }
+        if (threadIdx.x == 0) gp(J.vc_len)[a1] = m;
+        __syncthreads();
+        continue;
+      }
+      if (act == 3) {
+        for (int v = threadIdx.x; v < npl; v += PGC_BLOCK)
+          if (gp(J.branch)[v] == bid) gp(J.branch)[v] = a1;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          maxBranchId = a1 + a2;
+          const int nh = nHits < npl ? nHits : npl;
+          for (int h = 0; h < nh; ++h) {
+            const int x = gp(J.hits)[h];
+            if (x >= 0 && x < npl) heap.push(lbBranch, gp(J.branch)[x]);
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // output cycle
+    const int m = finalSlot >= 0 ? gp(J.vc_len)[finalSlot] : max(0, *gp(J.neg_len) - 1);
+    const PGC_GLOBAL int32_t* se = finalSlot >= 0 ? gp(J.vc_eid) + (int64_t)finalSlot * J.vc_stride : gp(J.neg_eid);
+    const PGC_GLOBAL double* sc = finalSlot >= 0 ? gp(J.vc_c) + (int64_t)finalSlot * J.vc_stride : gp(J.neg_c);
+    const PGC_GLOBAL double* stt = finalSlot >= 0 ? gp(J.vc_t) + (int64_t)finalSlot * J.vc_stride : gp(J.neg_t);
+    const bool fits = m <= J.out_cap;
+    if (fits)
+      for (int i = threadIdx.x; i < m; i += PGC_BLOCK) gp(J.out_eid)[i] = se[i];
+    if (threadIdx.x == 0) {
+      double cs = 0, ts = 0;
+      for (int i = 0; i < m; ++i) {
+        cs += sc[i];
+        ts += stt[i];
+      }
+      if (!fits && err == ERR_NONE) err = ERR_CYCLE_TOO_LONG;
+      write_result(lastStatus, err, lbBranch, rUB, numWraps, nHits, start, sweeps, bfc, its);
+      PGC_GLOBAL JobResult& r = *gp(J.res);
+      r.branches = branches;
+      r.out_len = m;
+      r.out_cs = cs;
+      r.out_ts = ts;
+    }
+    __syncthreads();
   }
 
   // one sweep of the minimum-cost solver + lower bounds and paths
@@ -993,25 +1157,40 @@ struct Solver {
   }
 };
 
+// One workgroup per job. With a queue counter the launch is persistent:
+// every workgroup takes job indices from the counter until all njobs are done
+// (jobs are independent; the dynamic LDS fits the largest job of the launch).
 template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
-__global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __restrict__ jobs) {
+__global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __restrict__ jobs, int njobs,
+                                                          int* __restrict__ queue) {
   extern __shared__ double lds[];
   __shared__ Shared sh;
-  if (threadIdx.x == 0) {
-    sh.orCount[0] = 0;
-    sh.orCount[1] = 0;
-  }
-  __syncthreads();
-  const SolverJob& J = jobs[blockIdx.x];
-  double* D = lds;
-  double* Dmin = lds + 2 * J.K * (J.npl + 1);
-  Solver<CT, HAS_TIME, MINCOST, MAXC, W> s(J, D, Dmin, sh);
-  if (MINCOST) {
-    s.min_cost();
-  } else if (J.mode == MODE_UPPER_BOUND) {
-    s.upper_bound();
-  } else {
-    s.lawler();
+  int next = blockIdx.x;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sh.orCount[0] = 0;
+      sh.orCount[1] = 0;
+      sh.i3 = queue ? atomicAdd(queue, 1) : next;
+    }
+    __syncthreads();
+    const int j = sh.i3;
+    if (j >= njobs) break;
+    const SolverJob& J = jobs[j];
+    double* D = lds;
+    double* Dmin = lds + 2 * J.K * (J.npl + 1);
+    Solver<CT, HAS_TIME, MINCOST, MAXC, W> s(J, D, Dmin, sh);
+    if (MINCOST) {
+      s.min_cost();
+    } else if (J.mode == MODE_UPPER_BOUND) {
+      s.upper_bound();
+    } else if (J.mode == MODE_FULL_RATIO) {
+      s.full_ratio();
+    } else {
+      s.lawler();
+    }
+    if (!queue) break;
+    next += gridDim.x;
   }
 }
 

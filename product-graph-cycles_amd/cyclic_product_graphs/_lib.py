@@ -49,6 +49,7 @@ EXPORTED = [
     "pgc_graph_generate_device",
     "pgc_graph_from_arrays", "pgc_graph_set_branch_graph", "pgc_graph_info", "pgc_graph_get_product_space",
     "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
+    "pgc_solve_ratio_batch",
 ]
 
 _lib = None
@@ -82,6 +83,7 @@ def load():
     L.pgc_graph_destroy.restype = None
     L.pgc_solve_ratio.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
     L.pgc_solve_min_cost.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
+    L.pgc_solve_ratio_batch.argtypes = [P, P, P, I64, C.c_uint32, P, P, P, P]
     for name in EXPORTED:
         getattr(L, name)
     _lib = L
@@ -250,3 +252,33 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+def solve_ratio_batch(ctx, graphs, max_depth, flags=PGC_LAWLER_PICK_LAST, with_rows=True):
+    """Complete lawlergpu solves of a list of Graphs in persistent launches.
+
+    max_depth: one int for all graphs or one per graph. Returns a list of
+    (edge_ids, rows or None, pgc_result) in the order of `graphs`.
+    """
+    L = load()
+    n = len(graphs)
+    if n == 0:
+        return []
+    K = np.broadcast_to(np.asarray(max_depth, np.int32), (n,)).copy()
+    infos = [g.info() for g in graphs]
+    caps = np.array([max(64, 101 * i["n_layers"] * int(k) + 8) for i, k in zip(infos, K)], np.int64)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(caps, out=off[1:])
+    eids = np.empty(int(off[-1]), np.int32)
+    cols = infos[0]["ps_cols"]
+    rows = np.empty((int(off[-1]), cols), np.int32) if with_rows else None
+    handles = (C.c_void_p * n)(*[g._h.value for g in graphs])
+    res = (pgc_result * n)()
+    check(L.pgc_solve_ratio_batch(ctx.handle, handles, ptr(K), n, int(flags), ptr(eids), ptr(rows), ptr(off), res))
+    out = []
+    for i in range(n):
+        m = res[i].n_edges
+        a = int(off[i])
+        out.append((eids[a:a + m].copy(), None if rows is None else rows[a:a + m].copy(), res[i]))
+    return out
+

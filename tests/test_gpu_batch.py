@@ -1,0 +1,108 @@
+"""Batched complete solves (pgc_solve_ratio_batch: device-resident branch and
+bound, persistent launches) against the single-graph solve and the oracle.
+
+Bar: for every graph of a batch, the cycle (edge ids, walk order) and its
+product-space rows are bit-exact with the oracle's and with pgc_solve_ratio's;
+the fp32 objective is equal (1e-5 relative written as the tolerance, observed
+difference 0); the sweep, Bellman-Ford call and branch counters are equal.
+"""
+import numpy as np
+import pytest
+
+import pgc_oracle as O
+import cyclic_product_graphs as cpg
+from cyclic_product_graphs import _lib
+from cyclic_product_graphs import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+OBJ_RTOL = 1e-5
+
+
+def _graph(ctx, p, reg=False, cost="vanilla", time=""):
+    return _lib.Graph.generate(ctx, p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"].astype(np.float64), False,
+                               reg, False, cost, time)
+
+
+def _oracle(p, K, reg=False, cost="vanilla", time=""):
+    rows, src, trgt = O.product_space(p["EX"], p["EY"])
+    en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"].astype(np.float64), reg=reg, cost_mode=cost,
+                  time_mode=time)
+    e, obj, passes, stats, q = O.ctr_solve(src, trgt, en, len(p["VX"]), K, gpu_pick=True)
+    return rows, e, obj, stats
+
+
+def _cases():
+    out = []
+    for s, (nx, ny, ring) in enumerate([(4, 4, True), (5, 6, True), (6, 5, False), (4, 7, False)]):
+        for K in (1, 2, 3):
+            out.append((S.tiny_pair(nx, ny, seed=s, ring=ring), K, {}))
+    for sub, ny, K, seed in [(1, 8, 2, 0), (2, 33, 2, 1), (2, 41, 3, 4), (3, 64, 2, 0), (2, 20, 1, 3)]:
+        out.append((S.make_pair(sub, ny, seed=seed), K, {}))
+    p = S.make_pair(2, 24, seed=7)
+    out.append((p, 2, {"reg": True}))
+    out.append((p, 2, {"cost": "feature", "time": "lengthnormalisation"}))
+    return out
+
+
+def test_batch_matches_single_and_oracle():
+    ctx = _lib.default_context(0)
+    cases = _cases()
+    print("batch of", len(cases), "graphs", flush=True)
+    graphs = [_graph(ctx, p, **kw) for p, K, kw in cases]
+    outs = _lib.solve_ratio_batch(ctx, graphs, [K for _, K, _ in cases])
+    for (p, K, kw), (eids, rows, res) in zip(cases, outs):
+        orow, oe, oobj, stats = _oracle(p, K, **kw)
+        assert np.array_equal(rows, orow[oe]), "matching differs from the oracle"
+        assert abs(res.objective - float(oobj)) <= OBJ_RTOL * abs(float(oobj))
+        assert res.objective == float(oobj)
+        assert (res.bf_calls, res.sweeps, res.branches) == (stats[0], stats[1], stats[3])
+        g1 = _graph(ctx, p, **kw)
+        e1, r1, res1 = g1.solve_ratio(K)
+        assert np.array_equal(eids, e1) and np.array_equal(rows, r1)
+        assert res.objective == res1.objective
+        assert (res.sweeps, res.bf_calls, res.lawler_iters, res.branches) == \
+            (res1.sweeps, res1.bf_calls, res1.lawler_iters, res1.branches)
+        g1.close()
+    for g in graphs:
+        g.close()
+
+
+def test_batch_repeated_requantizes_like_single():
+    # a second batch over the same graphs re-quantizes their quantized costs
+    # in place, as a second pgc_solve_ratio does
+    ctx = _lib.default_context(0)
+    p = S.make_pair(2, 16, seed=9)
+    gb = _graph(ctx, p)
+    gs = _graph(ctx, p)
+    for _ in range(2):
+        (eb, rb, resb), = _lib.solve_ratio_batch(ctx, [gb], 2)
+        es, rs, ress = gs.solve_ratio(2)
+        assert np.array_equal(eb, es) and resb.objective == ress.objective
+        assert np.array_equal(gb.cost(), gs.cost())
+
+
+def test_batch_c1_full_size_with_duplicated_pairs():
+    # eight independent copies of the full-size pair (icosphere s=4 x 128, K=3)
+    ctx = _lib.default_context(0)
+    p = S.make_pair(4, 128, seed=0)
+    graphs = [_graph(ctx, p) for _ in range(8)]
+    outs = _lib.solve_ratio_batch(ctx, graphs, 3)
+    orow, oe, oobj, stats = _oracle(p, 3)
+    for eids, rows, res in outs:
+        assert np.array_equal(rows, orow[oe])
+        assert res.objective == float(oobj)
+        assert res.sweeps == stats[1]
+    for g in graphs:
+        g.close()
+
+
+def test_batch_argument_errors():
+    ctx = _lib.default_context(0)
+    p = S.tiny_pair(4, 4, seed=0, ring=True)
+    g = _graph(ctx, p)
+    with pytest.raises(_lib.PgcError):
+        _lib.solve_ratio_batch(ctx, [g, g], 2)
+    with pytest.raises(_lib.PgcError):
+        _lib.solve_ratio_batch(ctx, [g], 0)
+    g.close()
