@@ -127,6 +127,35 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
                           uint32_t flags, int32_t* edge_ids, int32_t* rows, const int64_t* offsets,
                           pgc_result* res);
 
+/* Recomputes the graph's per-edge energy from another feat_diff ([nVX x nVY]
+ * row-major float64, device memory when on_device != 0, else host) with the
+ * graph's construction modes; the product space and layout are unchanged.
+ * The analogue of building a new generator for perturbed features
+ * (product_graph_generators.cpp:21-89) without rebuilding the graph. */
+int pgc_graph_set_feat_diff(pgc_graph* g, const double* feat_diff, int on_device);
+
+/* A new graph that shares base's product space, ids and in-edge layout
+ * (read-only) and owns its energy (from feat_diff, as above), quantized costs
+ * and solver scratch: the per-sample graphs of a perturbation batch. base may
+ * be destroyed first; its shared memory is freed with its last derived graph. */
+int pgc_graph_derive(pgc_graph* base, const double* feat_diff, int on_device, pgc_graph** out);
+
+/* Cycle ratio in raw energy units, lambda = sum_e c_e / sum_e t_e over edge_ids[
+ * offsets[i] .. offsets[i] + lengths[i]) of graph i (walk order, fp64), with c_e
+ * the raw edge energy (deformationEnergy.cpp:11-103 as computed on the device)
+ * for feat_diff[i] (device pointer, [nVX x nVY] float64) and t_e the time column
+ * (1 per edge without one). No counterpart in the reference (the value whose
+ * gradient the training interface needs). */
+int pgc_cycle_ratio_values(pgc_context* ctx, pgc_graph* const* graphs, int64_t n, const double* const* feat_diff,
+                           const int32_t* edge_ids, const int64_t* offsets, const int64_t* lengths, double* values);
+
+/* Adds grad_out[i] * d lambda_i / d feat_diff into dfeat_diff[i] (device
+ * pointers, [nVX x nVY] float64) for the same cycles: every cycle edge adds
+ * dc_e/dFD / T(C) at its two (x, y) entries. No counterpart in the reference. */
+int pgc_cycle_ratio_grad(pgc_context* ctx, pgc_graph* const* graphs, int64_t n, const int32_t* edge_ids,
+                         const int64_t* offsets, const int64_t* lengths, const double* grad_out,
+                         double* const* dfeat_diff);
+
 /* Minimum-cost once-wrapping cycle with branch and bound ("dijkstragpu").
  * Replaces DijkstraSolver::run -> solveWithBnBCuda
  * (dijkstra_solver.cpp:563-579, dijkstra_solver_cuda.cpp:136-322). */

@@ -64,6 +64,19 @@ __global__ void k_product_space(const int32_t* __restrict__ EX, int64_t nEX, con
 }
 
 // ------------------------------------------------------------- energy (a3)
+// Length of a product edge (the line-integral factor) and its feature sum.
+__device__ __forceinline__ double edge_length(const double* __restrict__ VX, const double* __restrict__ VY,
+                                              int64_t y0, int64_t y1, int64_t x0, int64_t x1) {
+  double len = 0;
+  for (int j = 0; j < 3; ++j) {
+    const double a = VY[3 * y0 + j] - VY[3 * y1 + j];
+    len = len + __dmul_rn(a, a);
+    const double b = VX[3 * x0 + j] - VX[3 * x1 + j];
+    len = len + __dmul_rn(b, b);
+  }
+  return sqrt(len);
+}
+
 // FD is |VX| x |VY| row-major (fp64); out is column-major E x ncols.
 __global__ void k_energy(const double* __restrict__ VX, const double* __restrict__ VY,
                          const int32_t* __restrict__ rows, int64_t E, const double* __restrict__ FD, int64_t nVY,
@@ -72,23 +85,71 @@ __global__ void k_energy(const double* __restrict__ VX, const double* __restrict
   if (e >= E) return;
   const int4 row = reinterpret_cast<const int4*>(rows)[e];
   const int64_t y0 = row.x, y1 = row.y, x0 = row.z, x1 = row.w;
-  double len = 1.0;
-  if (lineIntegral) {
-    len = 0;
-    for (int j = 0; j < 3; ++j) {
-      const double a = VY[3 * y0 + j] - VY[3 * y1 + j];
-      len = len + __dmul_rn(a, a);
-      const double b = VX[3 * x0 + j] - VX[3 * x1 + j];
-      len = len + __dmul_rn(b, b);
-    }
-    len = sqrt(len);
-  }
+  const double len = lineIntegral ? edge_length(VX, VY, y0, y1, x0, x1) : 1.0;
   const double fd = FD[x0 * nVY + y0] + FD[x1 * nVY + y1];
   if (!lengthMode) {
     out[e] = __dmul_rn(__dmul_rn(0.5, fd), len);
   } else {
     out[e] = fd;
     out[E + e] = len;
+  }
+}
+
+// --------------------------------- cycle ratio in raw units and its gradient
+// lambda = sum_e c_e / sum_e t_e over a cycle's edges (walk order), with c_e
+// the raw energy of k_energy and t_e its time column (1 per edge without
+// one); d lambda / d FD[x,y] = sum over edge endpoints (x,y) of dc_e/dFD / T.
+
+__device__ __forceinline__ void cycle_edge(const CycleItem& it, int i, int64_t& a, int64_t& b, double& fd,
+                                           double& len) {
+  const int4 row = reinterpret_cast<const int4*>(it.rows)[it.eids[i]];
+  a = (int64_t)row.z * it.nVY + row.x;
+  b = (int64_t)row.w * it.nVY + row.y;
+  len = it.line ? edge_length(it.vx, it.vy, row.x, row.y, row.z, row.w) : 1.0;
+  fd = it.fd ? it.fd[a] + it.fd[b] : 0.0;
+}
+
+// one workgroup per item; sequential sums in walk order (deterministic)
+__global__ void k_cycle_value(const CycleItem* __restrict__ items, int n, double* __restrict__ values) {
+  const CycleItem it = items[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  double cs = 0, ts = 0;
+  for (int i = 0; i < it.m; ++i) {
+    int64_t a, b;
+    double fd, len;
+    cycle_edge(it, i, a, b, fd, len);
+    if (!it.length) {
+      cs += __dmul_rn(__dmul_rn(0.5, fd), len);
+      ts += 1.0;
+    } else {
+      cs += fd;
+      ts += len;
+    }
+  }
+  values[blockIdx.x] = cs / ts;
+}
+
+__global__ void k_cycle_grad(const CycleItem* __restrict__ items, int n) {
+  const CycleItem it = items[blockIdx.x];
+  __shared__ double T;
+  if (threadIdx.x == 0) {
+    double ts = 0;
+    for (int i = 0; i < it.m; ++i) {
+      int64_t a, b;
+      double fd, len;
+      cycle_edge(it, i, a, b, fd, len);
+      ts += it.length ? len : 1.0;
+    }
+    T = ts;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < it.m; i += blockDim.x) {
+    int64_t a, b;
+    double fd, len;
+    cycle_edge(it, i, a, b, fd, len);
+    const double coef = (it.length ? 1.0 : 0.5 * len) / T * it.gout;
+    atomicAdd(&it.dfd[a], coef);
+    atomicAdd(&it.dfd[b], coef);
   }
 }
 
@@ -411,6 +472,18 @@ hipError_t launch_gather_rows(const int32_t* rows, int cols, const int32_t* eids
 hipError_t launch_int_max(const int32_t* x, int64_t n, int32_t* out, hipStream_t st) {
   const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
   hipLaunchKernelGGL(k_int_max, dim3(g), dim3(256), 0, st, x, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cycle_value(const CycleItem* items, int n, double* values, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cycle_value, dim3(n), dim3(64), 0, st, items, n, values);
+  return hipGetLastError();
+}
+
+hipError_t launch_cycle_grad(const CycleItem* items, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cycle_grad, dim3(n), dim3(256), 0, st, items, n);
   return hipGetLastError();
 }
 

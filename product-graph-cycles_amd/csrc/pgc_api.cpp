@@ -44,6 +44,8 @@ hipError_t launch_fill_costs(const int32_t*, const int32_t*, int64_t, const doub
 hipError_t launch_gather_rows(const int32_t*, int, const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
+hipError_t launch_cycle_value(const CycleItem*, int, double*, hipStream_t);
+hipError_t launch_cycle_grad(const CycleItem*, int, hipStream_t);
 // solver.hip
 size_t solver_lds_bytes(int npl, int K);
 hipError_t launch_solver(const SolverJob*, int, int, int*, int, int, bool, bool, bool, size_t, hipStream_t);
@@ -88,6 +90,13 @@ struct pgc_context {
   int32_t* d_rows = nullptr;
   int64_t rows_cap = 0;
   int32_t* d_queue = nullptr;  // one counter per launch group
+  // cycle value / gradient calls
+  pgc::CycleItem* d_items = nullptr;
+  int64_t items_cap = 0;
+  int32_t* d_ceids = nullptr;
+  int64_t ceids_cap = 0;
+  double* d_values = nullptr;
+  int64_t values_cap = 0;
 };
 
 namespace {
@@ -148,8 +157,27 @@ struct pgc_graph {
   bool has_bg = false;
   std::vector<int64_t> bg_off, bg_val;
   JobScratch js;
+  // energy inputs, kept to recompute the energy from another feat_diff
+  double *d_vx = nullptr, *d_vy = nullptr;
+  int64_t nVX = 0, nVY = 0;
+  int line_integral = 0, length_mode = 0;
+  // a derived graph shares base's product space, ids, layout and energy
+  // inputs (read-only) and owns its energy, costs and solver scratch
+  pgc_graph* base = nullptr;
+  int refs = 0;           // derived graphs alive
+  bool released = false;  // destroyed by the caller while derived graphs live
+  size_t cost_bytes[2] = {0, 0};
+  size_t time_bytes[2] = {0, 0};
   ~pgc_graph() {
     js.release();
+    if (base) {
+      d_rows = d_src = d_trgt = nullptr;
+      d_width = d_cnt = nullptr;
+      d_s[0] = d_s[1] = d_e[0] = d_e[1] = nullptr;
+      d_vx = d_vy = nullptr;
+    }
+    if (d_vx) (void)hipFree(d_vx);
+    if (d_vy) (void)hipFree(d_vy);
     void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_width, d_s[0], d_s[1], d_e[0], d_e[1],
                   d_cnt, d_cost[0], d_cost[1], d_time[0], d_time[1]};
     for (void* p : ps)
@@ -190,7 +218,7 @@ void pgc_context_destroy(pgc_context* c) {
   if (c->select_state) (void)hipFree(c->select_state);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue};
+  void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -280,17 +308,28 @@ int build_layout(pgc_graph* g) {
 int fill_costs(pgc_graph* g, int kind) {
   hipStream_t st = g->ctx->stream;
   for (int c = 0; c < 2; ++c) {
-    if (g->d_cost[c]) (void)hipFree(g->d_cost[c]);
-    if (g->d_time[c]) (void)hipFree(g->d_time[c]);
-    g->d_cost[c] = nullptr;
-    g->d_time[c] = nullptr;
     const int64_t n = g->nslots[c];
-    HIPCHK(hipMalloc(&g->d_cost[c], (n + kSlotPad) * (kind == 2 ? 8 : 4)));
-    HIPCHK(hipMemsetAsync(g->d_cost[c], 0, (n + kSlotPad) * (kind == 2 ? 8 : 4), st));
+    const size_t cb = (size_t)(n + kSlotPad) * (kind == 2 ? 8 : 4);
     const bool withTime = kind == 1 && g->ncols > 1;
-    if (withTime) {
-      HIPCHK(dalloc(&g->d_time[c], n + kSlotPad));
-      HIPCHK(hipMemsetAsync(g->d_time[c], 0, (n + kSlotPad) * 4, st));
+    const size_t tb = withTime ? (size_t)(n + kSlotPad) * 4 : 0;
+    // buffers are reused while their size fits; every slot is rewritten below
+    if (g->cost_bytes[c] != cb) {
+      if (g->d_cost[c]) (void)hipFree(g->d_cost[c]);
+      g->d_cost[c] = nullptr;
+      g->cost_bytes[c] = 0;
+      HIPCHK(hipMalloc(&g->d_cost[c], cb));
+      HIPCHK(hipMemsetAsync(g->d_cost[c], 0, cb, st));
+      g->cost_bytes[c] = cb;
+    }
+    if (g->time_bytes[c] != tb) {
+      if (g->d_time[c]) (void)hipFree(g->d_time[c]);
+      g->d_time[c] = nullptr;
+      g->time_bytes[c] = 0;
+      if (tb) {
+        HIPCHK(dalloc(&g->d_time[c], n + kSlotPad));
+        HIPCHK(hipMemsetAsync(g->d_time[c], 0, tb, st));
+        g->time_bytes[c] = tb;
+      }
     }
     HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], n, g->d_energy, g->E, g->ncols, kind == 2, g->d_cost[c],
                              withTime ? g->d_time[c] : nullptr, st));
@@ -545,8 +584,15 @@ int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   HIPCHK(dalloc(&g->d_trgt, E));
   HIPCHK(dalloc(&g->d_energy, E * g->ncols));
   HIPCHK(launch_product_space(dEX, nEX, dEY, nEY, nVXe, prune ? 1 : 0, g->d_rows, g->d_src, g->d_trgt, E, st));
-  HIPCHK(launch_energy(dVX, dVY, g->d_rows, E, dFD, nVY, (regularising || lengthMode) ? 1 : 0, lengthMode ? 1 : 0,
-                       g->d_energy, st));
+  g->nVX = nVX;
+  g->nVY = nVY;
+  g->line_integral = (regularising || lengthMode) ? 1 : 0;
+  g->length_mode = lengthMode ? 1 : 0;
+  HIPCHK(dalloc(&g->d_vx, 3 * nVX));
+  HIPCHK(dalloc(&g->d_vy, 3 * nVY));
+  HIPCHK(hipMemcpyAsync(g->d_vx, dVX, 3 * nVX * 8, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(g->d_vy, dVY, 3 * nVY * 8, hipMemcpyDeviceToDevice, st));
+  HIPCHK(launch_energy(dVX, dVY, g->d_rows, E, dFD, nVY, g->line_integral, g->length_mode, g->d_energy, st));
   // numNodes / numLayers as the solver derives them (run(), :870-871)
   g->numNodes = nEY * nVXe;
   g->L = nEY;
@@ -675,7 +721,101 @@ int pgc_graph_set_cost(pgc_graph* g, const double* cost) {
 void pgc_graph_destroy(pgc_graph* g) {
   if (!g) return;
   (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  if (pgc_graph* b = g->base) {
+    delete g;
+    if (--b->refs == 0 && b->released) delete b;
+    return;
+  }
+  if (g->refs > 0) {
+    g->released = true;  // freed with its last derived graph
+    return;
+  }
   delete g;
+}
+
+static int energy_from(pgc_graph* g, const double* feat_diff, int on_device) {
+  const pgc_graph* b = g->base ? g->base : g;
+  if (!b->d_vx) return fail(PGC_ERR_UNSUPPORTED, "the graph was not built by pgc_graph_generate*: no energy inputs");
+  hipStream_t st = g->ctx->stream;
+  const double* fd = feat_diff;
+  double* tmp = nullptr;
+  if (!on_device) {
+    HIPCHK(dalloc(&tmp, b->nVX * b->nVY));
+    HIPCHK(hipMemcpyAsync(tmp, feat_diff, b->nVX * b->nVY * 8, hipMemcpyHostToDevice, st));
+    fd = tmp;
+  }
+  HIPCHK(launch_energy(b->d_vx, b->d_vy, b->d_rows, b->E, fd, b->nVY, b->line_integral, b->length_mode,
+                       g->d_energy, st));
+  g->cost_kind = 0;
+  if (tmp) {
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(tmp);
+  }
+  return PGC_OK;
+}
+
+int pgc_graph_set_feat_diff(pgc_graph* g, const double* feat_diff, int on_device) {
+  if (!g || !feat_diff) return fail(PGC_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(g->ctx->device));
+  return energy_from(g, feat_diff, on_device);
+}
+
+int pgc_graph_derive(pgc_graph* base, const double* feat_diff, int on_device, pgc_graph** out) {
+  if (!base || !feat_diff || !out) return fail(PGC_ERR_ARG, "invalid arguments");
+  pgc_graph* b = base->base ? base->base : base;
+  if (b->released) return fail(PGC_ERR_ARG, "base graph was destroyed");
+  if (!b->d_vx) return fail(PGC_ERR_UNSUPPORTED, "the graph was not built by pgc_graph_generate*: no energy inputs");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  int rc;
+  if ((rc = build_layout(b))) return rc;
+  std::unique_ptr<pgc_graph> g(new pgc_graph());
+  g->base = b;
+  g->ctx = b->ctx;
+  g->E = b->E;
+  g->npl = b->npl;
+  g->numNodes = b->numNodes;
+  g->L = b->L;
+  g->ps_cols = b->ps_cols;
+  g->ncols = b->ncols;
+  g->d_rows = b->d_rows;
+  g->d_src = b->d_src;
+  g->d_trgt = b->d_trgt;
+  g->layout = true;
+  g->nChunks = b->nChunks;
+  g->nLayers = b->nLayers;
+  for (int c = 0; c < 2; ++c) {
+    g->nslots[c] = b->nslots[c];
+    g->d_s[c] = b->d_s[c];
+    g->d_e[c] = b->d_e[c];
+    g->width[c] = b->width[c];
+    g->lstride[c] = b->lstride[c];
+  }
+  g->d_width = b->d_width;
+  g->maxw = b->maxw;
+  g->d_cnt = b->d_cnt;
+  g->dropped = b->dropped;
+  g->intra_slots_valid = b->intra_slots_valid;
+  g->inter_slots_valid = b->inter_slots_valid;
+  g->has_bg = b->has_bg;
+  g->bg_off = b->bg_off;
+  g->bg_val = b->bg_val;
+  g->nVX = b->nVX;
+  g->nVY = b->nVY;
+  g->line_integral = b->line_integral;
+  g->length_mode = b->length_mode;
+  HIPCHK(dalloc(&g->d_energy, g->E * g->ncols));
+  // the derived graph counts as a user of b from here on, so that a failure
+  // below releases it through the normal path
+  b->refs++;
+  pgc_graph* raw = g.release();
+  raw->base = b;
+  if ((rc = energy_from(raw, feat_diff, on_device))) {
+    pgc_graph_destroy(raw);
+    return rc;
+  }
+  *out = raw;
+  return PGC_OK;
 }
 
 // ------------------------------------------------------------ Lawler + BnB
@@ -1001,6 +1141,86 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     const int64_t ticks[5] = {r.t_sweep, r.t_fold, r.t_chase, r.t_walk, r.t_extract};
     for (int k = 0; k < 5; ++k) o.phase_ms[k] = (double)ticks[k] * 1e-5;
   }
+  return PGC_OK;
+}
+
+// ------------------------------------- raw-unit cycle ratio and its gradient
+static int cycle_items(pgc_context* ctx, pgc_graph* const* graphs, int64_t n, const double* const* fd,
+                       double* const* dfd, const int32_t* edge_ids, const int64_t* offsets, const int64_t* lengths,
+                       const double* grad_out, std::vector<CycleItem>& items) {
+  if (!ctx || n < 0 || (n > 0 && (!graphs || !edge_ids || !offsets || !lengths)))
+    return fail(PGC_ERR_ARG, "invalid arguments");
+  if (n > INT32_MAX) return fail(PGC_ERR_ARG, "too many cycles in one call");
+  int64_t tot = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!graphs[i] || graphs[i]->ctx != ctx) return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " is invalid");
+    const pgc_graph* b = graphs[i]->base ? graphs[i]->base : graphs[i];
+    if (!b->d_vx) return fail(PGC_ERR_UNSUPPORTED, "graph " + std::to_string(i) + " has no energy inputs");
+    if (b->ps_cols != 4) return fail(PGC_ERR_UNSUPPORTED, "conjugate graphs have no raw-unit gradient here");
+    if (lengths[i] < 0 || lengths[i] > INT32_MAX) return fail(PGC_ERR_ARG, "invalid cycle length");
+    tot += lengths[i];
+  }
+  int rc;
+  if ((rc = grow(&ctx->d_items, ctx->items_cap, std::max<int64_t>(n, 1)))) return rc;
+  if ((rc = grow(&ctx->d_ceids, ctx->ceids_cap, std::max<int64_t>(tot, 1)))) return rc;
+  std::vector<int32_t> cat(std::max<int64_t>(tot, 1));
+  items.resize(n);
+  int64_t at = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const pgc_graph* g = graphs[i];
+    const pgc_graph* b = g->base ? g->base : g;
+    for (int64_t k = 0; k < lengths[i]; ++k) {
+      const int32_t e = edge_ids[offsets[i] + k];
+      if (e < 0 || e >= g->E) return fail(PGC_ERR_ARG, "edge id outside the graph");
+      cat[at + k] = e;
+    }
+    CycleItem& it = items[i];
+    std::memset(&it, 0, sizeof(it));
+    it.rows = b->d_rows;
+    it.vx = b->d_vx;
+    it.vy = b->d_vy;
+    it.fd = fd ? fd[i] : nullptr;
+    it.dfd = dfd ? dfd[i] : nullptr;
+    it.eids = ctx->d_ceids + at;
+    it.nVY = b->nVY;
+    it.m = (int32_t)lengths[i];
+    it.line = b->line_integral;
+    it.length = b->length_mode;
+    it.gout = grad_out ? grad_out[i] : 0.0;
+    at += lengths[i];
+  }
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipMemcpyAsync(ctx->d_ceids, cat.data(), std::max<int64_t>(tot, 1) * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctx->d_items, items.data(), n * sizeof(CycleItem), hipMemcpyHostToDevice, st));
+  return PGC_OK;
+}
+
+int pgc_cycle_ratio_values(pgc_context* ctx, pgc_graph* const* graphs, int64_t n, const double* const* feat_diff,
+                           const int32_t* edge_ids, const int64_t* offsets, const int64_t* lengths, double* values) {
+  if (!feat_diff || (n > 0 && !values)) return fail(PGC_ERR_ARG, "invalid arguments");
+  if (n == 0) return PGC_OK;
+  HIPCHK(hipSetDevice(ctx ? ctx->device : 0));
+  std::vector<CycleItem> items;
+  int rc = cycle_items(ctx, graphs, n, feat_diff, nullptr, edge_ids, offsets, lengths, nullptr, items);
+  if (rc) return rc;
+  if ((rc = grow(&ctx->d_values, ctx->values_cap, n))) return rc;
+  HIPCHK(launch_cycle_value(ctx->d_items, (int)n, ctx->d_values, ctx->stream));
+  HIPCHK(hipMemcpyAsync(values, ctx->d_values, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return PGC_OK;
+}
+
+int pgc_cycle_ratio_grad(pgc_context* ctx, pgc_graph* const* graphs, int64_t n, const int32_t* edge_ids,
+                         const int64_t* offsets, const int64_t* lengths, const double* grad_out,
+                         double* const* dfeat_diff) {
+  if (!dfeat_diff || (n > 0 && !grad_out)) return fail(PGC_ERR_ARG, "invalid arguments");
+  if (n == 0) return PGC_OK;
+  HIPCHK(hipSetDevice(ctx ? ctx->device : 0));
+  std::vector<CycleItem> items;
+  int rc = cycle_items(ctx, graphs, n, nullptr, dfeat_diff, edge_ids, offsets, lengths, grad_out, items);
+  if (rc) return rc;
+  HIPCHK(launch_cycle_grad(ctx->d_items, (int)n, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return PGC_OK;
 }
 

@@ -133,4 +133,17 @@ struct SolverJob {
   int32_t vc_cap, vc_stride, out_cap, pad3;
 };
 
+// One cycle of the raw-unit ratio value / gradient kernels.
+struct CycleItem {
+  const int32_t* rows;  // E x 4 product-space rows
+  const double* vx;
+  const double* vy;
+  const double* fd;     // forward: the item's feat_diff
+  double* dfd;          // backward: gradient accumulator (same shape)
+  const int32_t* eids;  // m edge ids
+  int64_t nVY;
+  int32_t m, line, length, pad;
+  double gout;          // backward: upstream gradient
+};
+
 }  // namespace pgc

@@ -49,7 +49,8 @@ EXPORTED = [
     "pgc_graph_generate_device",
     "pgc_graph_from_arrays", "pgc_graph_set_branch_graph", "pgc_graph_info", "pgc_graph_get_product_space",
     "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
-    "pgc_solve_ratio_batch",
+    "pgc_solve_ratio_batch", "pgc_graph_set_feat_diff", "pgc_graph_derive", "pgc_cycle_ratio_values",
+    "pgc_cycle_ratio_grad",
 ]
 
 _lib = None
@@ -84,6 +85,10 @@ def load():
     L.pgc_solve_ratio.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
     L.pgc_solve_min_cost.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
     L.pgc_solve_ratio_batch.argtypes = [P, P, P, I64, C.c_uint32, P, P, P, P]
+    L.pgc_graph_set_feat_diff.argtypes = [P, P, C.c_int]
+    L.pgc_graph_derive.argtypes = [P, P, C.c_int, C.POINTER(P)]
+    L.pgc_cycle_ratio_values.argtypes = [P, P, I64, P, P, P, P, P]
+    L.pgc_cycle_ratio_grad.argtypes = [P, P, I64, P, P, P, P, P]
     for name in EXPORTED:
         getattr(L, name)
     _lib = L
@@ -191,6 +196,29 @@ class Graph:
                                       cm.shape[0], int(nodes_per_layer), C.byref(h)))
         return cls(ctx, h)
 
+    @staticmethod
+    def _fd_arg(fd):
+        """(pointer, on_device, keep-alive) of a feat_diff given as a device
+        tensor (float64, contiguous) or a host array."""
+        if hasattr(fd, "is_cuda") and fd.is_cuda:
+            if str(fd.dtype) != "torch.float64" or not fd.is_contiguous():
+                raise ValueError("device feat_diff must be a contiguous float64 tensor")
+            return C.c_void_p(fd.data_ptr()), 1, fd
+        a = np.ascontiguousarray(fd, np.float64)
+        return ptr(a), 0, a
+
+    def set_feat_diff(self, fd):
+        """Recompute the energy from another feat_diff (same graph)."""
+        p, dev, keep = self._fd_arg(fd)
+        check(load().pgc_graph_set_feat_diff(self._h, p, dev))
+
+    def derive(self, fd):
+        """A graph sharing this one's product space and layout with the energy of fd."""
+        p, dev, keep = self._fd_arg(fd)
+        h = C.c_void_p()
+        check(load().pgc_graph_derive(self._h, p, dev, C.byref(h)))
+        return Graph(self.ctx, h)
+
     def info(self):
         e, npl, nl = C.c_int64(), C.c_int64(), C.c_int64()
         pc, cc = C.c_int32(), C.c_int32()
@@ -281,4 +309,35 @@ def solve_ratio_batch(ctx, graphs, max_depth, flags=PGC_LAWLER_PICK_LAST, with_r
         a = int(off[i])
         out.append((eids[a:a + m].copy(), None if rows is None else rows[a:a + m].copy(), res[i]))
     return out
+
+
+def _cycles(cycles):
+    n = len(cycles)
+    lengths = np.array([len(c) for c in cycles], np.int64)
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(lengths, out=off[1:])
+    cat = np.concatenate([np.asarray(c, np.int32) for c in cycles]) if int(off[-1]) else np.zeros(1, np.int32)
+    return np.ascontiguousarray(cat, np.int32), off, lengths
+
+
+def cycle_ratio_values(ctx, graphs, fds, cycles):
+    """Raw-unit ratios sum c / sum t of cycles (edge-id arrays) of graphs, for
+    feat_diffs fds (contiguous float64 device tensors)."""
+    n = len(graphs)
+    cat, off, lengths = _cycles(cycles)
+    handles = (C.c_void_p * n)(*[g._h.value for g in graphs])
+    fdp = (C.c_void_p * n)(*[f.data_ptr() for f in fds])
+    vals = np.zeros(n, np.float64)
+    check(load().pgc_cycle_ratio_values(ctx.handle, handles, n, fdp, ptr(cat), ptr(off), ptr(lengths), ptr(vals)))
+    return vals
+
+
+def cycle_ratio_grad(ctx, graphs, cycles, grad_out, dfds):
+    """dfds[i] += grad_out[i] * d ratio_i / d feat_diff (float64 device tensors)."""
+    n = len(graphs)
+    cat, off, lengths = _cycles(cycles)
+    handles = (C.c_void_p * n)(*[g._h.value for g in graphs])
+    dp = (C.c_void_p * n)(*[d.data_ptr() for d in dfds])
+    go = np.ascontiguousarray(grad_out, np.float64)
+    check(load().pgc_cycle_ratio_grad(ctx.handle, handles, n, ptr(cat), ptr(off), ptr(lengths), ptr(go), dp))
 
