@@ -14,6 +14,15 @@ roofline of the dominant kernel (the persistent solver kernel, HBM-bound:
 algorithmic bytes per sweep = E*(4 src + 4 cost [+ 4 time]) + V_x*4 parent,
 summed over the sweeps of each launch, divided by the launch's HIP-event
 duration), and the CPU oracle (restated reference CPU path) timed on the host.
+
+The "batched" object is the serving-scale workload: 256 shape pairs of uneven
+size (icosphere subdivision in {2,3,4}, |VY| in [32,160], max_depth in
+{1,2,3}), 8 Gaussian feat_diff perturbations (sigma 0.05) per pair = 2304
+lawlergpu solves, each step one forward (energy from the perturbed feat_diff,
+quantization, solve, lambda*) and backward (dL/dfeat_diff) of all 2304 items
+through cyclic_product_graphs.autograd; against the plain per-item loop
+over the same 2304 items (energy + one pgc_solve_ratio call per item), with
+the cycles of both compared item by item.
 """
 import argparse
 import json
@@ -67,6 +76,83 @@ def copy_calibration(mib=256, reps=4):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
 
 
+def batched(args):
+    """Batched forward+backward over uneven pairs vs the plain per-item loop."""
+    import numpy as np
+    import torch
+    from cyclic_product_graphs import _lib, synthetic as S
+    from cyclic_product_graphs import autograd as A
+
+    pairs = S.make_uneven_pairs(args.batch_pairs, seed=0)
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    prob = A.RatioProblem([(p["VX"], p["EX"], p["VY"], p["EY"]) for p, _ in pairs], [K for _, K in pairs])
+    gen = torch.Generator(device=dev).manual_seed(1)
+    base = [torch.tensor(p["feat_diff"], dtype=torch.float32, device=dev) for p, _ in pairs]
+    S_ = args.batch_samples
+    fds, owner = [], []
+    for i, f in enumerate(base):
+        # the unperturbed feat_diff and S_ Gaussian perturbations of it
+        fds.append(f.clone().requires_grad_(True))
+        owner.append(i)
+        for _ in range(S_):
+            fds.append((f + 0.05 * torch.randn(f.shape, generator=gen, device=dev)).requires_grad_(True))
+            owner.append(i)
+    n = len(fds)
+
+    def step():
+        for f in fds:
+            f.grad = None
+        lam = A.cycle_ratio(prob, fds, owner)
+        lam.sum().backward()
+        torch.cuda.synchronize()
+        return lam
+
+    step()  # allocates every item's graph and scratch
+    setup_s = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    b0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        step()
+    b_el = time.perf_counter() - b0
+    res = prob.last_results
+    cycles = prob.last_cycles
+    kms = res[0].kernel_ms
+    relax = sum(r.relaxations for r in res)
+    sweeps = sum(r.sweeps for r in res)
+
+    # plain loop over the same items: energy from the item's feat_diff, then
+    # one pgc_solve_ratio call per item, on one graph per pair
+    plain = [prob.bases[i].derive(fds[i * (S_ + 1)].detach().double().contiguous()) for i in range(len(pairs))]
+    fd64 = [f.detach().double().contiguous() for f in fds]
+    torch.cuda.synchronize()
+    same = True
+    p0 = time.perf_counter()
+    for i in range(n):
+        g = plain[owner[i]]
+        g.set_feat_diff(fd64[i])
+        e, _, r = g.solve_ratio(pairs[owner[i]][1])
+        same &= bool(np.array_equal(e, cycles[i]))
+    p_el = time.perf_counter() - p0
+    for g in plain:
+        g.close()
+    prob.close()
+    return {
+        "workload": "%d uneven pairs x (1 + %d perturbations) = %d lawlergpu solves, forward+backward" % (
+            len(pairs), S_, n),
+        "batched_solves_per_s": n * args.batch_steps / b_el,
+        "batched_ms_per_step": b_el / args.batch_steps * 1e3,
+        "batched_solver_kernel_ms": kms,
+        "batched_gteps": relax / (kms * 1e-3) / 1e9,
+        "batched_sweeps": sweeps,
+        "plain_loop_solves_per_s": n / p_el,
+        "plain_loop_s": p_el,
+        "batched_vs_plain": (n * args.batch_steps / b_el) / (n / p_el),
+        "same_cycles_as_plain_loop": same,
+        "setup_s": setup_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +160,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched serving-scale workload")
+    ap.add_argument("--batch-pairs", type=int, default=256)
+    ap.add_argument("--batch-samples", type=int, default=8, help="perturbations per pair (plus the unperturbed one)")
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -165,6 +255,8 @@ def main():
     except Exception as ex:  # the checker is optional at run time
         print("parity check skipped: %s" % ex, file=sys.stderr)
 
+    batch = None if args.no_batch else batched(args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, K, args.cpu_seconds)
@@ -200,6 +292,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_sweep * sweeps / launches,
                      "avg_launch_ms": kms / launches},
         "cpu_baseline": cpu,
+        "batched": batch,
     }
     print(json.dumps(line))
 

@@ -147,6 +147,7 @@ struct pgc_graph {
   int32_t width[2] = {0, 0};
   int32_t lstride[2] = {0, 0};
   int32_t maxw = 0;
+  int32_t max_in_degree = 0;
   int32_t* d_cnt = nullptr;
   void* d_cost[2] = {nullptr, nullptr};
   float* d_time[2] = {nullptr, nullptr};
@@ -273,11 +274,17 @@ int build_layout(pgc_graph* g) {
   int32_t wn[4];
   HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  g->width[0] = wn[0];
-  g->width[1] = wn[1];
+  // both classes get the solver's slot width: the smallest of 8, 16, 32 that
+  // holds the largest in-degree (wider graphs are rejected by check_width)
+  const int32_t mx = std::max(wn[0], wn[1]);
+  const int32_t W = mx <= 8 ? 8 : (mx <= 16 ? 16 : (mx <= 32 ? 32 : mx));
+  g->width[0] = W;
+  g->width[1] = W;
+  g->max_in_degree = mx;
   g->intra_slots_valid = wn[2];
   g->inter_slots_valid = wn[3];
-  g->maxw = std::max(g->width[0], g->width[1]);
+  HIPCHK(hipMemcpyAsync(g->d_width, g->width, 2 * 4, hipMemcpyHostToDevice, st));
+  g->maxw = g->max_in_degree;
   g->dropped = dropped;
   for (int c = 0; c < 2; ++c) {
     const int64_t per_layer = (int64_t)nC * g->width[c] * 64;

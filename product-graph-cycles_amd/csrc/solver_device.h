@@ -103,6 +103,12 @@ struct BarrierOr {
 
 __device__ __forceinline__ uint64_t wclock() { return wall_clock64(); }
 
+}  // namespace pgc
+
+#include "sweep_device.h"
+
+namespace pgc {
+
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ int block_sum(Shared& sh, int x) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -227,169 +233,15 @@ struct Solver {
     return v < npl ? v : -1;
   }
 
-  // Slots of one owned chunk (pass i) of a layer block: the class width w
-  // (wave-uniform, the same for every chunk) gives independent loads issued
-  // back to back with no table lookup in between. Padding slots point at the
-  // sentinel vertex npl (distance +inf); slots j >= w are set to it.
-  __device__ __forceinline__ void load_pass(const SlotClass& S, int b, int w, int (&s)[W], CT (&c)[W],
-                                            float (&t)[W]) const {
-    const PGC_GLOBAL int* __restrict__ src = gp(S.src);
-    const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)S.cost);
-    const PGC_GLOBAL float* __restrict__ tim = gp(S.time);
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      if (j < w) {
-        s[j] = src[b + j * 64];
-        c[j] = cost[b + j * 64];
-        if (HAS_TIME) t[j] = tim[b + j * 64];
-      } else {
-        s[j] = npl;
-        c[j] = 0;
-        if (HAS_TIME) t[j] = 0.f;
-      }
-    }
-  }
-
-  // c - lambda*t; rt = lambda*tconst (the same product for every edge when
-  // the time is constant, so hoisting it is exact)
-  __device__ __forceinline__ double weight(CT c, float t, double ratio, double rt) const {
-    if (MINCOST) return (double)c;
-    return __dsub_rn((double)c, HAS_TIME ? __dmul_rn(ratio, (double)t) : rt);
-  }
-
-  // Interlayer step of one owned vertex (depth 0 of layer `layer`).
-  __device__ __forceinline__ bool inter_vertex(int i, int v, int layer, int cur, int prev, const int (&s)[W],
-                                               const CT (&c)[W], const float (&t)[W], double ratio, double rt,
-                                               double ub, double (&dmin_r)[MAXC]) {
-    double best = PGC_INF;
-    int bs = npl;
-    CT bc = 0;
-    float bt = 0;
-    const double* Dp = (K == 1) ? D + prev * ds : Dmin;
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      const double nv = Dp[s[j]] + weight(c[j], t[j], ratio, rt);
-      const bool better = nv < best;
-      best = better ? nv : best;
-      bs = better ? s[j] : bs;
-      bc = better ? c[j] : bc;
-      if (HAS_TIME) bt = better ? t[j] : bt;
-    }
-    double res = PGC_INF;
-    bool imp = false;
-    if (best < PGC_INF && best < ub) {
-      int ks = 0;
-      if (K > 1) {
-        const double bw = weight(bc, bt, ratio, rt);
-        for (int kk = 0; kk < K; ++kk) {
-          if (D[(prev + kk) * ds + bs] + bw == best) {
-            ks = kk;
-            break;
-          }
-        }
-      }
-      gp(J.pred)[(int64_t)layer * npl * K + v] = bs + (layer - 1) * npl * K + ks * npl;
-      res = best;
-      imp = true;
-    }
-    D[cur * ds + v] = res;
-    dmin_r[i] = res;
-    return imp;
-  }
-
-  // Intralayer step (depth k) of one owned vertex.
-  __device__ __forceinline__ bool intra_vertex(int i, int v, int layer, int cur, int k, const int (&s)[W],
-                                               const CT (&c)[W], const float (&t)[W], double ratio, double rt,
-                                               double ub, double (&dmin_r)[MAXC]) {
-    double best = PGC_INF;
-    int bs = npl;
-    const double* Dk = D + (cur + k - 1) * ds;
-#pragma unroll
-    for (int j = 0; j < W; ++j) {
-      const double nv = Dk[s[j]] + weight(c[j], t[j], ratio, rt);
-      const bool better = nv < best;
-      best = better ? nv : best;
-      bs = better ? s[j] : bs;
-    }
-    const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
-    double res = curv;
-    bool imp = false;
-    if (best < curv && best < ub) {
-      gp(J.pred)[(int64_t)layer * npl * K + (int64_t)k * npl + v] = bs + layer * npl * K + (k - 1) * npl;
-      res = best;
-      imp = true;
-    }
-    D[(cur + k) * ds + v] = res;
-    dmin_r[i] = res < dmin_r[i] ? res : dmin_r[i];
-    if (k == K - 1) Dmin[v] = dmin_r[i];
-    return imp;
-  }
-
-  // One step over the owned chunks: the slots of every pass are loaded up
-  // front (independent loads, one memory latency per step), then computed.
-  template <bool INTER>
-  __device__ __forceinline__ bool step(int layer, int cur, int prev, int k, double ratio, double rt, double ub,
-                                       double (&dmin_r)[MAXC]) {
-    const SlotClass& S = INTER ? J.inter : J.intra;
-    const int w = S.w;
-    const int base = layer * S.lstride + lane;
-    int s[MAXC][W];
-    CT c[MAXC][W];
-    float t[MAXC][W];
-    bool imp = false;
-#pragma unroll
-    for (int i = 0; i < MAXC; ++i) {
-      const int ci = chunk_of(i);
-      // passes past the last chunk read chunk 0 (in bounds) and are discarded
-      load_pass(S, base + (ci < nC ? ci * w * 64 : 0), w, s[i], c[i], t[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < MAXC; ++i) {
-      const int v = vert(i);
-      if (v >= 0) {
-        if (INTER)
-          imp |= inter_vertex(i, v, layer, cur, prev, s[i], c[i], t[i], ratio, rt, ub, dmin_r);
-        else
-          imp |= intra_vertex(i, v, layer, cur, k, s[i], c[i], t[i], ratio, rt, ub, dmin_r);
-      }
-    }
-    return imp;
-  }
-
-  // One pass over layers 0..L. Returns 0 when a layer 0<l<L had no improvement.
+  // One pass over layers 0..L (the sweep runs in its own function, which
+  // gets the whole register file; the caller's state is saved once per sweep).
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
-    const double rt = __dmul_rn(ratio, J.time_const);
-    double dmin_r[MAXC];
-    for (int layer = 0; layer <= L; ++layer) {
-      const int cur = (layer & 1) * K;
-      const int prev = K - cur;
-      bool imp = false;
-      if (layer == 0) {
-#pragma unroll
-        for (int i = 0; i < MAXC; ++i) {
-          const int v = vert(i);
-          if (v >= 0) dmin_r[i] = D[v];
-        }
-      } else {
-        imp = step<true>(layer, cur, prev, 0, ratio, rt, ub, dmin_r);
-        if (K == 1) {
-          const bool any = bor(sh, imp);
-          if (!any && layer < L) return 0;
-          continue;
-        }
-        __syncthreads();
-      }
-      for (int k = 1; k < K; ++k) {
-        imp |= step<false>(layer, cur, prev, k, ratio, rt, ub, dmin_r);
-        if (k == K - 1) {
-          const bool any = bor(sh, imp);
-          if (layer > 0 && layer < L && !any) return 0;
-        } else {
-          __syncthreads();
-        }
-      }
-    }
-    return 1;
+    const SweepState r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>(&J, D, Dmin, &sh, bor.prev0, bor.prev1,
+                                                                      bor.odd ? 1 : 0, ratio, ub);
+    bor.prev0 = r.p0;
+    bor.prev1 = r.p1;
+    bor.odd = r.odd != 0;
+    return r.complete;
   }
 
   // D <- inf, branch sources <- 0, pred of sources <- -1; returns #sources.

@@ -1,0 +1,333 @@
+// The layer sweep of the solver kernels: one Bellman-Ford pass over layers
+// 0..L of the layered product graph (interlayer step, then the K-1
+// intralayer depth steps of every layer), relaxing the distance rows in LDS.
+//
+// It is compiled as a separate (non-inlined) device function: the solver
+// kernel around it keeps a large state (branch and bound, Lawler iteration,
+// cycle extraction) live, and inside one function that state would take the
+// registers the sweep needs for its slot prefetch and its LDS gathers. All
+// pointers carry explicit address spaces (LDS for the distance rows and the
+// barrier counters, global for the job), so the function emits no FLAT
+// accesses.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pgc_internal.h"
+
+namespace pgc {
+
+#define PGC_LDS __attribute__((address_space(3)))
+
+struct SweepState {
+  int complete;  // 0 when a layer 0<l<L had no improvement
+  int p0, p1, odd;  // barrier-or state (see BarrierOr)
+};
+
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
+struct Sweep {
+  const PGC_GLOBAL SolverJob* J;
+  PGC_LDS double* D;     // [2K][ds] distance rows; entry npl of every row is +inf
+  PGC_LDS double* Dmin;  // [ds] minimum over the depths of the previous layer
+  PGC_LDS Shared* sh;
+  PGC_GLOBAL int32_t* pred;
+  SlotClass Sa, Sx;  // intralayer and interlayer slot classes, read once per sweep
+  int npl, ds, K, L, nC, wave, lane, nw;
+  int p0, p1, odd;
+
+  __device__ __forceinline__ int chunk_of(int i) const { return wave + i * nw; }
+
+  __device__ __forceinline__ int vert(int i) const {
+    const int c = wave + i * nw;
+    if (c >= nC) return -1;
+    const int v = c * 64 + lane;
+    return v < npl ? v : -1;
+  }
+
+  // Block-wide OR with one barrier (the scheme of BarrierOr).
+  __device__ __forceinline__ bool bor(bool pred_) {
+    PGC_LDS int* cnt = odd ? &sh->orCount[1] : &sh->orCount[0];
+    const unsigned long long b = __ballot(pred_);
+    if (b && lane == 0) __atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED);
+    __syncthreads();
+    const int c = *cnt;
+    bool any;
+    if (odd) {
+      any = c != p1;
+      p1 = c;
+    } else {
+      any = c != p0;
+      p0 = c;
+    }
+    odd = !odd;
+    return any;
+  }
+
+  // Slots of one owned chunk (pass i) of a layer block. The layout's slot
+  // width is this kernel's W, so the W loads are unconditional and
+  // independent (no branch between them: the compiler then counts them
+  // exactly and a later wait covers only the loads it needs). Padding slots
+  // point at the sentinel vertex npl (distance +inf) with cost 0.
+  __device__ __forceinline__ void load_pass(const int32_t* src_, const void* cost_, const float* time_, int b,
+                                            int (&s)[W], CT (&c)[W], float (&t)[W]) const {
+    const PGC_GLOBAL int* __restrict__ src = gp(src_);
+    const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)cost_);
+    const PGC_GLOBAL float* __restrict__ tim = gp(time_);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      s[j] = src[b + j * 64];
+      c[j] = cost[b + j * 64];
+      if (HAS_TIME) t[j] = tim[b + j * 64];
+    }
+  }
+
+  // c - lambda*t; rt = lambda*tconst (the same product for every edge when
+  // the time is constant, so hoisting it is exact)
+  __device__ __forceinline__ double weight(CT c, float t, double ratio, double rt) const {
+    if (MINCOST) return (double)c;
+    return __dsub_rn((double)c, HAS_TIME ? __dmul_rn(ratio, (double)t) : rt);
+  }
+
+  // Interlayer step of one owned vertex (depth 0 of layer `layer`).
+  __device__ __forceinline__ bool inter_vertex(int i, int v, int layer, int cur, int prev, const int (&s)[W],
+                                               const CT (&c)[W], const float (&t)[W], double ratio, double rt,
+                                               double ub, double (&dmin_r)[MAXC]) {
+    double best = PGC_INF;
+    int bs = npl;
+    CT bc = 0;
+    float bt = 0;
+    const PGC_LDS double* Dp = (K == 1) ? D + prev * ds : Dmin;
+    // all gathers first (independent LDS reads in flight together), then the
+    // first-minimum scan in slot order
+    double dv[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double nv = dv[j] + weight(c[j], t[j], ratio, rt);
+      const bool better = nv < best;
+      best = better ? nv : best;
+      bs = better ? s[j] : bs;
+      bc = better ? c[j] : bc;
+      if (HAS_TIME) bt = better ? t[j] : bt;
+    }
+    double res = PGC_INF;
+    bool imp = false;
+    if (best < PGC_INF && best < ub) {
+      int ks = 0;
+      if (K > 1) {
+        const double bw = weight(bc, bt, ratio, rt);
+        for (int kk = 0; kk < K; ++kk) {
+          if (D[(prev + kk) * ds + bs] + bw == best) {
+            ks = kk;
+            break;
+          }
+        }
+      }
+      pred[(int64_t)layer * npl * K + v] = bs + (layer - 1) * npl * K + ks * npl;
+      res = best;
+      imp = true;
+    }
+    D[cur * ds + v] = res;
+    dmin_r[i] = res;
+    return imp;
+  }
+
+  // Intralayer step (depth k) of one owned vertex.
+  __device__ __forceinline__ bool intra_vertex(int i, int v, int layer, int cur, int k, const int (&s)[W],
+                                               const CT (&c)[W], const float (&t)[W], double ratio, double rt,
+                                               double ub, double (&dmin_r)[MAXC]) {
+    double best = PGC_INF;
+    int bs = npl;
+    const PGC_LDS double* Dk = D + (cur + k - 1) * ds;
+    double dv[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) dv[j] = Dk[s[j]];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double nv = dv[j] + weight(c[j], t[j], ratio, rt);
+      const bool better = nv < best;
+      best = better ? nv : best;
+      bs = better ? s[j] : bs;
+    }
+    const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
+    double res = curv;
+    bool imp = false;
+    if (best < curv && best < ub) {
+      pred[(int64_t)layer * npl * K + (int64_t)k * npl + v] = bs + layer * npl * K + (k - 1) * npl;
+      res = best;
+      imp = true;
+    }
+    D[(cur + k) * ds + v] = res;
+    dmin_r[i] = res < dmin_r[i] ? res : dmin_r[i];
+    if (k == K - 1) Dmin[v] = dmin_r[i];
+    return imp;
+  }
+
+  // Slot registers of one (class, layer) block of the owned chunks.
+  struct Slots {
+    int s[MAXC][W];
+    CT c[MAXC][W];
+    float t[MAXC][W];
+  };
+
+  // Issues the loads of the interlayer (inter) or intralayer block of a
+  // layer: independent, back to back, from one load site for both classes.
+  __device__ __forceinline__ void load_block(Slots& b, bool inter, int layer) const {
+    // field-wise selects of wave-uniform values (no indexed struct access)
+    const int32_t* src = inter ? Sx.src : Sa.src;
+    const void* cost = inter ? Sx.cost : Sa.cost;
+    const float* tim = inter ? Sx.time : Sa.time;
+    const int base = layer * (inter ? Sx.lstride : Sa.lstride) + lane;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int ci = chunk_of(i);
+      // passes past the last chunk read chunk 0 (in bounds) and are discarded
+      load_pass(src, cost, tim, base + (ci < nC ? ci * W * 64 : 0), b.s[i], b.c[i], b.t[i]);
+    }
+  }
+
+  // One relaxation step over the owned chunks with a loaded block.
+  template <bool INTER>
+  __device__ __forceinline__ bool compute(const Slots& b, int layer, int cur, int prev, int k, double ratio,
+                                          double rt, double ub, double (&dmin_r)[MAXC]) {
+    bool imp = false;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int v = vert(i);
+      if (v >= 0) {
+        if (INTER)
+          imp |= inter_vertex(i, v, layer, cur, prev, b.s[i], b.c[i], b.t[i], ratio, rt, ub, dmin_r);
+        else
+          imp |= intra_vertex(i, v, layer, cur, k, b.s[i], b.c[i], b.t[i], ratio, rt, ub, dmin_r);
+      }
+    }
+    return imp;
+  }
+
+  // Block b of a sweep: with K > 1 the blocks are intra(0), inter(1),
+  // intra(1), ..., inter(L), intra(L); with K == 1 inter(1), ..., inter(L).
+  __device__ __forceinline__ void block_of(int b, bool& inter, int& layer) const {
+    if (K > 1) {
+      inter = (b & 1) != 0;
+      layer = (b + 1) >> 1;
+    } else {
+      inter = true;
+      layer = b + 1;
+    }
+  }
+
+  // One pass over layers 0..L. Returns 0 when a layer 0<l<L had no improvement.
+  //
+  // The slots of a block do not depend on the distances, so when a wave owns
+  // at most two chunks (two blocks fit in registers) the next block's loads
+  // are issued before the current block is relaxed: a step then waits for
+  // LDS and the barrier, not for global memory. One intralayer block serves
+  // the K-1 intralayer steps of its layer.
+  __device__ __forceinline__ int run_layers(double ratio, double ub) {
+    constexpr bool PF = MAXC <= 2;
+    const double rt = __dmul_rn(ratio, J->time_const);
+    double dmin_r[MAXC];
+    const int nb = K > 1 ? 2 * L + 1 : L;
+    Slots A, B;
+    bool inter;
+    int layer;
+    if (PF && nb > 0) {
+      block_of(0, inter, layer);
+      load_block(A, inter, layer);
+      // wait here for the first block: inside the loop A then only ever
+      // holds copies of B made after B's loads completed, and the relaxation
+      // needs no wait on global memory (the compiler's wait analysis merges
+      // all paths, so a pending first load would put a wait in every step)
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    }
+    bool imp = false;
+    for (int b = 0; b < nb; ++b) {
+      block_of(b, inter, layer);
+      const int cur = (layer & 1) * K;
+      const int prev = K - cur;
+      if (PF) {
+        if (b + 1 < nb) {
+          bool in2;
+          int l2;
+          block_of(b + 1, in2, l2);
+          load_block(B, in2, l2);
+        }
+      } else {
+        load_block(A, inter, layer);
+      }
+      if (inter) {
+        imp = compute<true>(A, layer, cur, prev, 0, ratio, rt, ub, dmin_r);
+        if (K == 1) {
+          const bool any = bor(imp);
+          if (!any && layer < L) return 0;
+        } else {
+          __syncthreads();
+        }
+      } else {
+        if (layer == 0) {
+          imp = false;
+#pragma unroll
+          for (int i = 0; i < MAXC; ++i) {
+            const int v = vert(i);
+            if (v >= 0) dmin_r[i] = D[v];
+          }
+        }
+        for (int k = 1; k < K; ++k) {
+          imp |= compute<false>(A, layer, cur, prev, k, ratio, rt, ub, dmin_r);
+          if (k == K - 1) {
+            const bool any = bor(imp);
+            if (layer > 0 && layer < L && !any) return 0;
+          } else {
+            __syncthreads();
+          }
+        }
+      }
+      if (PF) A = B;
+    }
+    return 1;
+  }
+
+};
+
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
+__device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg, int p0,
+                                                int p1, int odd, double ratio, double ub) {
+  Sweep<CT, HAS_TIME, MINCOST, MAXC, W> w;
+  w.J = (const PGC_GLOBAL SolverJob*)Jp;
+  w.D = (PGC_LDS double*)Dg;
+  w.Dmin = (PGC_LDS double*)Dming;
+  w.sh = (PGC_LDS Shared*)shg;
+  w.pred = (PGC_GLOBAL int32_t*)w.J->pred;
+  w.Sa.src = w.J->intra.src;
+  w.Sa.cost = w.J->intra.cost;
+  w.Sa.time = w.J->intra.time;
+  w.Sa.eid = w.J->intra.eid;
+  w.Sa.w = w.J->intra.w;
+  w.Sa.lstride = w.J->intra.lstride;
+  w.Sx.src = w.J->inter.src;
+  w.Sx.cost = w.J->inter.cost;
+  w.Sx.time = w.J->inter.time;
+  w.Sx.eid = w.J->inter.eid;
+  w.Sx.w = w.J->inter.w;
+  w.Sx.lstride = w.J->inter.lstride;
+  w.npl = w.J->npl;
+  w.ds = w.npl + 1;
+  w.K = w.J->K;
+  w.L = w.J->L;
+  w.nC = w.J->nChunks;
+  w.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  w.lane = threadIdx.x & 63;
+  w.nw = PGC_BLOCK / PGC_WAVE;
+  w.p0 = p0;
+  w.p1 = p1;
+  w.odd = odd;
+  SweepState r;
+  r.complete = w.run_layers(ratio, ub);
+  r.p0 = w.p0;
+  r.p1 = w.p1;
+  r.odd = w.odd;
+  return r;
+}
+
+}  // namespace pgc

@@ -17,6 +17,7 @@ Construction:
 from __future__ import annotations
 
 import numpy as np
+from scipy.spatial.distance import cdist
 
 
 def icosphere(subdiv: int):
@@ -90,7 +91,7 @@ def make_pair(subdiv: int = 3, n_y: int = 64, seed: int = 0, dim: int = 32,
     EY = np.stack([np.arange(n_y), (np.arange(n_y) + 1) % n_y], axis=1).astype(np.int32)
     dX = rng.standard_normal((len(VX), dim))
     dY = dX[planted] + sigma_desc * rng.standard_normal((n_y, dim))
-    FD = np.linalg.norm(dX[:, None, :] - dY[None, :, :], axis=-1).astype(np.float32)
+    FD = cdist(dX, dY).astype(np.float32)
     out = dict(VX=VX, EX=EX, VY=VY, EY=EY, feat_diff=FD, planted=planted, F=F)
     if with_normals:
         out["NX"] = vertex_normals_area_weighted(VX, F).astype(np.float32)
@@ -114,3 +115,19 @@ def tiny_pair(n_x: int = 4, n_y: int = 4, seed: int = 0, ring: bool = True):
     EY = np.stack([np.arange(n_y), (np.arange(n_y) + 1) % n_y], axis=1).astype(np.int32)
     FD = rng.uniform(0.1, 2.0, (n_x, n_y)).astype(np.float32)
     return dict(VX=VX, EX=EX, VY=VY, EY=EY, feat_diff=FD)
+
+
+def make_uneven_pairs(n_pairs: int = 256, seed: int = 0, subdivisions=(2, 3, 4), n_y_range=(32, 160),
+                      depths=(1, 2, 3)):
+    """Serving-scale batch of shape pairs of uneven size: icosphere subdivision
+    drawn uniformly from `subdivisions`, |VY| uniform in n_y_range (inclusive),
+    max_depth uniform in `depths`, each pair from make_pair's generator with its
+    own seed. Returns a list of (pair, max_depth)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_pairs):
+        sub = int(rng.choice(subdivisions))
+        ny = int(rng.integers(n_y_range[0], n_y_range[1] + 1))
+        K = int(rng.choice(depths))
+        out.append((make_pair(sub, ny, seed=int(rng.integers(1 << 31))), K))
+    return out

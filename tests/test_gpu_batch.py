@@ -106,3 +106,26 @@ def test_batch_argument_errors():
     with pytest.raises(_lib.PgcError):
         _lib.solve_ratio_batch(ctx, [g], 0)
     g.close()
+
+
+def test_sweep_cap_case_golden():
+    # a perturbed pair whose Bellman-Ford calls run to the numNodes-1 sweep
+    # cap (26363 sweeps): single and batched solves against the oracle's
+    # answer stored in tests/golden/bf_cap_case.npz
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "bf_cap_case.npz"))
+    p, K = S.make_uneven_pairs(256, seed=0)[int(z["pair_index"])]
+    assert K == int(z["max_depth"])
+    p = dict(p, feat_diff=z["feat_diff"])
+    ctx = _lib.default_context(0)
+    rows, _, _ = O.product_space(p["EX"], p["EY"])
+    g1 = _graph(ctx, p)
+    e1, r1, res1 = g1.solve_ratio(K)
+    gb = _graph(ctx, p)
+    (eb, rb, resb), = _lib.solve_ratio_batch(ctx, [gb], K)
+    for e, r, res in ((e1, r1, res1), (eb, rb, resb)):
+        assert np.array_equal(r, rows[z["edges"]])
+        assert res.objective == float(z["objective"])
+        assert [res.bf_calls, res.sweeps, res.lawler_iters, res.branches] == list(z["stats"])
+    g1.close()
+    gb.close()
