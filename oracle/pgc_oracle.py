@@ -47,6 +47,14 @@ def lib():
                                          P, P, P, P]
         L.orc_last_error.restype = C.c_char_p
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_conj_product_space.restype = C.c_long
+        L.orc_conj_product_space.argtypes = [P, C.c_long, P, C.c_long, C.c_int, P, P, P]
+        L.orc_conj_branch_graph.restype = C.c_long
+        L.orc_conj_branch_graph.argtypes = [P, C.c_long, C.c_int, P, P]
+        L.orc_conj_quaternions.argtypes = [P, P, P, P, P, C.c_long, P, C.c_long, C.c_int, P]
+        L.orc_conj_energy.argtypes = [P, P, P, C.c_long, P, P, C.c_long, P, C.c_long, P, P, P, C.c_long, P,
+                                      C.c_long, C.c_int, C.c_char_p, C.c_char_p, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, P]
         _LIB = L
     return _LIB
 
@@ -148,3 +156,60 @@ def dijkstra_solve(src, trgt, cost0, npl, K, branch_graph=None, cpu_rules=False)
     if rc != 0:
         raise RuntimeError("oracle dijkstra solve failed: %s" % lib().orc_last_error().decode())
     return out[: n.value].copy(), np.float32(obj.value), bool(passes.value), stats
+
+
+# ------------------------------------------------------------ conjugate graph
+def conj_product_space(EX, EY, prune=False):
+    """Rows [E, 6] (src2d, mid2d, trgt2d, src3d, mid3d, trgt3d), SRC, TRGT."""
+    EX = np.ascontiguousarray(EX, dtype=np.int32)
+    EY = np.ascontiguousarray(EY, dtype=np.int32)
+    n = lib().orc_conj_product_space(_p(EX), len(EX), _p(EY), len(EY), int(prune), None, None, None)
+    rows = np.zeros((n, 6), np.int32)
+    src = np.zeros(n, np.int32)
+    trgt = np.zeros(n, np.int32)
+    m = lib().orc_conj_product_space(_p(EX), len(EX), _p(EY), len(EY), int(prune), _p(rows), _p(src), _p(trgt))
+    assert m == n
+    return rows, src, trgt
+
+
+def conj_branch_graph(EX, prune=False):
+    """List (per layer-0 node) of sorted branch-graph values."""
+    EX = np.ascontiguousarray(EX, dtype=np.int32)
+    npl = 3 * len(EX)
+    off = np.zeros(npl + 1, np.int64)
+    n = lib().orc_conj_branch_graph(_p(EX), len(EX), int(prune), _p(off), None)
+    vals = np.zeros(max(n, 1), np.int64)
+    lib().orc_conj_branch_graph(_p(EX), len(EX), int(prune), _p(off), _p(vals))
+    return [list(vals[off[v]:off[v + 1]]) for v in range(npl)]
+
+
+def conj_quaternions(VX, NX, VY, NY, EX, EY, prune=False):
+    """(w, x, y, z) per product node, [3*|EX|*|EY|, 4]."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (VX, NX, VY, NY)]
+    EX = np.ascontiguousarray(EX, dtype=np.int32)
+    EY = np.ascontiguousarray(EY, dtype=np.int32)
+    q = np.zeros((3 * len(EX) * len(EY), 4), np.float64)
+    lib().orc_conj_quaternions(_p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), _p(EX), len(EX), _p(EY), len(EY),
+                               int(prune), _p(q))
+    return q
+
+
+def conj_energy(VX, NX, VY, NY, EX, EY, rows, src, trgt, FD, reg=False, cost_mode="vanilla", time_mode="",
+                rl=(0.7, 0.6, 4.0), prune=False):
+    """Energy [E, ncols]; raises ValueError for an unsupported (cost, time) pair."""
+    VX, NX, VY, NY, FD = [np.ascontiguousarray(x, dtype=np.float64) for x in (VX, NX, VY, NY, FD)]
+    EX = np.ascontiguousarray(EX, dtype=np.int32)
+    EY = np.ascontiguousarray(EY, dtype=np.int32)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    trgt = np.ascontiguousarray(trgt, dtype=np.int32)
+    E = len(rows)
+    out = np.zeros(2 * E, np.float64)
+    nc = lib().orc_conj_energy(_p(VX), _p(NX), _p(VY), len(VY), _p(NY), _p(EX), len(EX), _p(EY), len(EY),
+                               _p(rows), _p(src), _p(trgt), E, _p(FD), len(VX), int(reg), cost_mode.encode(),
+                               time_mode.encode(), float(rl[0]), float(rl[1]), float(rl[2]), int(prune), _p(out))
+    if nc < 0:
+        raise ValueError("unsupported (cost, time) mode pair for the conjugate energy: (%s, %s)"
+                         % (cost_mode, time_mode))
+    return out[: nc * E].reshape(nc, E).T.copy()
+
