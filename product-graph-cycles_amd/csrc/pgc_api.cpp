@@ -45,6 +45,16 @@ hipError_t launch_gather_rows(const int32_t*, int, const int32_t*, int64_t, int3
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
 hipError_t launch_cycle_value(const CycleItem*, int, double*, hipStream_t);
+// conj_build.hip
+hipError_t launch_conj_product_space(const int32_t*, int64_t, const int32_t*, int64_t, int, const int32_t*,
+                                     const int32_t*, const int64_t*, int64_t, int64_t, int32_t*, int32_t*, int32_t*,
+                                     hipStream_t);
+hipError_t launch_conj_quat(const double*, const double*, const double*, const double*, const int32_t*, int64_t,
+                            const int32_t*, int64_t, int, double*, hipStream_t);
+hipError_t launch_conj_energy(const double*, const double*, int64_t, const int32_t*, const int32_t*, const int32_t*,
+                              int64_t, const double*, const double*, int, int, double, double, double, double, double,
+                              double*, hipStream_t);
+hipError_t launch_max_f64(const double*, int64_t, double*, hipStream_t);
 hipError_t launch_cycle_grad(const CycleItem*, int, hipStream_t);
 // solver.hip
 size_t solver_lds_bytes(int npl, int K);
@@ -161,6 +171,7 @@ struct pgc_graph {
   // energy inputs, kept to recompute the energy from another feat_diff
   double *d_vx = nullptr, *d_vy = nullptr;
   int64_t nVX = 0, nVY = 0;
+  bool conjugate = false;
   int line_integral = 0, length_mode = 0;
   // a derived graph shares base's product space, ids, layout and energy
   // inputs (read-only) and owns its energy, costs and solver scratch
@@ -496,6 +507,9 @@ int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, doub
 
 int check_solvable(pgc_graph* g, int K) {
   if (K < 1) return fail(PGC_ERR_ARG, "max_depth must be >= 1");
+  if (g->numNodes % g->npl != 0)
+    return fail(PGC_ERR_UNSUPPORTED, "max(SRC)+1 is not a multiple of the nodes per layer (pruned conjugate graph: "
+                                     "the reference's numLayers drops the last layer)");
   if (solver_lds_bytes((int)g->npl, K) > 160 * 1024 - 1024)
     return fail(PGC_ERR_UNSUPPORTED,
                 "distance table (2K+1)*npl*8 bytes exceeds the 160 KiB LDS of one workgroup");
@@ -556,11 +570,166 @@ namespace {
 
 // Shared body of the two generate entry points; all array arguments are
 // device pointers, EX_host (optional) is the host copy used for validation.
+// stringToMode of the conjugate energy (higherOrderDeformationEnergy.cpp:393-410),
+// including its inverted "penalisedegenerate" test: an unknown name (and "")
+// gives PEN_DEG, "penalisedegenerate" itself gives VANILLA
+enum ConjMode { CM_NOMODE, CM_VANILLA, CM_FEATURE, CM_LENGTH, CM_SO3_PLAIN, CM_SO3_ROBUST, CM_PEN_DEG };
+ConjMode conj_mode(const char* s) {
+  const std::string m(s ? s : "");
+  if (m == "nomode") return CM_NOMODE;
+  if (m == "vanilla") return CM_VANILLA;
+  if (m == "feature") return CM_FEATURE;
+  if (m == "lengthnormalisation") return CM_LENGTH;
+  if (m == "plainso3") return CM_SO3_PLAIN;
+  if (m == "robustso3") return CM_SO3_ROBUST;
+  if (m != "penalisedegenerate") return CM_PEN_DEG;
+  return CM_VANILLA;
+}
+
+// Energy table of higherOrderDeformationEnergy.cpp:281-336. (vanilla, nomode)
+// aborts in the reference (:216-218, :332-335) and is the one-column mean
+// problem here; (length, vanilla) and (penalisedegenerate, vanilla) write a
+// second column into a one-column matrix there (:299-313) and have two
+// columns here. -1: unsupported pair.
+int conj_kind(ConjMode c, ConjMode t) {
+  if (c == CM_VANILLA && (t == CM_VANILLA || t == CM_NOMODE)) return CONJ_VV;
+  if (c == CM_VANILLA && t == CM_LENGTH) return CONJ_VL;
+  if (c == CM_LENGTH && t == CM_VANILLA) return CONJ_LV;
+  if (c == CM_VANILLA && t == CM_PEN_DEG) return CONJ_VP;
+  if (c == CM_PEN_DEG && t == CM_VANILLA) return CONJ_PV;
+  if (c == CM_FEATURE && t == CM_SO3_PLAIN) return CONJ_FSP;
+  if (c == CM_FEATURE && t == CM_SO3_ROBUST) return CONJ_FSR;
+  if (c == CM_SO3_ROBUST && t == CM_FEATURE) return CONJ_SRF;
+  if (c == CM_SO3_PLAIN && t == CM_FEATURE) return CONJ_SPF;
+  return -1;
+}
+
+double robust_loss(double x, double alpha, double c, double pwr) {
+  const double xDivCSqr = std::pow((x / c), pwr);
+  const double absAlphaMinusTwo = std::abs(alpha - 2);
+  return absAlphaMinusTwo / alpha * (std::pow(xDivCSqr / absAlphaMinusTwo + 1, 0.5 * alpha) - 1);
+}
+
+// Conjugate product graph (ProductGraphGenerators::generateConjugate,
+// pgg/product_graph_generators.cpp:21-54): product space, branch graph and
+// energy. Arrays are device pointers; EX_host is the host copy of EX.
+int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_t* dEX, int64_t nEX,
+                  const double* dVY, int64_t nVY, const int32_t* dEY, int64_t nEY, const double* dFD,
+                  int regularising, int prune, const char* cost_mode, const char* time_mode, const double* dNX,
+                  const double* dNY, double rl_alpha, double rl_c, double rl_pwr, const int32_t* EX_host,
+                  pgc_graph** out) {
+  hipStream_t st = ctx->stream;
+  if (!dNX || !dNY)
+    return fail(PGC_ERR_ARG, "the conjugate product graph needs normals NX [|VX| x 3] and NY [|VY| x 3]");
+  const int kind = conj_kind(conj_mode(cost_mode), conj_mode(time_mode));
+  if (kind < 0)
+    return fail(PGC_ERR_UNSUPPORTED, std::string("conjugate energy has no (cost, time) mode (") +
+                                         (cost_mode ? cost_mode : "") + ", " + (time_mode ? time_mode : "") + ")");
+  const ConjMode cm = conj_mode(cost_mode), tm = conj_mode(time_mode);
+  // out-edges of every X vertex in ascending edge id; pairs (j, it) per j
+  std::vector<int32_t> outOff(nVX + 1, 0), outList(nEX);
+  for (int64_t e = 0; e < nEX; ++e) outOff[EX_host[2 * e] + 1]++;
+  for (int64_t v = 0; v < nVX; ++v) outOff[v + 1] += outOff[v];
+  {
+    std::vector<int32_t> fill(outOff.begin(), outOff.end() - 1);
+    for (int64_t e = 0; e < nEX; ++e) outList[fill[EX_host[2 * e]]++] = (int32_t)e;
+  }
+  std::vector<int64_t> pre(nEX + 1, 0);
+  for (int64_t j = 0; j < nEX; ++j) {
+    int64_t p = 0;
+    const int32_t s3 = EX_host[2 * j], m3 = EX_host[2 * j + 1];
+    for (int32_t k = outOff[m3]; k < outOff[m3 + 1]; ++k)
+      if (EX_host[2 * outList[k] + 1] != s3) p++;
+    pre[j + 1] = pre[j] + p;
+  }
+  const int64_t P = pre[nEX];
+  const int64_t intraTotal = prune ? 0 : 2 * P;
+  const int64_t perLayer = intraTotal + (prune ? 2 : 3) * P + 2 * nEX;
+  const int64_t E = nEY * perLayer;
+  if (E >= ((int64_t)1 << 31) - 1) return fail(PGC_ERR_UNSUPPORTED, "conjugate product graph exceeds 2^31 edges");
+  std::unique_ptr<pgc_graph> g(new pgc_graph());
+  g->ctx = ctx;
+  g->E = E;
+  g->npl = 3 * nEX;
+  g->ps_cols = 6;
+  g->conjugate = true;
+  g->ncols = kind == CONJ_VV ? 1 : 2;
+  int32_t *dOff = nullptr, *dList = nullptr;
+  int64_t* dPre = nullptr;
+  double *dQuat = nullptr, *dMax = nullptr;
+  HIPCHK(dalloc(&g->d_rows, E * 6));
+  HIPCHK(dalloc(&g->d_src, E));
+  HIPCHK(dalloc(&g->d_trgt, E));
+  HIPCHK(dalloc(&g->d_energy, E * g->ncols));
+  HIPCHK(dalloc(&dOff, nVX + 1));
+  HIPCHK(dalloc(&dList, nEX));
+  HIPCHK(dalloc(&dPre, nEX + 1));
+  HIPCHK(hipMemcpyAsync(dOff, outOff.data(), (nVX + 1) * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dList, outList.data(), nEX * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dPre, pre.data(), (nEX + 1) * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(launch_conj_product_space(dEX, nEX, dEY, nEY, prune, dOff, dList, dPre, perLayer, intraTotal, g->d_rows,
+                                   g->d_src, g->d_trgt, st));
+  if (regularising) {
+    HIPCHK(dalloc(&dQuat, 4 * 3 * nEX * nEY));
+    HIPCHK(launch_conj_quat(dVX, dNX, dVY, dNY, dEX, nEX, dEY, nEY, prune, dQuat, st));
+  }
+  // maxFeatDiff and maxSO3Val are float in the reference (:211-215)
+  float maxFeatDiff = 0;
+  if (tm == CM_FEATURE) {
+    HIPCHK(dalloc(&dMax, 1));
+    HIPCHK(launch_max_f64(dFD, nVX * nVY, dMax, st));
+    double m = 0;
+    HIPCHK(hipMemcpyAsync(&m, dMax, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    maxFeatDiff = (float)m;
+  }
+  const float maxSO3Val = (float)robust_loss(2 * M_PI, rl_alpha, rl_c, rl_pwr);
+  const int lineIntegral = (cm == CM_LENGTH || tm == CM_LENGTH) ? 1 : 0;
+  HIPCHK(launch_conj_energy(dVX, dVY, nVY, g->d_rows, g->d_src, g->d_trgt, E, dFD, dQuat, kind, lineIntegral,
+                            rl_alpha, rl_c, rl_pwr, (double)maxSO3Val, (double)maxFeatDiff, g->d_energy, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (void* p : {(void*)dOff, (void*)dList, (void*)dPre, (void*)dQuat, (void*)dMax})
+    if (p) (void)hipFree(p);
+  // numNodes = max(SRC) + 1, numLayers = numNodes / npl as run() derives them
+  int rc = finish_graph_meta(g.get());
+  if (rc && rc != PGC_ERR_UNSUPPORTED) return rc;
+  // branchGraph (conj_product_space.cpp:170-225; both of its loops read EX(j, 0))
+  const int64_t npl = g->npl;
+  std::vector<std::vector<int64_t>> bg(npl);
+  for (int64_t j = 0; j < nEX; ++j) {
+    bg[j].push_back(j + nEX);
+    if (!prune) {
+      bg[j + nEX].push_back(j + 2 * nEX);
+      bg[j + 2 * nEX].push_back(j);
+    } else {
+      bg[j + nEX].push_back(j);
+    }
+    const int32_t v = EX_host[2 * j];
+    for (int32_t k = outOff[v]; k < outOff[v + 1]; ++k) {
+      const int64_t it = outList[k];
+      bg[j].push_back(it);
+      bg[j + nEX].push_back(it + nEX);
+      if (!prune) bg[j + 2 * nEX].push_back(it + 2 * nEX);
+    }
+  }
+  g->bg_off.assign(npl + 1, 0);
+  g->bg_val.clear();
+  for (int64_t v = 0; v < npl; ++v) {
+    std::sort(bg[v].begin(), bg[v].end());
+    bg[v].erase(std::unique(bg[v].begin(), bg[v].end()), bg[v].end());
+    g->bg_val.insert(g->bg_val.end(), bg[v].begin(), bg[v].end());
+    g->bg_off[v + 1] = (int64_t)g->bg_val.size();
+  }
+  g->has_bg = true;
+  *out = g.release();
+  return PGC_OK;
+}
+
 int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_t* dEX, int64_t nEX,
                   const double* dVY, int64_t nVY, const int32_t* dEY, int64_t nEY, const double* dFD, int conjugate,
-                  int regularising, int prune, const char* cost_mode, const char* time_mode, const int32_t* EX_host,
+                  int regularising, int prune, const char* cost_mode, const char* time_mode, const double* dNX,
+                  const double* dNY, double rl_alpha, double rl_c, double rl_pwr, const int32_t* EX_host,
                   pgc_graph** out) {
-  if (conjugate) return fail(PGC_ERR_UNSUPPORTED, "conjugate product graph is not available in this build");
   hipStream_t st = ctx->stream;
   std::vector<int32_t> exh;
   if (!EX_host) {
@@ -577,6 +746,9 @@ int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   const int64_t nVXe = (int64_t)mx + 1;  // product_space.cpp:20
   if (nVXe != nVX)
     return fail(PGC_ERR_ARG, "max(EX)+1 differs from |VX|: the product space and the solver would disagree");
+  if (conjugate)
+    return generate_conj(ctx, dVX, nVX, dEX, nEX, dVY, nVY, dEY, nEY, dFD, regularising, prune, cost_mode, time_mode,
+                         dNX, dNY, rl_alpha, rl_c, rl_pwr, EX_host, out);
   std::unique_ptr<pgc_graph> g(new pgc_graph());
   g->ctx = ctx;
   const int64_t E = nEY * ((prune ? 1 : 2) * nEX + nVXe);
@@ -630,11 +802,6 @@ int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const in
                        int conjugate, int regularising, int prune, const char* cost_mode, const char* time_mode,
                        const double* NX, const double* NY, double rl_alpha, double rl_c, double rl_pwr,
                        pgc_graph** out) {
-  (void)NX;
-  (void)NY;
-  (void)rl_alpha;
-  (void)rl_c;
-  (void)rl_pwr;
   if (!ctx || !out || !VX || !EX || !VY || !EY || !feat_diff || nVX <= 0 || nEX <= 0 || nVY <= 0 || nEY <= 0)
     return fail(PGC_ERR_ARG, "invalid arguments");
   for (int64_t i = 0; i < 2 * nEY; ++i)
@@ -653,9 +820,18 @@ int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const in
   HIPCHK(hipMemcpyAsync(dVX, VX, 3 * nVX * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dVY, VY, 3 * nVY * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dFD, feat_diff, nVX * nVY * 8, hipMemcpyHostToDevice, st));
+  double *dNX = nullptr, *dNY = nullptr;
+  if (NX && NY) {
+    HIPCHK(dalloc(&dNX, 3 * nVX));
+    HIPCHK(dalloc(&dNY, 3 * nVY));
+    HIPCHK(hipMemcpyAsync(dNX, NX, 3 * nVX * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dNY, NY, 3 * nVY * 8, hipMemcpyHostToDevice, st));
+  }
   const int rc = generate_impl(ctx, dVX, nVX, dEX, nEX, dVY, nVY, dEY, nEY, dFD, conjugate, regularising, prune,
-                               cost_mode, time_mode, EX, out);
+                               cost_mode, time_mode, dNX, dNY, rl_alpha, rl_c, rl_pwr, EX, out);
   HIPCHK(hipStreamSynchronize(st));
+  if (dNX) (void)hipFree(dNX);
+  if (dNY) (void)hipFree(dNY);
   (void)hipFree(dEX);
   (void)hipFree(dEY);
   (void)hipFree(dVX);
@@ -669,16 +845,11 @@ int pgc_graph_generate_device(pgc_context* ctx, const double* VX, int64_t nVX, c
                               const double* feat_diff, int conjugate, int regularising, int prune,
                               const char* cost_mode, const char* time_mode, const double* NX, const double* NY,
                               double rl_alpha, double rl_c, double rl_pwr, pgc_graph** out) {
-  (void)NX;
-  (void)NY;
-  (void)rl_alpha;
-  (void)rl_c;
-  (void)rl_pwr;
   if (!ctx || !out || !VX || !EX || !VY || !EY || !feat_diff || nVX <= 0 || nEX <= 0 || nVY <= 0 || nEY <= 0)
     return fail(PGC_ERR_ARG, "invalid arguments");
   HIPCHK(hipSetDevice(ctx->device));
   return generate_impl(ctx, VX, nVX, EX, nEX, VY, nVY, EY, nEY, feat_diff, conjugate, regularising, prune,
-                       cost_mode, time_mode, nullptr, out);
+                       cost_mode, time_mode, NX, NY, rl_alpha, rl_c, rl_pwr, nullptr, out);
 }
 
 int pgc_graph_set_branch_graph(pgc_graph* g, const int64_t* offsets, const int64_t* values) {
@@ -743,6 +914,7 @@ void pgc_graph_destroy(pgc_graph* g) {
 
 static int energy_from(pgc_graph* g, const double* feat_diff, int on_device) {
   const pgc_graph* b = g->base ? g->base : g;
+  if (b->conjugate) return fail(PGC_ERR_UNSUPPORTED, "feat_diff updates are available for non-conjugate graphs only");
   if (!b->d_vx) return fail(PGC_ERR_UNSUPPORTED, "the graph was not built by pgc_graph_generate*: no energy inputs");
   hipStream_t st = g->ctx->stream;
   const double* fd = feat_diff;
@@ -772,6 +944,7 @@ int pgc_graph_derive(pgc_graph* base, const double* feat_diff, int on_device, pg
   if (!base || !feat_diff || !out) return fail(PGC_ERR_ARG, "invalid arguments");
   pgc_graph* b = base->base ? base->base : base;
   if (b->released) return fail(PGC_ERR_ARG, "base graph was destroyed");
+  if (b->conjugate) return fail(PGC_ERR_UNSUPPORTED, "derived graphs are available for non-conjugate graphs only");
   if (!b->d_vx) return fail(PGC_ERR_UNSUPPORTED, "the graph was not built by pgc_graph_generate*: no energy inputs");
   HIPCHK(hipSetDevice(b->ctx->device));
   int rc;

@@ -32,6 +32,9 @@ struct SlotClass {
   int32_t lstride;       // slots per layer = nChunks * w * 64
 };
 
+// Conjugate energy (cost, time) kinds of the device kernel.
+enum ConjKind : int32_t { CONJ_VV, CONJ_VL, CONJ_LV, CONJ_VP, CONJ_PV, CONJ_FSP, CONJ_FSR, CONJ_SRF, CONJ_SPF };
+
 enum JobMode : int32_t { MODE_RATIO = 0, MODE_UPPER_BOUND = 1, MODE_MIN_COST = 2, MODE_FULL_RATIO = 3 };
 
 enum Status : int32_t {
