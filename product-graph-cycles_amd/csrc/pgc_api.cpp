@@ -122,6 +122,7 @@ struct JobScratch {
   int32_t *hits = nullptr, *back = nullptr, *branch = nullptr, *nodes_io = nullptr, *eids_io = nullptr;
   double* lb_key = nullptr;
   double* backup = nullptr;
+  double* dtab = nullptr;  // distance table in global memory (graphs beyond the LDS)
   JobResult* res = nullptr;
   SolverJob* djob = nullptr;
   // device-resident branch and bound (batched solves)
@@ -131,7 +132,7 @@ struct JobScratch {
   int vc_cap = 0, vc_stride = 0;
   void release() {
     void* ps[] = {pred, wnext, wlen, chain, tmp_nodes, tmp_eid, neg_nodes, neg_eid, neg_len, tmp_c, tmp_t,
-                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, res, djob,
+                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, dtab, res, djob,
                   hkey, hval, hwhere, vslot, vc_eid, vc_len, vc_c, vc_t};
     for (void* p : ps)
       if (p) (void)hipFree(p);
@@ -356,6 +357,22 @@ int fill_costs(pgc_graph* g, int kind) {
   return PGC_OK;
 }
 
+// Chunks (64-vertex slices) per wave of the solver kernel for this graph:
+// 1, 2, 3, 4 or 8 with the distance table in LDS, 0 for the variant with the
+// table in global memory (tables beyond the LDS or more than 8 chunks per wave).
+int kernel_maxc(const pgc_graph* g, int K) {
+  // PGC_GLOBAL_TABLE=1 selects the global-memory variant for every graph (a
+  // validation switch: both variants compute the same results)
+  static const bool force_global = [] {
+    const char* e = std::getenv("PGC_GLOBAL_TABLE");
+    return e && e[0] == '1';
+  }();
+  if (force_global) return 0;
+  const int per = (int)((g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE));
+  if (per > 8 || solver_lds_bytes((int)g->npl, K) > 160 * 1024 - 1024) return 0;
+  return per <= 4 ? per : 8;
+}
+
 int ensure_scratch(pgc_graph* g, int K) {
   JobScratch& s = g->js;
   if (s.K == K && s.pred) return PGC_OK;
@@ -391,6 +408,7 @@ int ensure_scratch(pgc_graph* g, int K) {
   HIPCHK(dalloc(&s.backup, npl));
   HIPCHK(dalloc(&s.res, 1));
   HIPCHK(dalloc(&s.djob, 1));
+  if (kernel_maxc(g, K) == 0) HIPCHK(dalloc(&s.dtab, (int64_t)(2 * K + 1) * (npl + 1)));
   return PGC_OK;
 }
 
@@ -466,6 +484,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.hval = s.hval;
   j.hwhere = s.hwhere;
   j.vslot = s.vslot;
+  j.dtab = s.dtab;
   j.vc_eid = s.vc_eid;
   j.vc_c = s.vc_c;
   j.vc_t = s.vc_t;
@@ -489,8 +508,8 @@ int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, doub
   pgc_context* c = g->ctx;
   hipStream_t st = c->stream;
   HIPCHK(hipMemcpyAsync(g->js.djob, &j, sizeof(j), hipMemcpyHostToDevice, st));
-  const size_t lds = solver_lds_bytes((int)g->npl, j.K);
-  const int maxc = (g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE);
+  const int maxc = kernel_maxc(g, j.K);
+  const size_t lds = maxc == 0 ? 0 : solver_lds_bytes((int)g->npl, j.K);
   HIPCHK(hipEventRecord(c->ev0, st));
   HIPCHK(launch_solver(g->js.djob, 1, 1, nullptr, maxc, g->maxw, mincost, g->d_time[0] != nullptr,
                        g->cost_kind == 2, lds, st));
@@ -510,11 +529,8 @@ int check_solvable(pgc_graph* g, int K) {
   if (g->numNodes % g->npl != 0)
     return fail(PGC_ERR_UNSUPPORTED, "max(SRC)+1 is not a multiple of the nodes per layer (pruned conjugate graph: "
                                      "the reference's numLayers drops the last layer)");
-  if (solver_lds_bytes((int)g->npl, K) > 160 * 1024 - 1024)
-    return fail(PGC_ERR_UNSUPPORTED,
-                "distance table (2K+1)*npl*8 bytes exceeds the 160 KiB LDS of one workgroup");
-  const int maxc = (int)((g->npl + 63) / 64 + 15) / 16;
-  if (maxc > 8) return fail(PGC_ERR_UNSUPPORTED, "nodes per layer above 8192");
+  if ((int64_t)(g->L + 1) * K * g->npl >= ((int64_t)1 << 31))
+    return fail(PGC_ERR_UNSUPPORTED, "expanded graph exceeds int32 node ids");
   return PGC_OK;
 }
 
@@ -1230,8 +1246,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   std::map<Key, std::vector<int64_t>> groups;
   for (int64_t i = 0; i < n; ++i) {
     pgc_graph* g = graphs[i];
-    const int maxc = (g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE);
-    const int mc = maxc <= 4 ? maxc : 8;
+    const int mc = kernel_maxc(g, max_depth[i]);
     const int mw = g->maxw <= 8 ? 8 : (g->maxw <= 16 ? 16 : 32);
     groups[{mc, mw, g->d_time[0] != nullptr}].push_back(i);
   }
@@ -1250,7 +1265,8 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
       return graphs[a]->E * max_depth[a] > graphs[b]->E * max_depth[b];
     });
     size_t lds = 0;
-    for (int64_t i : idx) lds = std::max(lds, solver_lds_bytes((int)graphs[i]->npl, max_depth[i]));
+    if (kv.first.maxc != 0)
+      for (int64_t i : idx) lds = std::max(lds, solver_lds_bytes((int)graphs[i]->npl, max_depth[i]));
     launches.push_back({kv.first, (int64_t)ordered.size(), (int64_t)idx.size(), lds});
     for (int64_t i : idx) ordered.push_back(jobs[i]);
   }

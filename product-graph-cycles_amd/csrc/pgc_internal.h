@@ -134,6 +134,7 @@ struct SolverJob {
   int32_t* vc_len;        // [vc_cap]
   int32_t* out_eid;       // [out_cap] output cycle
   int32_t vc_cap, vc_stride, out_cap, pad3;
+  double* dtab;           // [(2K+1)*(npl+1)] distance table in global memory (graphs beyond the LDS)
 };
 
 // One cycle of the raw-unit ratio value / gradient kernels.

@@ -201,13 +201,17 @@ __device__ __forceinline__ bool lookup_edge(const SolverJob& J, int a, int b, in
 }
 
 // ---------------------------------------------------------------- kernel
+// MAXC = chunks (64-vertex slices) per wave, unrolled; MAXC == 0: any number
+// of chunks per wave in runtime loops, with the distance table in global
+// memory (graphs whose table exceeds the LDS).
 template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
 struct Solver {
+  static constexpr bool DYN = MAXC == 0;
   const SolverJob& J;
   double* D;     // [2K][ds] distance rows; entry npl of every row is +inf (padding slots point there)
   double* Dmin;  // [ds]
   Shared& sh;
-  int npl, ds, K, L, nC, wave, lane, nw;
+  int npl, ds, K, L, nC, wave, lane, nw, npass;
   BarrierOr bor;
   uint64_t tSweep = 0, tFold = 0, tChase = 0, tWalk = 0, tExtract = 0;
 
@@ -221,7 +225,11 @@ struct Solver {
     wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     lane = threadIdx.x & 63;
     nw = PGC_BLOCK / PGC_WAVE;
+    npass = DYN ? (nC + nw - 1) / nw : MAXC;
   }
+
+  // passes over the owned chunks (a constant when MAXC > 0)
+  __device__ __forceinline__ int np() const { return DYN ? npass : MAXC; }
 
   // geometry of owned chunk i (wave-uniform: scalar loads)
   __device__ __forceinline__ int chunk_of(int i) const { return wave + i * nw; }
@@ -236,8 +244,13 @@ struct Solver {
   // One pass over layers 0..L (the sweep runs in its own function, which
   // gets the whole register file; the caller's state is saved once per sweep).
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
-    const SweepState r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>(&J, D, Dmin, &sh, bor.prev0, bor.prev1,
-                                                                      bor.odd ? 1 : 0, ratio, ub);
+    SweepState r;
+    if constexpr (DYN)
+      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>(&J, D, Dmin, &sh, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
+                                                        ratio, ub);
+    else
+      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>(&J, D, Dmin, &sh, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
+                                                       ratio, ub);
     bor.prev0 = r.p0;
     bor.prev1 = r.p1;
     bor.odd = r.odd != 0;
@@ -251,7 +264,7 @@ struct Solver {
     __syncthreads();
     int cnt = 0, last = -1;
 #pragma unroll
-    for (int i = 0; i < MAXC; ++i) {
+    for (int i = 0; i < np(); ++i) {
       const int v = vert(i);
       if (v < 0) continue;
       const bool in = onlyV >= 0 ? (v == onlyV) : (gp(J.branch)[v] == bid);
@@ -281,7 +294,7 @@ struct Solver {
     const int lastRow = L % 2;
     bool imp = false;
 #pragma unroll
-    for (int i = 0; i < MAXC; ++i) {
+    for (int i = 0; i < np(); ++i) {
       const int v = vert(i);
       if (v < 0 || !in_branch(v, bid, onlyV)) continue;
       const double dl = D[lastRow * ds + v];
@@ -301,53 +314,86 @@ struct Solver {
   // layer-0 vertex (one wrap), recording the original node ids.
   __device__ __forceinline__ void chase_wraps() {
     const int stride = J.chain_stride;
-    int p[MAXC], pos[MAXC];
-    bool act[MAXC];
-#pragma unroll
-    for (int i = 0; i < MAXC; ++i) {
-      const int v = vert(i);
-      act[i] = v >= 0;
-      pos[i] = 0;
-      p[i] = act[i] ? gp(J.pred)[v] : -1;
-      if (act[i] && p[i] == -1) {
-        gp(J.wnext)[v] = -1;
-        gp(J.wlen)[v] = 0;
-        act[i] = false;
-      }
-    }
-    bool any = true;
-    while (any) {
-      any = false;
-#pragma unroll
-      for (int i = 0; i < MAXC; ++i) {
-        if (!act[i]) continue;
+    if constexpr (DYN) {
+      // one chain at a time per thread
+      for (int i = 0; i < npass; ++i) {
         const int v = vert(i);
-        const int pp = p[i];
-        if (pp < npl) {
-          gp(J.chain)[(int64_t)v * stride + pos[i]] = pp;
-          pos[i]++;
-          gp(J.wnext)[v] = pp;
-          gp(J.wlen)[v] = pos[i];
-          act[i] = false;
-          continue;
+        if (v < 0) continue;
+        int pp = gp(J.pred)[v];
+        int pos = 0;
+        while (true) {
+          if (pp == -1) {
+            gp(J.wnext)[v] = -1;
+            gp(J.wlen)[v] = pos;
+            break;
+          }
+          if (pp < npl) {
+            gp(J.chain)[(int64_t)v * stride + pos] = pp;
+            pos++;
+            gp(J.wnext)[v] = pp;
+            gp(J.wlen)[v] = pos;
+            break;
+          }
+          if (pos >= stride - 1) {
+            gp(J.wnext)[v] = -2;
+            gp(J.wlen)[v] = pos;
+            break;
+          }
+          gp(J.chain)[(int64_t)v * stride + pos] = orig_index(npl, K, pp);
+          pos++;
+          pp = gp(J.pred)[pp];
         }
-        if (pos[i] >= stride - 1) {
-          gp(J.wnext)[v] = -2;
-          gp(J.wlen)[v] = pos[i];
-          act[i] = false;
-          continue;
-        }
-        gp(J.chain)[(int64_t)v * stride + pos[i]] = orig_index(npl, K, pp);
-        pos[i]++;
-        const int np = gp(J.pred)[pp];
-        p[i] = np;
-        if (np == -1) {
+      }
+    } else {
+      constexpr int NR = DYN ? 1 : MAXC;
+      int p[NR], pos[NR];
+      bool act[NR];
+#pragma unroll
+      for (int i = 0; i < np(); ++i) {
+        const int v = vert(i);
+        act[i] = v >= 0;
+        pos[i] = 0;
+        p[i] = act[i] ? gp(J.pred)[v] : -1;
+        if (act[i] && p[i] == -1) {
           gp(J.wnext)[v] = -1;
-          gp(J.wlen)[v] = pos[i];
+          gp(J.wlen)[v] = 0;
           act[i] = false;
-          continue;
         }
-        any = true;
+      }
+      bool any = true;
+      while (any) {
+        any = false;
+#pragma unroll
+        for (int i = 0; i < np(); ++i) {
+          if (!act[i]) continue;
+          const int v = vert(i);
+          const int pp = p[i];
+          if (pp < npl) {
+            gp(J.chain)[(int64_t)v * stride + pos[i]] = pp;
+            pos[i]++;
+            gp(J.wnext)[v] = pp;
+            gp(J.wlen)[v] = pos[i];
+            act[i] = false;
+            continue;
+          }
+          if (pos[i] >= stride - 1) {
+            gp(J.wnext)[v] = -2;
+            gp(J.wlen)[v] = pos[i];
+            act[i] = false;
+            continue;
+          }
+          gp(J.chain)[(int64_t)v * stride + pos[i]] = orig_index(npl, K, pp);
+          pos[i]++;
+          const int nxt = gp(J.pred)[pp];
+          p[i] = nxt;
+          if (nxt == -1) {
+            gp(J.wnext)[v] = -1;
+            gp(J.wlen)[v] = pos[i];
+            act[i] = false;
+            continue;
+          }
+          any = true;
+        }
       }
     }
   }
@@ -464,7 +510,7 @@ struct Solver {
     int64_t iter = 0;
     while (true) {
 #pragma unroll
-      for (int i = 0; i < MAXC; ++i) {
+      for (int i = 0; i < np(); ++i) {
         const int v = vert(i);
         if (v >= 0) gp(J.backup)[v] = D[v];
       }
@@ -530,7 +576,7 @@ struct Solver {
       double bestKey = PGC_INF;
       int bestV = 0x7fffffff;
 #pragma unroll
-      for (int i = 0; i < MAXC; ++i) {
+      for (int i = 0; i < np(); ++i) {
         const int v = vert(i);
         if (v < 0 || !in_branch(v, bid, onlyV)) continue;
         const double lb = D[v];
@@ -952,7 +998,7 @@ struct Solver {
       const int stride = J.chain_stride;
       const int64_t maxLen = (int64_t)L * K;
 #pragma unroll
-      for (int i = 0; i < MAXC; ++i) {
+      for (int i = 0; i < np(); ++i) {
         const int v = vert(i);
         if (v < 0) continue;
         if (gp(J.branch)[v] != bid) {
@@ -1029,8 +1075,8 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __res
     const int j = sh.i3;
     if (j >= njobs) break;
     const SolverJob& J = jobs[j];
-    double* D = lds;
-    double* Dmin = lds + 2 * J.K * (J.npl + 1);
+    double* D = MAXC == 0 ? J.dtab : lds;
+    double* Dmin = D + 2 * J.K * (J.npl + 1);
     Solver<CT, HAS_TIME, MINCOST, MAXC, W> s(J, D, Dmin, sh);
     if (MINCOST) {
       s.min_cost();

@@ -24,6 +24,7 @@ template <typename CT, bool HT, bool MC, int W>
 static hipError_t launch_maxc(int maxc, const SolverJob* jobs, int njobs, int grid, int* queue, size_t lds,
                               hipStream_t st) {
   switch (maxc) {
+    case 0: return launch_one<CT, HT, MC, 0, W>(jobs, njobs, grid, queue, lds, st);
     case 1: return launch_one<CT, HT, MC, 1, W>(jobs, njobs, grid, queue, lds, st);
     case 2: return launch_one<CT, HT, MC, 2, W>(jobs, njobs, grid, queue, lds, st);
     case 3: return launch_one<CT, HT, MC, 3, W>(jobs, njobs, grid, queue, lds, st);

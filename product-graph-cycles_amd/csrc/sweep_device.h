@@ -330,4 +330,214 @@ __device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg,
   return r;
 }
 
+// ------------------------------------------------------------------------
+// The same sweep for graphs whose distance table does not fit the LDS: the
+// rows live in global memory (L2-resident), every wave loops over its chunks
+// at run time, and the per-vertex minimum over the depths of a layer is
+// recomputed from the rows at its last intralayer step (the minimum is exact,
+// so this equals the running minimum of the LDS sweep).
+template <typename CT, bool HAS_TIME, bool MINCOST, int W>
+struct SweepG {
+  const PGC_GLOBAL SolverJob* J;
+  PGC_GLOBAL double* D;
+  PGC_GLOBAL double* Dmin;
+  PGC_LDS Shared* sh;
+  PGC_GLOBAL int32_t* pred;
+  SlotClass Sa, Sx;
+  int npl, ds, K, L, nC, wave, lane, nw;
+  int p0, p1, odd;
+
+  __device__ __forceinline__ bool bor(bool pred_) {
+    PGC_LDS int* cnt = odd ? &sh->orCount[1] : &sh->orCount[0];
+    const unsigned long long b = __ballot(pred_);
+    if (b && lane == 0) __atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED);
+    __syncthreads();
+    const int c = *cnt;
+    bool any;
+    if (odd) {
+      any = c != p1;
+      p1 = c;
+    } else {
+      any = c != p0;
+      p0 = c;
+    }
+    odd = !odd;
+    return any;
+  }
+
+  __device__ __forceinline__ double weight(CT c, float t, double ratio, double rt) const {
+    if (MINCOST) return (double)c;
+    return __dsub_rn((double)c, HAS_TIME ? __dmul_rn(ratio, (double)t) : rt);
+  }
+
+  __device__ __forceinline__ void load(const SlotClass& S, int b, int (&s)[W], CT (&c)[W], float (&t)[W]) const {
+    const PGC_GLOBAL int* __restrict__ src = gp(S.src);
+    const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)S.cost);
+    const PGC_GLOBAL float* __restrict__ tim = gp(S.time);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      s[j] = src[b + j * 64];
+      c[j] = cost[b + j * 64];
+      if (HAS_TIME) t[j] = tim[b + j * 64];
+    }
+  }
+
+  __device__ __forceinline__ bool inter_step(int layer, int cur, int prev, double ratio, double rt, double ub) {
+    bool imp = false;
+    const PGC_GLOBAL double* Dp = (K == 1) ? D + prev * ds : Dmin;
+    for (int c = wave; c < nC; c += nw) {
+      int s[W];
+      CT cc[W];
+      float t[W];
+      load(Sx, layer * Sx.lstride + c * W * 64 + lane, s, cc, t);
+      const int v = c * 64 + lane;
+      if (v >= npl) continue;
+      double dv[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
+      double best = PGC_INF;
+      int bs = npl;
+      CT bc = 0;
+      float bt = 0;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
+        const bool better = nv < best;
+        best = better ? nv : best;
+        bs = better ? s[j] : bs;
+        bc = better ? cc[j] : bc;
+        if (HAS_TIME) bt = better ? t[j] : bt;
+      }
+      double res = PGC_INF;
+      if (best < PGC_INF && best < ub) {
+        int ks = 0;
+        if (K > 1) {
+          const double bw = weight(bc, bt, ratio, rt);
+          for (int kk = 0; kk < K; ++kk) {
+            if (D[(prev + kk) * ds + bs] + bw == best) {
+              ks = kk;
+              break;
+            }
+          }
+        }
+        pred[(int64_t)layer * npl * K + v] = bs + (layer - 1) * npl * K + ks * npl;
+        res = best;
+        imp = true;
+      }
+      D[cur * ds + v] = res;
+    }
+    return imp;
+  }
+
+  __device__ __forceinline__ bool intra_step(int layer, int cur, int k, double ratio, double rt, double ub) {
+    bool imp = false;
+    const PGC_GLOBAL double* Dk = D + (cur + k - 1) * ds;
+    for (int c = wave; c < nC; c += nw) {
+      int s[W];
+      CT cc[W];
+      float t[W];
+      load(Sa, layer * Sa.lstride + c * W * 64 + lane, s, cc, t);
+      const int v = c * 64 + lane;
+      if (v >= npl) continue;
+      double dv[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) dv[j] = Dk[s[j]];
+      double best = PGC_INF;
+      int bs = npl;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
+        const bool better = nv < best;
+        best = better ? nv : best;
+        bs = better ? s[j] : bs;
+      }
+      const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
+      double res = curv;
+      if (best < curv && best < ub) {
+        pred[(int64_t)layer * npl * K + (int64_t)k * npl + v] = bs + layer * npl * K + (k - 1) * npl;
+        res = best;
+        imp = true;
+      }
+      D[(cur + k) * ds + v] = res;
+      if (k == K - 1) {
+        double m = D[cur * ds + v];
+        for (int kk = 1; kk < K; ++kk) {
+          const double x = D[(cur + kk) * ds + v];
+          m = x < m ? x : m;
+        }
+        Dmin[v] = m;
+      }
+    }
+    return imp;
+  }
+
+  __device__ __forceinline__ int run_layers(double ratio, double ub) {
+    const double rt = __dmul_rn(ratio, J->time_const);
+    for (int layer = 0; layer <= L; ++layer) {
+      const int cur = (layer & 1) * K;
+      const int prev = K - cur;
+      bool imp = false;
+      if (layer > 0) {
+        imp = inter_step(layer, cur, prev, ratio, rt, ub);
+        if (K == 1) {
+          const bool any = bor(imp);
+          if (!any && layer < L) return 0;
+          continue;
+        }
+        __syncthreads();
+      }
+      for (int k = 1; k < K; ++k) {
+        imp |= intra_step(layer, cur, k, ratio, rt, ub);
+        if (k == K - 1) {
+          const bool any = bor(imp);
+          if (layer > 0 && layer < L && !any) return 0;
+        } else {
+          __syncthreads();
+        }
+      }
+    }
+    return 1;
+  }
+};
+
+template <typename CT, bool HAS_TIME, bool MINCOST, int W>
+__device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
+                                                       int p0, int p1, int odd, double ratio, double ub) {
+  SweepG<CT, HAS_TIME, MINCOST, W> w;
+  w.J = (const PGC_GLOBAL SolverJob*)Jp;
+  w.D = (PGC_GLOBAL double*)Dg;
+  w.Dmin = (PGC_GLOBAL double*)Dming;
+  w.sh = (PGC_LDS Shared*)shg;
+  w.pred = (PGC_GLOBAL int32_t*)w.J->pred;
+  w.Sa.src = w.J->intra.src;
+  w.Sa.cost = w.J->intra.cost;
+  w.Sa.time = w.J->intra.time;
+  w.Sa.eid = w.J->intra.eid;
+  w.Sa.w = w.J->intra.w;
+  w.Sa.lstride = w.J->intra.lstride;
+  w.Sx.src = w.J->inter.src;
+  w.Sx.cost = w.J->inter.cost;
+  w.Sx.time = w.J->inter.time;
+  w.Sx.eid = w.J->inter.eid;
+  w.Sx.w = w.J->inter.w;
+  w.Sx.lstride = w.J->inter.lstride;
+  w.npl = w.J->npl;
+  w.ds = w.npl + 1;
+  w.K = w.J->K;
+  w.L = w.J->L;
+  w.nC = w.J->nChunks;
+  w.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  w.lane = threadIdx.x & 63;
+  w.nw = PGC_BLOCK / PGC_WAVE;
+  w.p0 = p0;
+  w.p1 = p1;
+  w.odd = odd;
+  SweepState r;
+  r.complete = w.run_layers(ratio, ub);
+  r.p0 = w.p0;
+  r.p1 = w.p1;
+  r.odd = w.odd;
+  return r;
+}
+
 }  // namespace pgc
