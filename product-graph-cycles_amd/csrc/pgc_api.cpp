@@ -373,6 +373,21 @@ int kernel_maxc(const pgc_graph* g, int K) {
   return per <= 4 ? per : 8;
 }
 
+// The predecessor table of small graphs lives in LDS after the distance table.
+bool pred_fits_lds(const pgc_graph* g, int K) {
+  if (kernel_maxc(g, K) == 0) return false;
+  const size_t pred = (size_t)(g->L + 1) * K * g->npl * 4;
+  return solver_lds_bytes((int)g->npl, K) + pred <= 160 * 1024 - 1024;
+}
+
+// dynamic LDS of a job's launch
+size_t job_lds_bytes(const pgc_graph* g, int K) {
+  if (kernel_maxc(g, K) == 0) return 0;
+  size_t b = solver_lds_bytes((int)g->npl, K);
+  if (pred_fits_lds(g, K)) b += (size_t)(g->L + 1) * K * g->npl * 4;
+  return b;
+}
+
 int ensure_scratch(pgc_graph* g, int K) {
   JobScratch& s = g->js;
   if (s.K == K && s.pred) return PGC_OK;
@@ -485,6 +500,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.hwhere = s.hwhere;
   j.vslot = s.vslot;
   j.dtab = s.dtab;
+  j.pred_lds = pred_fits_lds(g, K) ? 1 : 0;
   j.vc_eid = s.vc_eid;
   j.vc_c = s.vc_c;
   j.vc_t = s.vc_t;
@@ -509,7 +525,7 @@ int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, doub
   hipStream_t st = c->stream;
   HIPCHK(hipMemcpyAsync(g->js.djob, &j, sizeof(j), hipMemcpyHostToDevice, st));
   const int maxc = kernel_maxc(g, j.K);
-  const size_t lds = maxc == 0 ? 0 : solver_lds_bytes((int)g->npl, j.K);
+  const size_t lds = job_lds_bytes(g, j.K);
   HIPCHK(hipEventRecord(c->ev0, st));
   HIPCHK(launch_solver(g->js.djob, 1, 1, nullptr, maxc, g->maxw, mincost, g->d_time[0] != nullptr,
                        g->cost_kind == 2, lds, st));
@@ -1265,8 +1281,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
       return graphs[a]->E * max_depth[a] > graphs[b]->E * max_depth[b];
     });
     size_t lds = 0;
-    if (kv.first.maxc != 0)
-      for (int64_t i : idx) lds = std::max(lds, solver_lds_bytes((int)graphs[i]->npl, max_depth[i]));
+    for (int64_t i : idx) lds = std::max(lds, job_lds_bytes(graphs[i], max_depth[i]));
     launches.push_back({kv.first, (int64_t)ordered.size(), (int64_t)idx.size(), lds});
     for (int64_t i : idx) ordered.push_back(jobs[i]);
   }

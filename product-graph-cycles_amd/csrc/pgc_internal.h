@@ -133,7 +133,8 @@ struct SolverJob {
   double* vc_t;
   int32_t* vc_len;        // [vc_cap]
   int32_t* out_eid;       // [out_cap] output cycle
-  int32_t vc_cap, vc_stride, out_cap, pad3;
+  int32_t vc_cap, vc_stride, out_cap;
+  int32_t pred_lds;       // 1: pred lives in LDS after the distance table (small graphs)
   double* dtab;           // [(2K+1)*(npl+1)] distance table in global memory (graphs beyond the LDS)
 };
 

@@ -211,12 +211,16 @@ struct Solver {
   double* D;     // [2K][ds] distance rows; entry npl of every row is +inf (padding slots point there)
   double* Dmin;  // [ds]
   Shared& sh;
+  PGC_LDS int32_t* Pl;  // pred in LDS (J.pred_lds), else J.pred in global memory
+  bool predLds;
   int npl, ds, K, L, nC, wave, lane, nw, npass;
   BarrierOr bor;
   uint64_t tSweep = 0, tFold = 0, tChase = 0, tWalk = 0, tExtract = 0;
 
-  __device__ __forceinline__ Solver(const SolverJob& j, double* d, double* dm, Shared& s)
+  __device__ __forceinline__ Solver(const SolverJob& j, double* d, double* dm, Shared& s, int32_t* pl)
       : J(j), D(d), Dmin(dm), sh(s) {
+    Pl = (PGC_LDS int32_t*)pl;
+    predLds = !DYN && J.pred_lds != 0;
     npl = J.npl;
     ds = npl + 1;
     K = J.K;
@@ -226,6 +230,15 @@ struct Solver {
     lane = threadIdx.x & 63;
     nw = PGC_BLOCK / PGC_WAVE;
     npass = DYN ? (nC + nw - 1) / nw : MAXC;
+  }
+
+  // predecessor table (expanded node ids) in LDS or global memory
+  __device__ __forceinline__ int pget(int64_t i) const { return predLds ? Pl[i] : gp(J.pred)[i]; }
+  __device__ __forceinline__ void pset(int64_t i, int v) const {
+    if (predLds)
+      Pl[i] = v;
+    else
+      gp(J.pred)[i] = v;
   }
 
   // passes over the owned chunks (a constant when MAXC > 0)
@@ -249,8 +262,8 @@ struct Solver {
       r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>(&J, D, Dmin, &sh, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
                                                         ratio, ub);
     else
-      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>(&J, D, Dmin, &sh, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
-                                                       ratio, ub);
+      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>(&J, D, Dmin, &sh, (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,
+                                                       bor.prev1, bor.odd ? 1 : 0, ratio, ub);
     bor.prev0 = r.p0;
     bor.prev1 = r.p1;
     bor.odd = r.odd != 0;
@@ -270,7 +283,7 @@ struct Solver {
       const bool in = onlyV >= 0 ? (v == onlyV) : (gp(J.branch)[v] == bid);
       if (in) {
         D[v] = 0;
-        gp(J.pred)[v] = -1;
+        pset(v, -1);
         cnt++;
         last = max(last, v);
       }
@@ -302,7 +315,7 @@ struct Solver {
       if (dl < bk) {
         imp = true;
         D[v] = dl;
-        gp(J.pred)[v] = gp(J.pred)[(int64_t)npl * K * L + v];
+        pset(v, pget((int64_t)npl * K * L + v));
       } else {
         D[v] = bk;
       }
@@ -319,7 +332,7 @@ struct Solver {
       for (int i = 0; i < npass; ++i) {
         const int v = vert(i);
         if (v < 0) continue;
-        int pp = gp(J.pred)[v];
+        int pp = pget(v);
         int pos = 0;
         while (true) {
           if (pp == -1) {
@@ -341,7 +354,7 @@ struct Solver {
           }
           gp(J.chain)[(int64_t)v * stride + pos] = orig_index(npl, K, pp);
           pos++;
-          pp = gp(J.pred)[pp];
+          pp = pget(pp);
         }
       }
     } else {
@@ -353,7 +366,7 @@ struct Solver {
         const int v = vert(i);
         act[i] = v >= 0;
         pos[i] = 0;
-        p[i] = act[i] ? gp(J.pred)[v] : -1;
+        p[i] = act[i] ? pget(v) : -1;
         if (act[i] && p[i] == -1) {
           gp(J.wnext)[v] = -1;
           gp(J.wlen)[v] = 0;
@@ -384,7 +397,7 @@ struct Solver {
           }
           gp(J.chain)[(int64_t)v * stride + pos[i]] = orig_index(npl, K, pp);
           pos[i]++;
-          const int nxt = gp(J.pred)[pp];
+          const int nxt = pget(pp);
           p[i] = nxt;
           if (nxt == -1) {
             gp(J.wnext)[v] = -1;
@@ -529,11 +542,11 @@ struct Solver {
           int cnt = 0;
           int64_t firstV = start + (int64_t)npl * L * K;
           gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, firstV);
-          int64_t p = gp(J.pred)[firstV];
+          int64_t p = pget(firstV);
           gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, p);
           for (int64_t kk = maxLength - 1; kk > 0 && cnt < J.cyc_cap; --kk) {
             if (p == -1) break;
-            p = gp(J.pred)[p];
+            p = pget(p);
             gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, p);
             if (p == start) break;
           }
@@ -1012,11 +1025,11 @@ struct Solver {
         gp(J.lb_key)[v] = key;
         int back = -1, cnt = 0;
         if (!isinf((float)key)) {
-          int64_t p = gp(J.pred)[(int64_t)v + (int64_t)npl * L * K];
+          int64_t p = pget((int64_t)v + (int64_t)npl * L * K);
           if (p != -1) {
             gp(J.chain)[(int64_t)v * stride + cnt++] = orig_index(npl, K, p);
             for (int64_t kk = maxLen; kk > 0; --kk) {
-              const int64_t q = gp(J.pred)[p];
+              const int64_t q = pget(p);
               if (q == -1) break;
               p = q;
               if (cnt < stride) gp(J.chain)[(int64_t)v * stride + cnt] = orig_index(npl, K, p);
@@ -1077,7 +1090,19 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __res
     const SolverJob& J = jobs[j];
     double* D = MAXC == 0 ? J.dtab : lds;
     double* Dmin = D + 2 * J.K * (J.npl + 1);
-    Solver<CT, HAS_TIME, MINCOST, MAXC, W> s(J, D, Dmin, sh);
+    // Small graphs keep pred in LDS after the table. It is loaded from and
+    // written back to the global table around the job: predecessors persist
+    // across launches as in the reference (with an odd layer count the fold
+    // and the lower bounds read a row whose node may not have been relaxed in
+    // the current call, and then follow its predecessor from an earlier one).
+    int32_t* P = (int32_t*)(lds + (int64_t)(2 * J.K + 1) * (J.npl + 1));
+    const bool predLds = MAXC != 0 && J.pred_lds;
+    const int64_t nPred = (int64_t)(J.L + 1) * J.K * J.npl;
+    if (predLds) {
+      for (int64_t i = threadIdx.x; i < nPred; i += PGC_BLOCK) ((PGC_LDS int32_t*)P)[i] = gp(J.pred)[i];
+      __syncthreads();
+    }
+    Solver<CT, HAS_TIME, MINCOST, MAXC, W> s(J, D, Dmin, sh, P);
     if (MINCOST) {
       s.min_cost();
     } else if (J.mode == MODE_UPPER_BOUND) {
@@ -1086,6 +1111,10 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __res
       s.full_ratio();
     } else {
       s.lawler();
+    }
+    if (predLds) {
+      __syncthreads();
+      for (int64_t i = threadIdx.x; i < nPred; i += PGC_BLOCK) gp(J.pred)[i] = ((PGC_LDS int32_t*)P)[i];
     }
     if (!queue) break;
     next += gridDim.x;

@@ -31,6 +31,8 @@ struct Sweep {
   PGC_LDS double* Dmin;  // [ds] minimum over the depths of the previous layer
   PGC_LDS Shared* sh;
   PGC_GLOBAL int32_t* pred;
+  PGC_LDS int32_t* predL;  // pred in LDS (small graphs), selected by predLds
+  int predLds;
   SlotClass Sa, Sx;  // intralayer and interlayer slot classes, read once per sweep
   int npl, ds, K, L, nC, wave, lane, nw;
   int p0, p1, odd;
@@ -124,7 +126,12 @@ struct Sweep {
           }
         }
       }
-      pred[(int64_t)layer * npl * K + v] = bs + (layer - 1) * npl * K + ks * npl;
+      const int64_t pi = (int64_t)layer * npl * K + v;
+      const int pv = bs + (layer - 1) * npl * K + ks * npl;
+      if (predLds)
+        predL[pi] = pv;
+      else
+        pred[pi] = pv;
       res = best;
       imp = true;
     }
@@ -154,7 +161,12 @@ struct Sweep {
     double res = curv;
     bool imp = false;
     if (best < curv && best < ub) {
-      pred[(int64_t)layer * npl * K + (int64_t)k * npl + v] = bs + layer * npl * K + (k - 1) * npl;
+      const int64_t pi = (int64_t)layer * npl * K + (int64_t)k * npl + v;
+      const int pv = bs + layer * npl * K + (k - 1) * npl;
+      if (predLds)
+        predL[pi] = pv;
+      else
+        pred[pi] = pv;
       res = best;
       imp = true;
     }
@@ -291,14 +303,17 @@ struct Sweep {
 };
 
 template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
-__device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg, int p0,
-                                                int p1, int odd, double ratio, double ub) {
+__device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
+                                                int32_t* predLg, int predLds, int p0, int p1, int odd, double ratio,
+                                                double ub) {
   Sweep<CT, HAS_TIME, MINCOST, MAXC, W> w;
   w.J = (const PGC_GLOBAL SolverJob*)Jp;
   w.D = (PGC_LDS double*)Dg;
   w.Dmin = (PGC_LDS double*)Dming;
   w.sh = (PGC_LDS Shared*)shg;
   w.pred = (PGC_GLOBAL int32_t*)w.J->pred;
+  w.predL = (PGC_LDS int32_t*)predLg;
+  w.predLds = predLds;
   w.Sa.src = w.J->intra.src;
   w.Sa.cost = w.J->intra.cost;
   w.Sa.time = w.J->intra.time;
