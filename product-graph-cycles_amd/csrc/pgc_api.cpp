@@ -1446,7 +1446,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   if (rc) return rc;
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int64_t npl = g->npl, L = g->L;
+  const int64_t npl = g->npl;
   const bool cpuRules = (flags & PGC_MINCOST_CPU_RULES) != 0;
   if ((rc = build_layout(g))) return rc;
   if ((rc = check_width(g))) return rc;
@@ -1574,13 +1574,13 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
           upperBound = newUB;
           cycle.resize(clen[v]);
           if (clen[v])
-            HIPCHK(hipMemcpy(cycle.data(), s.chain + v * (L * K + 2), clen[v] * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy2D(cycle.data(), 4, s.chain + v, (size_t)npl * 4, 4, clen[v], hipMemcpyDeviceToHost));
         }
         break;
       } else if (first) {
         if (cpuRules) {
           int32_t front = -1;
-          if (clen[v] > 0) HIPCHK(hipMemcpy(&front, s.chain + v * (L * K + 2), 4, hipMemcpyDeviceToHost));
+          if (clen[v] > 0) HIPCHK(hipMemcpy(&front, s.chain + v, 4, hipMemcpyDeviceToHost));
           bSrc = clen[v] > 0 ? (int)(front - g->numNodes) : -1;
         } else {
           bSrc = (int)v;

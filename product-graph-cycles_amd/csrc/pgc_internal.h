@@ -107,7 +107,7 @@ struct SolverJob {
   double* backup;         // [npl] layer-0 values at the start of a sweep
   int32_t* wnext;         // [npl]
   int32_t* wlen;          // [npl]
-  int32_t* chain;         // [npl*chain_stride] original node ids along each wrap
+  int32_t* chain;         // [chain_stride*npl] original node ids along each wrap; entry j of v at j*npl+v
   int32_t* tmp_nodes;     // [cyc_cap]
   int32_t* tmp_eid;       // [cyc_cap]
   double* tmp_c;          // [cyc_cap]

@@ -25,6 +25,7 @@
 #include <stdint.h>
 
 #include "bnb_device.h"
+#include "fastdiv.h"
 #include "pgc_internal.h"
 
 namespace pgc {
@@ -46,13 +47,6 @@ __device__ __forceinline__ const PGC_GLOBAL T* gp(const T* p) {
   return (const PGC_GLOBAL T*)p;
 }
 
-__device__ __forceinline__ int orig_index(int64_t npl, int64_t K, int64_t p) {
-  const int64_t per = npl * K;
-  const int64_t layer = p / per;
-  const int64_t rem = p - layer * per;
-  const int64_t depth = rem / npl;
-  return (int)(rem - depth * npl + layer * npl);
-}
 
 // std::round then double->int as on x86-64 (INT_MIN outside the int range)
 __device__ __forceinline__ int round_to_int(double w) {
@@ -213,6 +207,7 @@ struct Solver {
   Shared& sh;
   PGC_LDS int32_t* Pl;  // pred in LDS (J.pred_lds), else J.pred in global memory
   bool predLds;
+  FastDiv fdNpl, fdPer;  // division by npl and by npl*K
   int npl, ds, K, L, nC, wave, lane, nw, npass;
   BarrierOr bor;
   uint64_t tSweep = 0, tFold = 0, tChase = 0, tWalk = 0, tExtract = 0;
@@ -230,6 +225,18 @@ struct Solver {
     lane = threadIdx.x & 63;
     nw = PGC_BLOCK / PGC_WAVE;
     npass = DYN ? (nC + nw - 1) / nw : MAXC;
+    fdNpl.init((uint32_t)npl);
+    fdPer.init((uint32_t)(npl * K));
+  }
+
+  // original graph index (layer * npl + vertex) of an expanded node id
+  // (getOriginalGraphIndex, cost_time_ratio_solver.cpp:240-250)
+  __device__ __forceinline__ int orig(int64_t p) const {
+    const uint32_t u = (uint32_t)p;
+    const uint32_t layer = fdPer.div(u);
+    const uint32_t rem = u - layer * (uint32_t)(npl * K);
+    const uint32_t depth = fdNpl.div(rem);
+    return (int)(rem - depth * (uint32_t)npl + layer * (uint32_t)npl);
   }
 
   // predecessor table (expanded node ids) in LDS or global memory
@@ -324,7 +331,9 @@ struct Solver {
   }
 
   // Chase every layer-0 vertex's predecessor chain back to the previous
-  // layer-0 vertex (one wrap), recording the original node ids.
+  // layer-0 vertex (one wrap), recording the original node ids. Entry j of
+  // vertex v's chain is at chain[j * npl + v] (lanes of a wave write
+  // consecutive words).
   __device__ __forceinline__ void chase_wraps() {
     const int stride = J.chain_stride;
     if constexpr (DYN) {
@@ -341,7 +350,7 @@ struct Solver {
             break;
           }
           if (pp < npl) {
-            gp(J.chain)[(int64_t)v * stride + pos] = pp;
+            gp(J.chain)[(int64_t)pos * npl + v] = pp;
             pos++;
             gp(J.wnext)[v] = pp;
             gp(J.wlen)[v] = pos;
@@ -352,7 +361,7 @@ struct Solver {
             gp(J.wlen)[v] = pos;
             break;
           }
-          gp(J.chain)[(int64_t)v * stride + pos] = orig_index(npl, K, pp);
+          gp(J.chain)[(int64_t)pos * npl + v] = orig(pp);
           pos++;
           pp = pget(pp);
         }
@@ -382,7 +391,7 @@ struct Solver {
           const int v = vert(i);
           const int pp = p[i];
           if (pp < npl) {
-            gp(J.chain)[(int64_t)v * stride + pos[i]] = pp;
+            gp(J.chain)[(int64_t)pos[i] * npl + v] = pp;
             pos[i]++;
             gp(J.wnext)[v] = pp;
             gp(J.wlen)[v] = pos[i];
@@ -395,7 +404,7 @@ struct Solver {
             act[i] = false;
             continue;
           }
-          gp(J.chain)[(int64_t)v * stride + pos[i]] = orig_index(npl, K, pp);
+          gp(J.chain)[(int64_t)pos[i] * npl + v] = orig(pp);
           pos[i]++;
           const int nxt = pget(pp);
           p[i] = nxt;
@@ -541,13 +550,13 @@ struct Solver {
           const int64_t maxLength = (int64_t)L * K;
           int cnt = 0;
           int64_t firstV = start + (int64_t)npl * L * K;
-          gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, firstV);
+          gp(J.tmp_nodes)[cnt++] = orig(firstV);
           int64_t p = pget(firstV);
-          gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, p);
+          gp(J.tmp_nodes)[cnt++] = orig(p);
           for (int64_t kk = maxLength - 1; kk > 0 && cnt < J.cyc_cap; --kk) {
             if (p == -1) break;
             p = pget(p);
-            gp(J.tmp_nodes)[cnt++] = orig_index(npl, K, p);
+            gp(J.tmp_nodes)[cnt++] = orig(p);
             if (p == start) break;
           }
           for (int i = 0, j = cnt - 1; i < j; ++i, --j) {
@@ -632,7 +641,7 @@ struct Solver {
           do {
             const int len = gp(J.wlen)[x];
             for (int j = lane; j < len && offp + j < m; j += 64)
-              gp(J.tmp_nodes)[m - 1 - (offp + j)] = gp(J.chain)[(int64_t)x * J.chain_stride + j];
+              gp(J.tmp_nodes)[m - 1 - (offp + j)] = gp(J.chain)[(int64_t)j * npl + x];
             offp += len;
             x = gp(J.wnext)[x];
           } while (x != vs && x >= 0 && offp < m && ++guard < 4 * npl + 8);
@@ -1027,15 +1036,15 @@ struct Solver {
         if (!isinf((float)key)) {
           int64_t p = pget((int64_t)v + (int64_t)npl * L * K);
           if (p != -1) {
-            gp(J.chain)[(int64_t)v * stride + cnt++] = orig_index(npl, K, p);
+            gp(J.chain)[(int64_t)(cnt++) * npl + v] = orig(p);
             for (int64_t kk = maxLen; kk > 0; --kk) {
               const int64_t q = pget(p);
               if (q == -1) break;
               p = q;
-              if (cnt < stride) gp(J.chain)[(int64_t)v * stride + cnt] = orig_index(npl, K, p);
+              if (cnt < stride) gp(J.chain)[(int64_t)cnt * npl + v] = orig(p);
               cnt++;
             }
-            back = orig_index(npl, K, p);
+            back = orig(p);
           }
         }
         gp(J.back)[v] = back;
