@@ -153,6 +153,41 @@ def batched(args):
     }
 
 
+def conjugate(steps=2):
+    """Conjugate product graph with normals: icosphere subdivision 4 (radial
+    noise 0.05) x 100-point curve, max_depth 2, regularising term on, cost
+    vanilla / time nomode (the one-column mean problem). One step = graph
+    construction (product space, node rotations, energy) + one lawlergpu solve."""
+    import numpy as np
+    import cyclic_product_graphs as cpg
+    from cyclic_product_graphs import synthetic as S
+    p = S.make_pair(4, 100, seed=0, with_normals=True)
+    NX, NY = p["NX"].astype(np.float64), p["NY"].astype(np.float64)
+
+    def one():
+        g = cpg.product_graph_generator(p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"], True, True)
+        g.set_normals(NX, NY)
+        g.set_cost_time_ratio_mode("vanilla", "nomode")
+        t0 = time.perf_counter()
+        g.generate()
+        t1 = time.perf_counter()
+        g.solve_with_cost_time_ratio("lawlergpu", 2)
+        t2 = time.perf_counter()
+        return g, t1 - t0, t2 - t1
+
+    one()
+    runs = [one() for _ in range(steps)]
+    g = runs[-1][0]
+    info = g._graph.info()
+    r = g.last_result
+    return {"workload": "conjugate, icosphere s=4 x 100-point curve, max_depth 2, regularised, vanilla/nomode",
+            "n_edges": info["n_edges"], "nodes_per_layer": info["nodes_per_layer"],
+            "generate_s": sum(x[1] for x in runs) / steps, "lawler_solve_s": sum(x[2] for x in runs) / steps,
+            "sweeps": r["sweeps"], "solver_kernel_ms": r["kernel_ms"],
+            "gteps": r["relaxations"] / (r["kernel_ms"] * 1e-3) / 1e9,
+            "distance_table": "global memory (46080 nodes per layer exceed the LDS)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +199,7 @@ def main():
     ap.add_argument("--batch-pairs", type=int, default=256)
     ap.add_argument("--batch-samples", type=int, default=8, help="perturbations per pair (plus the unperturbed one)")
     ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--no-conj", action="store_true", help="skip the conjugate-graph measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -256,6 +292,7 @@ def main():
         print("parity check skipped: %s" % ex, file=sys.stderr)
 
     batch = None if args.no_batch else batched(args)
+    conj = None if args.no_conj else conjugate()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -293,6 +330,7 @@ def main():
                      "avg_launch_ms": kms / launches},
         "cpu_baseline": cpu,
         "batched": batch,
+        "conjugate": conj,
     }
     print(json.dumps(line))
 

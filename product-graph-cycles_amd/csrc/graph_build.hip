@@ -428,7 +428,8 @@ hipError_t launch_layout_count(const int32_t* src, const int32_t* trgt, int64_t 
 hipError_t launch_layout_width(const int32_t* cnt, int64_t nTot, int32_t* width, hipStream_t st) {
   hipError_t e = hipMemsetAsync(width, 0, 4 * 4, st);
   if (e != hipSuccess) return e;
-  const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (nTot + 255) / 256));
+  // few blocks: every wave ends with four atomics on the same four words
+  const unsigned g = (unsigned)std::min<int64_t>(32, std::max<int64_t>(1, (nTot + 255) / 256));
   hipLaunchKernelGGL(k_class_width, dim3(g), dim3(256), 0, st, cnt, nTot, width);
   return hipGetLastError();
 }
