@@ -236,6 +236,94 @@ __global__ void k_quantize(double* __restrict__ col, int64_t n, const SelectStat
   }
 }
 
+// ---- batched forms: one launch over many columns (grid.y = column), the
+// same arithmetic per column as the single-column kernels above
+__global__ void k_bselect_init(const QItem* __restrict__ items, SelectState* __restrict__ states) {
+  SelectState* st = states + blockIdx.x;
+  const int64_t n = items[blockIdx.x].n;
+  const int t = threadIdx.x;
+  if (t < 256) {
+    st->hist[0][t] = 0;
+    st->hist[1][t] = 0;
+  }
+  if (t == 0) {
+    const int64_t k1 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
+    st->prefix[0] = st->prefix[1] = 0;
+    st->mask[0] = st->mask[1] = 0;
+    st->k[0] = k1;
+    st->k[1] = (n % 2 == 0) ? k1 + 1 : k1;
+  }
+}
+
+__global__ void k_bselect_hist(const QItem* __restrict__ items, SelectState* __restrict__ states, int shift) {
+  __shared__ uint32_t h[2][256];
+  const QItem it = items[blockIdx.y];
+  SelectState* st = states + blockIdx.y;
+  h[0][threadIdx.x] = 0;
+  h[1][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t p0 = st->prefix[0], m0 = st->mask[0], p1 = st->prefix[1], m1 = st->mask[1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t kk = dkey(it.col[i]);
+    const uint32_t b = (uint32_t)(kk >> shift) & 255u;
+    if ((kk & m0) == p0) atomicAdd(&h[0][b], 1u);
+    if ((kk & m1) == p1) atomicAdd(&h[1][b], 1u);
+  }
+  __syncthreads();
+  if (h[0][threadIdx.x]) atomicAdd(&st->hist[0][threadIdx.x], h[0][threadIdx.x]);
+  if (h[1][threadIdx.x]) atomicAdd(&st->hist[1][threadIdx.x], h[1][threadIdx.x]);
+}
+
+__global__ void k_bselect_step(SelectState* __restrict__ states, int shift) {
+  SelectState* st = states + blockIdx.x;
+  if (threadIdx.x < 2) {
+    const int t = threadIdx.x;
+    int64_t cum = 0;
+    int b = 0;
+    for (; b < 256; ++b) {
+      const int64_t c = st->hist[t][b];
+      if (st->k[t] < cum + c) break;
+      cum += c;
+    }
+    if (b > 255) b = 255;
+    st->k[t] -= cum;
+    st->prefix[t] |= ((uint64_t)b) << shift;
+    st->mask[t] |= ((uint64_t)255) << shift;
+    if (shift == 0) st->value[t] = dkey_inv(st->prefix[t]);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) (&st->hist[0][0])[i] = 0;
+}
+
+__global__ void k_bquantize(const QItem* __restrict__ items, const SelectState* __restrict__ states, double scaling) {
+  const QItem it = items[blockIdx.y];
+  const SelectState* st = states + blockIdx.y;
+  const double med = (it.n % 2 == 0) ? (st->value[0] + st->value[1]) / 2.0 : st->value[0];
+  const double m = med > 0.001 ? med : 0.001;
+  const double s = scaling / (2.0 * m);
+  const double maxVal = 100 * scaling;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.n; i += (int64_t)gridDim.x * blockDim.x) {
+    double q = round(__dmul_rn(s, it.col[i]));
+    if (q > maxVal) q = maxVal;
+    it.col[i] = q;
+  }
+}
+
+__global__ void k_bfill_costs(const FItem* __restrict__ items) {
+  const FItem it = items[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.nslots;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (it.s[i] < 0 || it.eid[i] < 0) {
+      it.cost[i] = 0.f;
+      if (it.time) it.time[i] = 0.f;
+      continue;
+    }
+    const int64_t e = it.eid[i];
+    it.cost[i] = (float)it.energy[e];
+    if (it.time) it.time[i] = (float)it.energy[it.E + e];
+  }
+}
+
 // ------------------------------------------------- sliced-ELL in-edge layout
 // class 0: source on the target's layer (intralayer), class 1: source on the
 // previous layer (interlayer). Targets on layer 0 whose source id is > npl
@@ -418,6 +506,25 @@ hipError_t launch_quantize(double* col, int64_t n, void* selectState, hipStream_
 }
 
 size_t select_state_bytes() { return sizeof(SelectState); }
+
+// Quantization of n columns (the median of each, then q-units) in 18 launches.
+hipError_t launch_quantize_batch(const QItem* d_items, int n, void* states, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  SelectState* S = (SelectState*)states;
+  hipLaunchKernelGGL(k_bselect_init, dim3(n), dim3(256), 0, st, d_items, S);
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_bselect_hist, dim3(32, n), dim3(256), 0, st, d_items, S, shift);
+    hipLaunchKernelGGL(k_bselect_step, dim3(n), dim3(256), 0, st, S, shift);
+  }
+  hipLaunchKernelGGL(k_bquantize, dim3(32, n), dim3(256), 0, st, d_items, S, 100000.0);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_costs_batch(const FItem* d_items, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bfill_costs, dim3(64, n), dim3(256), 0, st, d_items);
+  return hipGetLastError();
+}
 
 hipError_t launch_layout_count(const int32_t* src, const int32_t* trgt, int64_t E, int64_t npl, int64_t numNodes,
                                int32_t* cnt, int32_t* dropped, hipStream_t st) {

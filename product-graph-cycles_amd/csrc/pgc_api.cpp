@@ -33,6 +33,8 @@ hipError_t launch_product_space(const int32_t*, int64_t, const int32_t*, int64_t
 hipError_t launch_energy(const double*, const double*, const int32_t*, int64_t, const double*, int64_t, int, int,
                          double*, hipStream_t);
 hipError_t launch_quantize(double*, int64_t, void*, hipStream_t);
+hipError_t launch_quantize_batch(const QItem*, int, void*, hipStream_t);
+hipError_t launch_fill_costs_batch(const FItem*, int, hipStream_t);
 size_t select_state_bytes();
 hipError_t launch_layout_count(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int32_t*, int32_t*,
                                hipStream_t);
@@ -107,6 +109,13 @@ struct pgc_context {
   int64_t ceids_cap = 0;
   double* d_values = nullptr;
   int64_t values_cap = 0;
+  // batched quantization and cost fill
+  pgc::QItem* d_qitems = nullptr;
+  int64_t qitems_cap = 0;
+  pgc::FItem* d_fitems = nullptr;
+  int64_t fitems_cap = 0;
+  uint8_t* d_select = nullptr;  // select states of the batched median
+  int64_t select_cap = 0;
 };
 
 namespace {
@@ -231,7 +240,8 @@ void pgc_context_destroy(pgc_context* c) {
   if (c->select_state) (void)hipFree(c->select_state);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values};
+  void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values,
+                c->d_qitems, c->d_fitems, c->d_select};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -324,7 +334,9 @@ int build_layout(pgc_graph* g) {
 }
 
 // kind 1: float q-unit cost (+ float time when a time column exists); kind 2: double raw cost
-int fill_costs(pgc_graph* g, int kind) {
+// Slot cost buffers of the given kind (sized, reused while the size fits);
+// fill_costs then writes every slot.
+int cost_buffers(pgc_graph* g, int kind) {
   hipStream_t st = g->ctx->stream;
   for (int c = 0; c < 2; ++c) {
     const int64_t n = g->nslots[c];
@@ -350,10 +362,20 @@ int fill_costs(pgc_graph* g, int kind) {
         g->time_bytes[c] = tb;
       }
     }
-    HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], n, g->d_energy, g->E, g->ncols, kind == 2, g->d_cost[c],
-                             withTime ? g->d_time[c] : nullptr, st));
   }
   g->cost_kind = kind;
+  return PGC_OK;
+}
+
+int fill_costs(pgc_graph* g, int kind) {
+  int rc = cost_buffers(g, kind);
+  if (rc) return rc;
+  hipStream_t st = g->ctx->stream;
+  for (int c = 0; c < 2; ++c) {
+    const bool withTime = kind == 1 && g->ncols > 1;
+    HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, g->ncols, kind == 2,
+                             g->d_cost[c], withTime ? g->d_time[c] : nullptr, st));
+  }
   return PGC_OK;
 }
 
@@ -1223,22 +1245,45 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     for (int64_t k = 0; k < i && k < 64; ++k)
       if (graphs[k] == g) return fail(PGC_ERR_ARG, "a graph appears twice in one batch");
   }
-  // per graph: in-place quantization, layout, fp32 slot costs, scratch
+  // per graph: layout, cost buffers, scratch; then the in-place
+  // quantization of every energy column and the fp32 slot costs of every
+  // graph in a few batched launches (each column as pgc_solve_ratio does it)
   std::vector<SolverJob> jobs(n);
+  std::vector<QItem> qitems;
+  std::vector<FItem> fitems;
   for (int64_t i = 0; i < n; ++i) {
     pgc_graph* g = graphs[i];
     const int K = max_depth[i];
     if ((rc = check_solvable(g, K))) return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
-    for (int c = 0; c < g->ncols; ++c)
-      HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
     if ((rc = build_layout(g))) return rc;
     if ((rc = check_width(g))) return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
-    if ((rc = fill_costs(g, 1))) return rc;
+    if ((rc = cost_buffers(g, 1))) return rc;
+    for (int c = 0; c < g->ncols; ++c) qitems.push_back({g->d_energy + (int64_t)c * g->E, g->E});
+    for (int c = 0; c < 2; ++c)
+      fitems.push_back({g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, (float*)g->d_cost[c],
+                        g->ncols > 1 ? g->d_time[c] : nullptr});
     if ((rc = ensure_scratch(g, K))) return rc;
     if ((rc = ensure_bnb(g))) return rc;
     HIPCHK(hipMemsetAsync(g->js.pred, 0xff, g->js.predN * 4, st));
     HIPCHK(hipMemsetAsync(g->js.neg_len, 0, 4, st));
     jobs[i] = make_job(g, K, MODE_FULL_RATIO, flags);
+  }
+  {
+    // chunks of at most 65535 columns (the launch grid's y limit)
+    const int64_t chunk = 65535;
+    if ((rc = grow(&ctx->d_qitems, ctx->qitems_cap, (int64_t)qitems.size()))) return rc;
+    if ((rc = grow(&ctx->d_fitems, ctx->fitems_cap, (int64_t)fitems.size()))) return rc;
+    if ((rc = grow(&ctx->d_select, ctx->select_cap,
+                   (int64_t)std::min<size_t>(qitems.size(), (size_t)chunk) * (int64_t)select_state_bytes())))
+      return rc;
+    HIPCHK(hipMemcpyAsync(ctx->d_qitems, qitems.data(), qitems.size() * sizeof(QItem), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->d_fitems, fitems.data(), fitems.size() * sizeof(FItem), hipMemcpyHostToDevice, st));
+    for (int64_t a = 0; a < (int64_t)qitems.size(); a += chunk)
+      HIPCHK(launch_quantize_batch(ctx->d_qitems + a, (int)std::min<int64_t>(chunk, (int64_t)qitems.size() - a),
+                                   ctx->d_select, st));
+    for (int64_t a = 0; a < (int64_t)fitems.size(); a += chunk)
+      HIPCHK(launch_fill_costs_batch(ctx->d_fitems + a, (int)std::min<int64_t>(chunk, (int64_t)fitems.size() - a),
+                                     st));
   }
   // batch tables: results and output cycles in one arena each
   if ((rc = grow(&ctx->d_jobs, ctx->jobs_cap, n))) return rc;

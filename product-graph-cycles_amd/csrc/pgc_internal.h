@@ -151,4 +151,21 @@ struct CycleItem {
   double gout;          // backward: upstream gradient
 };
 
+// One column of a batched quantization.
+struct QItem {
+  double* col;
+  int64_t n;
+};
+
+// One slot class of a batched fp32 cost fill.
+struct FItem {
+  const int32_t* eid;
+  const int32_t* s;
+  int64_t nslots;
+  const double* energy;
+  int64_t E;
+  float* cost;
+  float* time;  // nullptr without a time column
+};
+
 }  // namespace pgc
