@@ -38,6 +38,11 @@ namespace pgc {
 // barrier would then wait for all outstanding global loads. Hot paths
 // access them through explicit global-address-space pointers instead.
 #define PGC_GLOBAL __attribute__((address_space(1)))
+// The job table is read-only for a launch: read through the constant address
+// space with a wave-uniform index, its fields come in scalar loads and stay
+// in scalar registers (a job reference in the generic address space would
+// make every field a per-lane vector load).
+#define PGC_CONST __attribute__((address_space(4)))
 template <typename T>
 __device__ __forceinline__ PGC_GLOBAL T* gp(T* p) {
   return (PGC_GLOBAL T*)p;
@@ -159,24 +164,18 @@ __device__ __forceinline__ void block_min_key(Shared& sh, double& key, int& v) {
 // ------------------------------------------------------------ edge lookup
 // Edge joining consecutive cycle nodes a -> b (original node ids): the first
 // in-edge of b (remapped to the closing copy when it wraps) whose source is a.
-template <typename CT, bool HAS_TIME>
-__device__ __forceinline__ bool lookup_edge(const SolverJob& J, int a, int b, int& eid, double& c, double& t) {
+template <typename CT, bool HAS_TIME, typename JT>
+__device__ __forceinline__ bool lookup_edge(const JT& J, int a, int b, int& eid, double& c, double& t) {
   const int64_t npl = J.npl;
   int64_t bb = b;
   if (bb < npl && a > npl) bb += J.numNodes;
   if (a < 0 || bb < 0 || bb >= J.numNodes + npl) return false;
   const int lb = (int)(bb / npl);
   const int vi = (int)(bb - (int64_t)lb * npl);
-  const SlotClass* S;
-  int sl;
-  if ((int64_t)a / npl == lb) {
-    S = &J.intra;
-    sl = (int)(a - (int64_t)lb * npl);
-  } else {
-    S = &J.inter;
-    sl = (int)(a - (int64_t)(lb - 1) * npl);
-    if (lb == 0 || sl < 0 || sl >= npl) return false;
-  }
+  const bool same = (int64_t)a / npl == lb;
+  const auto* S = same ? &J.intra : &J.inter;
+  const int sl = same ? (int)(a - (int64_t)lb * npl) : (int)(a - (int64_t)(lb - 1) * npl);
+  if (!same && (lb == 0 || sl < 0 || sl >= npl)) return false;
   const int ch = vi >> 6, lane = vi & 63;
   const int width = S->w;
   const int base = lb * S->lstride + ch * width * 64;
@@ -201,7 +200,7 @@ __device__ __forceinline__ bool lookup_edge(const SolverJob& J, int a, int b, in
 template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
 struct Solver {
   static constexpr bool DYN = MAXC == 0;
-  const SolverJob& J;
+  const PGC_CONST SolverJob& J;
   double* D;     // [2K][ds] distance rows; entry npl of every row is +inf (padding slots point there)
   double* Dmin;  // [ds]
   Shared& sh;
@@ -212,7 +211,7 @@ struct Solver {
   BarrierOr bor;
   uint64_t tSweep = 0, tFold = 0, tChase = 0, tWalk = 0, tExtract = 0;
 
-  __device__ __forceinline__ Solver(const SolverJob& j, double* d, double* dm, Shared& s, int32_t* pl)
+  __device__ __forceinline__ Solver(const PGC_CONST SolverJob& j, double* d, double* dm, Shared& s, int32_t* pl)
       : J(j), D(d), Dmin(dm), sh(s) {
     Pl = (PGC_LDS int32_t*)pl;
     predLds = !DYN && J.pred_lds != 0;
@@ -266,10 +265,10 @@ struct Solver {
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     SweepState r;
     if constexpr (DYN)
-      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>(&J, D, Dmin, &sh, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
+      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
                                                         ratio, ub);
     else
-      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>(&J, D, Dmin, &sh, (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,
+      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,
                                                        bor.prev1, bor.odd ? 1 : 0, ratio, ub);
     bor.prev0 = r.p0;
     bor.prev1 = r.p1;
@@ -331,9 +330,8 @@ struct Solver {
   }
 
   // Chase every layer-0 vertex's predecessor chain back to the previous
-  // layer-0 vertex (one wrap), recording the original node ids. Entry j of
-  // vertex v's chain is at chain[j * npl + v] (lanes of a wave write
-  // consecutive words).
+  // layer-0 vertex (one wrap): the wrap map wnext/wlen. The chains' node ids
+  // are not stored; the extraction re-walks the few chains on its cycle.
   __device__ __forceinline__ void chase_wraps() {
     const int stride = J.chain_stride;
     if constexpr (DYN) {
@@ -350,7 +348,6 @@ struct Solver {
             break;
           }
           if (pp < npl) {
-            gp(J.chain)[(int64_t)pos * npl + v] = pp;
             pos++;
             gp(J.wnext)[v] = pp;
             gp(J.wlen)[v] = pos;
@@ -361,7 +358,6 @@ struct Solver {
             gp(J.wlen)[v] = pos;
             break;
           }
-          gp(J.chain)[(int64_t)pos * npl + v] = orig(pp);
           pos++;
           pp = pget(pp);
         }
@@ -391,7 +387,6 @@ struct Solver {
           const int v = vert(i);
           const int pp = p[i];
           if (pp < npl) {
-            gp(J.chain)[(int64_t)pos[i] * npl + v] = pp;
             pos[i]++;
             gp(J.wnext)[v] = pp;
             gp(J.wlen)[v] = pos[i];
@@ -404,7 +399,6 @@ struct Solver {
             act[i] = false;
             continue;
           }
-          gp(J.chain)[(int64_t)pos[i] * npl + v] = orig(pp);
           pos[i]++;
           const int nxt = pget(pp);
           p[i] = nxt;
@@ -636,16 +630,21 @@ struct Solver {
           err = ERR_CYCLE_TOO_LONG;
           return ST_NO_NEGATIVE_CYCLE;
         }
-        if (wave == 0) {
+        if (threadIdx.x == 0) {
+          // re-walk each wrap's predecessor chain (unchanged since the
+          // chase), writing original node ids back to front
           int x = vs, offp = 0, guard = 0;
           do {
             const int len = gp(J.wlen)[x];
-            for (int j = lane; j < len && offp + j < m; j += 64)
-              gp(J.tmp_nodes)[m - 1 - (offp + j)] = gp(J.chain)[(int64_t)j * npl + x];
+            int pp = pget(x);
+            for (int j = 0; j < len && offp + j < m; ++j) {
+              gp(J.tmp_nodes)[m - 1 - (offp + j)] = pp < npl ? pp : orig(pp);
+              if (j + 1 < len) pp = pget(pp);
+            }
             offp += len;
             x = gp(J.wnext)[x];
           } while (x != vs && x >= 0 && offp < m && ++guard < 4 * npl + 8);
-          if (lane == 0) gp(J.tmp_nodes)[m] = vs + (int)J.numNodes;
+          gp(J.tmp_nodes)[m] = vs + (int)J.numNodes;
         }
         __syncthreads();
         const int nn = m + 1;
@@ -784,7 +783,7 @@ struct Solver {
         int nextE = 0x7fffffff;
         double nextC = 0;
         for (int cls = 0; cls < 2; ++cls) {
-          const SlotClass& S = cls == 0 ? J.intra : J.inter;
+          const auto& S = cls == 0 ? J.intra : J.inter;
           const int ch = v >> 6, ln = v & 63;
           const int width = S.w;
           const int base = node_layer * S.lstride + ch * width * 64;
@@ -1094,9 +1093,9 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __res
       sh.i3 = queue ? atomicAdd(queue, 1) : next;
     }
     __syncthreads();
-    const int j = sh.i3;
+    const int j = __builtin_amdgcn_readfirstlane(sh.i3);
     if (j >= njobs) break;
-    const SolverJob& J = jobs[j];
+    const PGC_CONST SolverJob& J = ((const PGC_CONST SolverJob*)jobs)[j];
     double* D = MAXC == 0 ? J.dtab : lds;
     double* Dmin = D + 2 * J.K * (J.npl + 1);
     // Small graphs keep pred in LDS after the table. It is loaded from and

@@ -19,6 +19,35 @@ namespace pgc {
 
 #define PGC_LDS __attribute__((address_space(3)))
 
+// Wave-uniform copies: the value of the first active lane, in scalar
+// registers. The arguments of a device function arrive in vector registers,
+// and values loaded through them would otherwise count as varying per lane:
+// branches on them would run under exec masks and their address arithmetic on
+// the vector ALU, with spills of the vector registers this takes.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uni(int64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double uni(double x) { return __longlong_as_double(uni((int64_t)__double_as_longlong(x))); }
+template <typename T>
+__device__ __forceinline__ T* uni(T* p) {
+  return (T*)(intptr_t)uni((int64_t)(intptr_t)p);
+}
+
+// Slot class of the job with wave-uniform fields.
+__device__ __forceinline__ SlotClass uni_class(const PGC_GLOBAL SlotClass& c) {
+  SlotClass r;
+  r.src = uni(c.src);
+  r.cost = uni(c.cost);
+  r.time = uni(c.time);
+  r.eid = uni(c.eid);
+  r.w = uni(c.w);
+  r.lstride = uni(c.lstride);
+  return r;
+}
+
 struct SweepState {
   int complete;  // 0 when a layer 0<l<L had no improvement
   int p0, p1, odd;  // barrier-or state (see BarrierOr)
@@ -52,7 +81,7 @@ struct Sweep {
     const unsigned long long b = __ballot(pred_);
     if (b && lane == 0) __atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED);
     __syncthreads();
-    const int c = *cnt;
+    const int c = uni(*cnt);
     bool any;
     if (odd) {
       any = c != p1;
@@ -90,7 +119,9 @@ struct Sweep {
     return __dsub_rn((double)c, HAS_TIME ? __dmul_rn(ratio, (double)t) : rt);
   }
 
-  // Interlayer step of one owned vertex (depth 0 of layer `layer`).
+  // Interlayer step of one owned vertex (depth 0 of layer `layer`). K1: the
+  // job's K is 1 (no depth to recover, so the winner's cost is not tracked).
+  template <bool K1>
   __device__ __forceinline__ bool inter_vertex(int i, int v, int layer, int cur, int prev, const int (&s)[W],
                                                const CT (&c)[W], const float (&t)[W], double ratio, double rt,
                                                double ub, double (&dmin_r)[MAXC]) {
@@ -98,7 +129,7 @@ struct Sweep {
     int bs = npl;
     CT bc = 0;
     float bt = 0;
-    const PGC_LDS double* Dp = (K == 1) ? D + prev * ds : Dmin;
+    const PGC_LDS double* Dp = K1 ? D + prev * ds : Dmin;
     // all gathers first (independent LDS reads in flight together), then the
     // first-minimum scan in slot order
     double dv[W];
@@ -110,14 +141,16 @@ struct Sweep {
       const bool better = nv < best;
       best = better ? nv : best;
       bs = better ? s[j] : bs;
-      bc = better ? c[j] : bc;
-      if (HAS_TIME) bt = better ? t[j] : bt;
+      if (!K1) {
+        bc = better ? c[j] : bc;
+        if (HAS_TIME) bt = better ? t[j] : bt;
+      }
     }
     double res = PGC_INF;
     bool imp = false;
     if (best < PGC_INF && best < ub) {
       int ks = 0;
-      if (K > 1) {
+      if (!K1) {
         const double bw = weight(bc, bt, ratio, rt);
         for (int kk = 0; kk < K; ++kk) {
           if (D[(prev + kk) * ds + bs] + bw == best) {
@@ -200,7 +233,8 @@ struct Sweep {
   }
 
   // One relaxation step over the owned chunks with a loaded block.
-  template <bool INTER>
+  // INTER: 0 intralayer, 1 interlayer with K > 1, 2 interlayer with K == 1.
+  template <int INTER>
   __device__ __forceinline__ bool compute(const Slots& b, int layer, int cur, int prev, int k, double ratio,
                                           double rt, double ub, double (&dmin_r)[MAXC]) {
     bool imp = false;
@@ -209,7 +243,7 @@ struct Sweep {
       const int v = vert(i);
       if (v >= 0) {
         if (INTER)
-          imp |= inter_vertex(i, v, layer, cur, prev, b.s[i], b.c[i], b.t[i], ratio, rt, ub, dmin_r);
+          imp |= inter_vertex<INTER == 2>(i, v, layer, cur, prev, b.s[i], b.c[i], b.t[i], ratio, rt, ub, dmin_r);
         else
           imp |= intra_vertex(i, v, layer, cur, k, b.s[i], b.c[i], b.t[i], ratio, rt, ub, dmin_r);
       }
@@ -231,6 +265,12 @@ struct Sweep {
 
   // One pass over layers 0..L. Returns 0 when a layer 0<l<L had no improvement.
   //
+  // A layer without improvement has only +inf rows, and so has every layer
+  // after it: whether some layer 0<l<L had none is whether layer L-1 had
+  // none. Only that layer's steps end in a block-wide OR; the others end in a
+  // plain barrier (a pass that dies early runs on with +inf rows and writes
+  // no predecessor, and its caller reads no row of it).
+  //
   // The slots of a block do not depend on the distances, so when a wave owns
   // at most two chunks (two blocks fit in registers) the next block's loads
   // are issued before the current block is relaxed: a step then waits for
@@ -238,7 +278,7 @@ struct Sweep {
   // the K-1 intralayer steps of its layer.
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     constexpr bool PF = MAXC <= 2;
-    const double rt = __dmul_rn(ratio, J->time_const);
+    const double rt = __dmul_rn(ratio, uni(J->time_const));
     double dmin_r[MAXC];
     const int nb = K > 1 ? 2 * L + 1 : L;
     Slots A, B;
@@ -268,12 +308,17 @@ struct Sweep {
       } else {
         load_block(A, inter, layer);
       }
+      const bool check = layer == L - 1 && layer > 0;
       if (inter) {
-        imp = compute<true>(A, layer, cur, prev, 0, ratio, rt, ub, dmin_r);
         if (K == 1) {
-          const bool any = bor(imp);
-          if (!any && layer < L) return 0;
+          imp = compute<2>(A, layer, cur, prev, 0, ratio, rt, ub, dmin_r);
+          if (check) {
+            if (!bor(imp)) return 0;
+          } else {
+            __syncthreads();
+          }
         } else {
+          imp = compute<1>(A, layer, cur, prev, 0, ratio, rt, ub, dmin_r);
           __syncthreads();
         }
       } else {
@@ -286,10 +331,9 @@ struct Sweep {
           }
         }
         for (int k = 1; k < K; ++k) {
-          imp |= compute<false>(A, layer, cur, prev, k, ratio, rt, ub, dmin_r);
-          if (k == K - 1) {
-            const bool any = bor(imp);
-            if (layer > 0 && layer < L && !any) return 0;
+          imp |= compute<0>(A, layer, cur, prev, k, ratio, rt, ub, dmin_r);
+          if (k == K - 1 && check) {
+            if (!bor(imp)) return 0;
           } else {
             __syncthreads();
           }
@@ -307,38 +351,28 @@ __device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg,
                                                 int32_t* predLg, int predLds, int p0, int p1, int odd, double ratio,
                                                 double ub) {
   Sweep<CT, HAS_TIME, MINCOST, MAXC, W> w;
-  w.J = (const PGC_GLOBAL SolverJob*)Jp;
-  w.D = (PGC_LDS double*)Dg;
-  w.Dmin = (PGC_LDS double*)Dming;
-  w.sh = (PGC_LDS Shared*)shg;
-  w.pred = (PGC_GLOBAL int32_t*)w.J->pred;
-  w.predL = (PGC_LDS int32_t*)predLg;
-  w.predLds = predLds;
-  w.Sa.src = w.J->intra.src;
-  w.Sa.cost = w.J->intra.cost;
-  w.Sa.time = w.J->intra.time;
-  w.Sa.eid = w.J->intra.eid;
-  w.Sa.w = w.J->intra.w;
-  w.Sa.lstride = w.J->intra.lstride;
-  w.Sx.src = w.J->inter.src;
-  w.Sx.cost = w.J->inter.cost;
-  w.Sx.time = w.J->inter.time;
-  w.Sx.eid = w.J->inter.eid;
-  w.Sx.w = w.J->inter.w;
-  w.Sx.lstride = w.J->inter.lstride;
-  w.npl = w.J->npl;
+  w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
+  w.D = (PGC_LDS double*)uni(Dg);
+  w.Dmin = (PGC_LDS double*)uni(Dming);
+  w.sh = (PGC_LDS Shared*)uni(shg);
+  w.pred = (PGC_GLOBAL int32_t*)uni(w.J->pred);
+  w.predL = (PGC_LDS int32_t*)uni(predLg);
+  w.predLds = uni(predLds);
+  w.Sa = uni_class(w.J->intra);
+  w.Sx = uni_class(w.J->inter);
+  w.npl = uni(w.J->npl);
   w.ds = w.npl + 1;
-  w.K = w.J->K;
-  w.L = w.J->L;
-  w.nC = w.J->nChunks;
+  w.K = uni(w.J->K);
+  w.L = uni(w.J->L);
+  w.nC = uni(w.J->nChunks);
   w.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   w.lane = threadIdx.x & 63;
   w.nw = PGC_BLOCK / PGC_WAVE;
-  w.p0 = p0;
-  w.p1 = p1;
-  w.odd = odd;
+  w.p0 = uni(p0);
+  w.p1 = uni(p1);
+  w.odd = uni(odd);
   SweepState r;
-  r.complete = w.run_layers(ratio, ub);
+  r.complete = w.run_layers(uni(ratio), uni(ub));
   r.p0 = w.p0;
   r.p1 = w.p1;
   r.odd = w.odd;
@@ -367,7 +401,7 @@ struct SweepG {
     const unsigned long long b = __ballot(pred_);
     if (b && lane == 0) __atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED);
     __syncthreads();
-    const int c = *cnt;
+    const int c = uni(*cnt);
     bool any;
     if (odd) {
       any = c != p1;
@@ -487,25 +521,29 @@ struct SweepG {
   }
 
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
-    const double rt = __dmul_rn(ratio, J->time_const);
+    const double rt = __dmul_rn(ratio, uni(J->time_const));
     for (int layer = 0; layer <= L; ++layer) {
       const int cur = (layer & 1) * K;
       const int prev = K - cur;
       bool imp = false;
+      // only layer L-1 decides (see the LDS sweep)
+      const bool check = layer == L - 1 && layer > 0;
       if (layer > 0) {
         imp = inter_step(layer, cur, prev, ratio, rt, ub);
         if (K == 1) {
-          const bool any = bor(imp);
-          if (!any && layer < L) return 0;
+          if (check) {
+            if (!bor(imp)) return 0;
+          } else {
+            __syncthreads();
+          }
           continue;
         }
         __syncthreads();
       }
       for (int k = 1; k < K; ++k) {
         imp |= intra_step(layer, cur, k, ratio, rt, ub);
-        if (k == K - 1) {
-          const bool any = bor(imp);
-          if (layer > 0 && layer < L && !any) return 0;
+        if (k == K - 1 && check) {
+          if (!bor(imp)) return 0;
         } else {
           __syncthreads();
         }
@@ -519,36 +557,26 @@ template <typename CT, bool HAS_TIME, bool MINCOST, int W>
 __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                        int p0, int p1, int odd, double ratio, double ub) {
   SweepG<CT, HAS_TIME, MINCOST, W> w;
-  w.J = (const PGC_GLOBAL SolverJob*)Jp;
-  w.D = (PGC_GLOBAL double*)Dg;
-  w.Dmin = (PGC_GLOBAL double*)Dming;
-  w.sh = (PGC_LDS Shared*)shg;
-  w.pred = (PGC_GLOBAL int32_t*)w.J->pred;
-  w.Sa.src = w.J->intra.src;
-  w.Sa.cost = w.J->intra.cost;
-  w.Sa.time = w.J->intra.time;
-  w.Sa.eid = w.J->intra.eid;
-  w.Sa.w = w.J->intra.w;
-  w.Sa.lstride = w.J->intra.lstride;
-  w.Sx.src = w.J->inter.src;
-  w.Sx.cost = w.J->inter.cost;
-  w.Sx.time = w.J->inter.time;
-  w.Sx.eid = w.J->inter.eid;
-  w.Sx.w = w.J->inter.w;
-  w.Sx.lstride = w.J->inter.lstride;
-  w.npl = w.J->npl;
+  w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
+  w.D = (PGC_GLOBAL double*)uni(Dg);
+  w.Dmin = (PGC_GLOBAL double*)uni(Dming);
+  w.sh = (PGC_LDS Shared*)uni(shg);
+  w.pred = (PGC_GLOBAL int32_t*)uni(w.J->pred);
+  w.Sa = uni_class(w.J->intra);
+  w.Sx = uni_class(w.J->inter);
+  w.npl = uni(w.J->npl);
   w.ds = w.npl + 1;
-  w.K = w.J->K;
-  w.L = w.J->L;
-  w.nC = w.J->nChunks;
+  w.K = uni(w.J->K);
+  w.L = uni(w.J->L);
+  w.nC = uni(w.J->nChunks);
   w.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   w.lane = threadIdx.x & 63;
   w.nw = PGC_BLOCK / PGC_WAVE;
-  w.p0 = p0;
-  w.p1 = p1;
-  w.odd = odd;
+  w.p0 = uni(p0);
+  w.p1 = uni(p1);
+  w.odd = uni(odd);
   SweepState r;
-  r.complete = w.run_layers(ratio, ub);
+  r.complete = w.run_layers(uni(ratio), uni(ub));
   r.p0 = w.p0;
   r.p1 = w.p1;
   r.odd = w.odd;
