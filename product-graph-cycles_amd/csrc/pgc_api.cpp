@@ -402,11 +402,20 @@ bool pred_fits_lds(const pgc_graph* g, int K) {
   return solver_lds_bytes((int)g->npl, K) + pred <= 160 * 1024 - 1024;
 }
 
+// The wrap map (two ints per vertex) follows in LDS when it also fits.
+bool wrap_fits_lds(const pgc_graph* g, int K) {
+  if (kernel_maxc(g, K) == 0) return false;
+  size_t b = solver_lds_bytes((int)g->npl, K);
+  if (pred_fits_lds(g, K)) b += (size_t)(g->L + 1) * K * g->npl * 4;
+  return b + (size_t)g->npl * 8 <= 160 * 1024 - 1024;
+}
+
 // dynamic LDS of a job's launch
 size_t job_lds_bytes(const pgc_graph* g, int K) {
   if (kernel_maxc(g, K) == 0) return 0;
   size_t b = solver_lds_bytes((int)g->npl, K);
   if (pred_fits_lds(g, K)) b += (size_t)(g->L + 1) * K * g->npl * 4;
+  if (wrap_fits_lds(g, K)) b += (size_t)g->npl * 8;
   return b;
 }
 
@@ -523,6 +532,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.vslot = s.vslot;
   j.dtab = s.dtab;
   j.pred_lds = pred_fits_lds(g, K) ? 1 : 0;
+  j.wrap_lds = wrap_fits_lds(g, K) ? 1 : 0;
   j.vc_eid = s.vc_eid;
   j.vc_c = s.vc_c;
   j.vc_t = s.vc_t;

@@ -135,6 +135,7 @@ struct SolverJob {
   int32_t* out_eid;       // [out_cap] output cycle
   int32_t vc_cap, vc_stride, out_cap;
   int32_t pred_lds;       // 1: pred lives in LDS after the distance table (small graphs)
+  int32_t wrap_lds;       // 1: the wrap map (wnext, wlen) lives in LDS after the table and pred
   double* dtab;           // [(2K+1)*(npl+1)] distance table in global memory (graphs beyond the LDS)
 };
 

@@ -105,6 +105,7 @@ __device__ __forceinline__ uint64_t wclock() { return wall_clock64(); }
 }  // namespace pgc
 
 #include "sweep_device.h"
+#include "wrap_device.h"
 
 namespace pgc {
 
@@ -206,6 +207,9 @@ struct Solver {
   Shared& sh;
   PGC_LDS int32_t* Pl;  // pred in LDS (J.pred_lds), else J.pred in global memory
   bool predLds;
+  PGC_LDS int32_t* WnL;  // wrap map in LDS after pred (J.wrap_lds), else J.wnext/J.wlen
+  PGC_LDS int32_t* WlL;
+  bool wrapLds;
   FastDiv fdNpl, fdPer;  // division by npl and by npl*K
   int npl, ds, K, L, nC, wave, lane, nw, npass;
   BarrierOr bor;
@@ -215,6 +219,9 @@ struct Solver {
       : J(j), D(d), Dmin(dm), sh(s) {
     Pl = (PGC_LDS int32_t*)pl;
     predLds = !DYN && J.pred_lds != 0;
+    wrapLds = !DYN && J.wrap_lds != 0;
+    WnL = Pl + (predLds ? (int64_t)(J.L + 1) * J.K * J.npl : 0);
+    WlL = WnL + J.npl;
     npl = J.npl;
     ds = npl + 1;
     K = J.K;
@@ -264,9 +271,18 @@ struct Solver {
   // gets the whole register file; the caller's state is saved once per sweep).
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     SweepState r;
+    // The unrolled sweep (slots of the next block prefetched into registers)
+    // pays off with one chunk per wave and 8 slots per vertex; with more, its
+    // register sets spill, and the sweep that takes one chunk at a time (rows
+    // in LDS) is faster.
+    constexpr bool UNROLLED = MAXC == 1 && W == 8;
     if constexpr (DYN)
-      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
-                                                        ratio, ub);
+      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, nullptr, 0, bor.prev0,
+                                                        bor.prev1, bor.odd ? 1 : 0, ratio, ub);
+    else if constexpr (!UNROLLED)
+      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+                                                              predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
+                                                              ratio, ub);
     else
       r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,
                                                        bor.prev1, bor.odd ? 1 : 0, ratio, ub);
@@ -363,62 +379,20 @@ struct Solver {
         }
       }
     } else {
-      constexpr int NR = DYN ? 1 : MAXC;
-      int p[NR], pos[NR];
-      bool act[NR];
-#pragma unroll
-      for (int i = 0; i < np(); ++i) {
-        const int v = vert(i);
-        act[i] = v >= 0;
-        pos[i] = 0;
-        p[i] = act[i] ? pget(v) : -1;
-        if (act[i] && p[i] == -1) {
-          gp(J.wnext)[v] = -1;
-          gp(J.wlen)[v] = 0;
-          act[i] = false;
-        }
-      }
-      bool any = true;
-      while (any) {
-        any = false;
-#pragma unroll
-        for (int i = 0; i < np(); ++i) {
-          if (!act[i]) continue;
-          const int v = vert(i);
-          const int pp = p[i];
-          if (pp < npl) {
-            pos[i]++;
-            gp(J.wnext)[v] = pp;
-            gp(J.wlen)[v] = pos[i];
-            act[i] = false;
-            continue;
-          }
-          if (pos[i] >= stride - 1) {
-            gp(J.wnext)[v] = -2;
-            gp(J.wlen)[v] = pos[i];
-            act[i] = false;
-            continue;
-          }
-          pos[i]++;
-          const int nxt = pget(pp);
-          p[i] = nxt;
-          if (nxt == -1) {
-            gp(J.wnext)[v] = -1;
-            gp(J.wlen)[v] = pos[i];
-            act[i] = false;
-            continue;
-          }
-          any = true;
-        }
-      }
+      chase_wraps_fn<DYN ? 1 : MAXC>(predLds ? (int32_t*)Pl : J.pred, predLds ? 1 : 0,
+                                     wrapLds ? (int32_t*)WnL : J.wnext, wrapLds ? (int32_t*)WlL : J.wlen,
+                                     wrapLds ? 1 : 0, npl, nC, stride);
     }
   }
 
+  // wrap map entries (either placement; for the rare sequential paths)
+  __device__ __forceinline__ int wn_get(int v) const { return wrapLds ? WnL[v] : gp(J.wnext)[v]; }
+  __device__ __forceinline__ int wl_get(int v) const { return wrapLds ? WlL[v] : gp(J.wlen)[v]; }
+
   // Walk of the reference (cost_time_ratio_solver.cpp:443-489) expressed on
   // the wrap map W: first repeated layer-0 vertex from v and its step count.
-  __device__ __forceinline__ bool walk_from(int v, int64_t capSteps, int& vstar) const {
-    const PGC_GLOBAL int* Wn = gp(J.wnext);
-    const PGC_GLOBAL int* Ln = gp(J.wlen);
+  template <typename WT>
+  __device__ __forceinline__ bool walk_in(WT Wn, WT Ln, int v, int64_t capSteps, int& vstar) const {
     const int lim = 4 * npl + 8;
     int t = Wn[v];
     if (t < 0) return false;
@@ -456,6 +430,35 @@ struct Solver {
     if (P > capSteps) return false;
     vstar = e;
     return true;
+  }
+  __device__ __forceinline__ bool walk_from(int v, int64_t capSteps, int& vstar) const {
+    if (wrapLds) return walk_in((const PGC_LDS int32_t*)WnL, (const PGC_LDS int32_t*)WlL, v, capSteps, vstar);
+    return walk_in(gp((const int32_t*)J.wnext), gp((const int32_t*)J.wlen), v, capSteps, vstar);
+  }
+
+  // Best candidate of the negative-cycle search: branch vertices with a
+  // negative (or, searching zero cycles, |D| < tol) layer-0 value whose walk
+  // closes within the step cap; key -v (pick last) or D.
+  template <typename WT>
+  __device__ __forceinline__ void candidates(WT Wn, WT Ln, int bid, int onlyV, bool zeroSearch, double& bestKey,
+                                             int& bestV) const {
+    const bool pickLast = (J.flags & FLAG_PICK_LAST_VERTEX) != 0;
+    const int64_t capSteps = J.numNodes - 1;
+#pragma unroll
+    for (int i = 0; i < np(); ++i) {
+      const int v = vert(i);
+      if (v < 0 || !in_branch(v, bid, onlyV)) continue;
+      const double lb = D[v];
+      if (!zeroSearch && lb >= 0) continue;
+      if (zeroSearch && (lb <= -J.tol || lb >= J.tol)) continue;
+      int vs;
+      if (!walk_in(Wn, Ln, v, capSteps, vs)) continue;
+      const double key = pickLast ? -(double)v : lb;
+      if (key < bestKey || (key == bestKey && v < bestV)) {
+        bestKey = key;
+        bestV = v;
+      }
+    }
   }
 
   // sequential sums over a cycle's pairs (thread 0 only)
@@ -585,26 +588,17 @@ struct Solver {
       t0 = wclock();
       tChase += t0 - t1;
 
-      // candidates: branch vertices with a negative (or, searching zero
-      // cycles, |D| < tol) layer-0 value
-      const bool pickLast = (J.flags & FLAG_PICK_LAST_VERTEX) != 0;
       const int64_t capSteps = J.numNodes - 1;
       double bestKey = PGC_INF;
       int bestV = 0x7fffffff;
-#pragma unroll
-      for (int i = 0; i < np(); ++i) {
-        const int v = vert(i);
-        if (v < 0 || !in_branch(v, bid, onlyV)) continue;
-        const double lb = D[v];
-        if (!zeroSearch && lb >= 0) continue;
-        if (zeroSearch && (lb <= -J.tol || lb >= J.tol)) continue;
-        int vs;
-        if (!walk_from(v, capSteps, vs)) continue;
-        const double key = pickLast ? -(double)v : lb;
-        if (key < bestKey || (key == bestKey && v < bestV)) {
-          bestKey = key;
-          bestV = v;
-        }
+      if constexpr (DYN) {
+        candidates(gp((const int32_t*)J.wnext), gp((const int32_t*)J.wlen), bid, onlyV, zeroSearch, bestKey, bestV);
+      } else {
+        const Candidate c = candidates_fn<DYN ? 1 : MAXC>(
+            D, J.branch, bid, onlyV, zeroSearch ? 1 : 0, J.tol, (J.flags & FLAG_PICK_LAST_VERTEX) != 0 ? 1 : 0,
+            capSteps, wrapLds ? (int32_t*)WnL : J.wnext, wrapLds ? (int32_t*)WlL : J.wlen, wrapLds ? 1 : 0, npl, nC);
+        bestKey = c.key;
+        bestV = c.v;
       }
       block_min_key(sh, bestKey, bestV);
       t1 = wclock();
@@ -617,8 +611,8 @@ struct Solver {
           int64_t m = 0;
           int x = vs, it = 0;
           do {
-            m += gp(J.wlen)[x];
-            x = gp(J.wnext)[x];
+            m += wl_get(x);
+            x = wn_get(x);
           } while (x != vs && ++it < 4 * npl + 8);
           sh.i0 = vs;
           sh.i1 = (m + 1 <= J.cyc_cap) ? (int)m : -1;
@@ -635,14 +629,14 @@ struct Solver {
           // chase), writing original node ids back to front
           int x = vs, offp = 0, guard = 0;
           do {
-            const int len = gp(J.wlen)[x];
+            const int len = wl_get(x);
             int pp = pget(x);
             for (int j = 0; j < len && offp + j < m; ++j) {
               gp(J.tmp_nodes)[m - 1 - (offp + j)] = pp < npl ? pp : orig(pp);
               if (j + 1 < len) pp = pget(pp);
             }
             offp += len;
-            x = gp(J.wnext)[x];
+            x = wn_get(x);
           } while (x != vs && x >= 0 && offp < m && ++guard < 4 * npl + 8);
           gp(J.tmp_nodes)[m] = vs + (int)J.numNodes;
         }

@@ -263,6 +263,38 @@ struct Sweep {
     }
   }
 
+  // The pass with K == 1 (interlayer blocks only) and prefetch: the two
+  // register sets alternate roles (the loop is unrolled by two), so no block
+  // is copied between them, and a block's wait covers only its own loads.
+  // The prefetch is unconditional (past the last layer it reloads layer L,
+  // in bounds, unused): with a branch around it the wait analysis would
+  // merge the two paths and wait for the prefetch as well.
+  __device__ __forceinline__ int run_layers_k1(Slots& A, Slots& B, double ratio, double rt, double ub,
+                                               double (&dmin_r)[MAXC]) {
+    const int nb = L;
+    load_block(A, true, 1);
+    for (int b = 0; b < nb; b += 2) {
+      int layer = b + 1;
+      load_block(B, true, min(layer + 1, L));
+      bool imp = compute<2>(A, layer, (layer & 1), 1 - (layer & 1), 0, ratio, rt, ub, dmin_r);
+      if (layer == L - 1) {
+        if (!bor(imp)) return 0;
+      } else {
+        __syncthreads();
+      }
+      if (b + 1 >= nb) break;
+      layer = b + 2;
+      load_block(A, true, min(layer + 1, L));
+      imp = compute<2>(B, layer, (layer & 1), 1 - (layer & 1), 0, ratio, rt, ub, dmin_r);
+      if (layer == L - 1) {
+        if (!bor(imp)) return 0;
+      } else {
+        __syncthreads();
+      }
+    }
+    return 1;
+  }
+
   // One pass over layers 0..L. Returns 0 when a layer 0<l<L had no improvement.
   //
   // A layer without improvement has only +inf rows, and so has every layer
@@ -282,6 +314,7 @@ struct Sweep {
     double dmin_r[MAXC];
     const int nb = K > 1 ? 2 * L + 1 : L;
     Slots A, B;
+    if (PF && K == 1 && nb > 0) return run_layers_k1(A, B, ratio, rt, ub, dmin_r);
     bool inter;
     int layer;
     if (PF && nb > 0) {
@@ -380,18 +413,34 @@ __device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg,
 }
 
 // ------------------------------------------------------------------------
-// The same sweep for graphs whose distance table does not fit the LDS: the
-// rows live in global memory (L2-resident), every wave loops over its chunks
-// at run time, and the per-vertex minimum over the depths of a layer is
-// recomputed from the rows at its last intralayer step (the minimum is exact,
-// so this equals the running minimum of the LDS sweep).
-template <typename CT, bool HAS_TIME, bool MINCOST, int W>
+// The same sweep with one chunk at a time: every wave loops over its chunks at
+// run time, so only one chunk's slots are live, and the per-vertex minimum
+// over the depths of a layer is recomputed from the rows at its last
+// intralayer step (the minimum is exact, so this equals the running minimum
+// of the unrolled sweep). The rows live in LDS for graphs with several chunks
+// per wave or wide slots, and in global memory (L2-resident) for graphs whose
+// distance table does not fit the LDS.
+template <bool LD>
+struct DistPtr {
+  typedef PGC_GLOBAL double* type;
+};
+template <>
+struct DistPtr<true> {
+  typedef PGC_LDS double* type;
+};
+
+// LD: the rows live in LDS (graphs with many chunks per wave, for which the
+// unrolled sweep would hold too many registers), else in global memory.
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false>
 struct SweepG {
+  typedef typename DistPtr<LD>::type DP;
   const PGC_GLOBAL SolverJob* J;
-  PGC_GLOBAL double* D;
-  PGC_GLOBAL double* Dmin;
+  DP D;
+  DP Dmin;
   PGC_LDS Shared* sh;
   PGC_GLOBAL int32_t* pred;
+  PGC_LDS int32_t* predL;  // pred in LDS (small graphs, LD only), selected by predLds
+  int predLds;
   SlotClass Sa, Sx;
   int npl, ds, K, L, nC, wave, lane, nw;
   int p0, p1, odd;
@@ -433,7 +482,7 @@ struct SweepG {
 
   __device__ __forceinline__ bool inter_step(int layer, int cur, int prev, double ratio, double rt, double ub) {
     bool imp = false;
-    const PGC_GLOBAL double* Dp = (K == 1) ? D + prev * ds : Dmin;
+    const DP Dp = (K == 1) ? D + prev * ds : Dmin;
     for (int c = wave; c < nC; c += nw) {
       int s[W];
       CT cc[W];
@@ -469,7 +518,12 @@ struct SweepG {
             }
           }
         }
-        pred[(int64_t)layer * npl * K + v] = bs + (layer - 1) * npl * K + ks * npl;
+        const int64_t pi = (int64_t)layer * npl * K + v;
+        const int pv = bs + (layer - 1) * npl * K + ks * npl;
+        if (LD && predLds)
+          predL[pi] = pv;
+        else
+          pred[pi] = pv;
         res = best;
         imp = true;
       }
@@ -480,7 +534,7 @@ struct SweepG {
 
   __device__ __forceinline__ bool intra_step(int layer, int cur, int k, double ratio, double rt, double ub) {
     bool imp = false;
-    const PGC_GLOBAL double* Dk = D + (cur + k - 1) * ds;
+    const DP Dk = D + (cur + k - 1) * ds;
     for (int c = wave; c < nC; c += nw) {
       int s[W];
       CT cc[W];
@@ -503,7 +557,12 @@ struct SweepG {
       const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
       double res = curv;
       if (best < curv && best < ub) {
-        pred[(int64_t)layer * npl * K + (int64_t)k * npl + v] = bs + layer * npl * K + (k - 1) * npl;
+        const int64_t pi = (int64_t)layer * npl * K + (int64_t)k * npl + v;
+        const int pv = bs + layer * npl * K + (k - 1) * npl;
+        if (LD && predLds)
+          predL[pi] = pv;
+        else
+          pred[pi] = pv;
         res = best;
         imp = true;
       }
@@ -553,15 +612,18 @@ struct SweepG {
   }
 };
 
-template <typename CT, bool HAS_TIME, bool MINCOST, int W>
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false>
 __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
-                                                       int p0, int p1, int odd, double ratio, double ub) {
-  SweepG<CT, HAS_TIME, MINCOST, W> w;
+                                                       int32_t* predLg, int predLds, int p0, int p1, int odd,
+                                                       double ratio, double ub) {
+  SweepG<CT, HAS_TIME, MINCOST, W, LD> w;
   w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
-  w.D = (PGC_GLOBAL double*)uni(Dg);
-  w.Dmin = (PGC_GLOBAL double*)uni(Dming);
+  w.D = (typename SweepG<CT, HAS_TIME, MINCOST, W, LD>::DP)uni(Dg);
+  w.Dmin = (typename SweepG<CT, HAS_TIME, MINCOST, W, LD>::DP)uni(Dming);
   w.sh = (PGC_LDS Shared*)uni(shg);
   w.pred = (PGC_GLOBAL int32_t*)uni(w.J->pred);
+  w.predL = (PGC_LDS int32_t*)uni(predLg);
+  w.predLds = LD ? uni(predLds) : 0;
   w.Sa = uni_class(w.J->intra);
   w.Sx = uni_class(w.J->inter);
   w.npl = uni(w.J->npl);
