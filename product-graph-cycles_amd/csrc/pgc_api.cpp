@@ -18,6 +18,8 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <unordered_map>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -79,11 +81,99 @@ static int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(PGC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// ------------------------------------------------ device memory cache
+// Graphs, their solver scratch and temporaries take device memory from a
+// per-device cache of freed blocks, keyed by rounded size: hipMalloc and
+// hipFree cost tens to hundreds of microseconds each (hipFree also waits for
+// the device), and a solve on a new graph makes about forty allocations. A
+// block returns to the cache only when no queued work can still use it: every
+// caller of cache_free has synchronized the stream (or device) that used it.
+// Cached blocks are released when an allocation runs out of memory. The cache
+// lives until the process ends (no HIP calls from static destructors).
+namespace {
+struct MemCache {
+  std::mutex m;
+  std::map<std::pair<int, size_t>, std::vector<void*>> free;
+  std::unordered_map<void*, std::pair<int, size_t>> owned;
+};
+MemCache& memcache() {
+  static MemCache* c = new MemCache();
+  return *c;
+}
+size_t round_bytes(size_t b) {
+  if (b <= 65536) return (b + 511) & ~(size_t)511;
+  size_t p = 65536;
+  while (p < b) p <<= 1;
+  const size_t q = p >> 4;  // at most 1/16 rounding waste
+  return (b + q - 1) / q * q;
+}
+void cache_trim(int dev) {
+  MemCache& c = memcache();
+  std::lock_guard<std::mutex> lk(c.m);
+  for (auto it = c.free.begin(); it != c.free.end();) {
+    if (it->first.first != dev) {
+      ++it;
+      continue;
+    }
+    for (void* q : it->second) {
+      c.owned.erase(q);
+      (void)hipFree(q);
+    }
+    it = c.free.erase(it);
+  }
+}
+}  // namespace
+
+static hipError_t cache_malloc(void** p, size_t bytes) {
+  *p = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const size_t r = round_bytes(bytes ? bytes : 1);
+  MemCache& c = memcache();
+  {
+    std::lock_guard<std::mutex> lk(c.m);
+    auto it = c.free.find({dev, r});
+    if (it != c.free.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      return hipSuccess;
+    }
+  }
+  e = hipMalloc(p, r);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    cache_trim(dev);
+    e = hipMalloc(p, r);
+  }
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return e;
+  }
+  std::lock_guard<std::mutex> lk(c.m);
+  c.owned[*p] = {dev, r};
+  return hipSuccess;
+}
+
+// Returns a block to the cache (blocks not from the cache are freed).
+static void cache_free(void* p) {
+  if (!p) return;
+  MemCache& c = memcache();
+  {
+    std::lock_guard<std::mutex> lk(c.m);
+    auto it = c.owned.find(p);
+    if (it != c.owned.end()) {
+      c.free[it->second].push_back(p);
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
 template <typename T>
 static hipError_t dalloc(T** p, size_t n) {
-  *p = nullptr;
   if (n == 0) n = 1;
-  return hipMalloc((void**)p, n * sizeof(T));
+  return cache_malloc((void**)p, n * sizeof(T));
 }
 
 struct pgc_context {
@@ -140,11 +230,12 @@ struct JobScratch {
   double *vc_c = nullptr, *vc_t = nullptr;
   int vc_cap = 0, vc_stride = 0;
   void release() {
+    if (pred) (void)hipDeviceSynchronize();  // queued solver work may still use the scratch
     void* ps[] = {pred, wnext, wlen, chain, tmp_nodes, tmp_eid, neg_nodes, neg_eid, neg_len, tmp_c, tmp_t,
                   neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, dtab, res, djob,
                   hkey, hval, hwhere, vslot, vc_eid, vc_len, vc_c, vc_t};
     for (void* p : ps)
-      if (p) (void)hipFree(p);
+      if (p) cache_free(p);
     *this = JobScratch();
   }
 };
@@ -191,6 +282,7 @@ struct pgc_graph {
   size_t cost_bytes[2] = {0, 0};
   size_t time_bytes[2] = {0, 0};
   ~pgc_graph() {
+    (void)hipDeviceSynchronize();  // queued work may still use the graph's buffers
     js.release();
     if (base) {
       d_rows = d_src = d_trgt = nullptr;
@@ -198,12 +290,12 @@ struct pgc_graph {
       d_s[0] = d_s[1] = d_e[0] = d_e[1] = nullptr;
       d_vx = d_vy = nullptr;
     }
-    if (d_vx) (void)hipFree(d_vx);
-    if (d_vy) (void)hipFree(d_vy);
+    if (d_vx) cache_free(d_vx);
+    if (d_vy) cache_free(d_vy);
     void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_width, d_s[0], d_s[1], d_e[0], d_e[1],
                   d_cnt, d_cost[0], d_cost[1], d_time[0], d_time[1]};
     for (void* p : ps)
-      if (p) (void)hipFree(p);
+      if (p) cache_free(p);
   }
 };
 
@@ -237,13 +329,14 @@ int pgc_context_create(int device, pgc_context** out) {
 void pgc_context_destroy(pgc_context* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  if (c->select_state) (void)hipFree(c->select_state);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->select_state) cache_free(c->select_state);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values,
                 c->d_qitems, c->d_fitems, c->d_select};
   for (void* p : ps)
-    if (p) (void)hipFree(p);
+    if (p) cache_free(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -262,7 +355,7 @@ int finish_graph_meta(pgc_graph* g) {
   int32_t mx = -1;
   HIPCHK(hipMemcpyAsync(&mx, d_max, 4, hipMemcpyDeviceToHost, g->ctx->stream));
   HIPCHK(hipStreamSynchronize(g->ctx->stream));
-  (void)hipFree(d_max);
+  cache_free(d_max);
   g->numNodes = (int64_t)mx + 1;
   g->L = g->numNodes / g->npl;
   if (g->L < 1) return fail(PGC_ERR_ARG, "product graph has fewer nodes than one layer");
@@ -327,8 +420,8 @@ int build_layout(pgc_graph* g) {
                             g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
   for (int c = 0; c < 2; ++c) HIPCHK(launch_pad_src(g->d_s[c], g->nslots[c] + kSlotPad, (int32_t)npl, st));
   HIPCHK(hipStreamSynchronize(st));
-  (void)hipFree(d_dropped);
-  (void)hipFree(d_fill);
+  cache_free(d_dropped);
+  cache_free(d_fill);
   g->layout = true;
   return PGC_OK;
 }
@@ -345,15 +438,21 @@ int cost_buffers(pgc_graph* g, int kind) {
     const size_t tb = withTime ? (size_t)(n + kSlotPad) * 4 : 0;
     // buffers are reused while their size fits; every slot is rewritten below
     if (g->cost_bytes[c] != cb) {
-      if (g->d_cost[c]) (void)hipFree(g->d_cost[c]);
+      if (g->d_cost[c]) {
+        HIPCHK(hipStreamSynchronize(st));
+        cache_free(g->d_cost[c]);
+      }
       g->d_cost[c] = nullptr;
       g->cost_bytes[c] = 0;
-      HIPCHK(hipMalloc(&g->d_cost[c], cb));
+      HIPCHK(cache_malloc(&g->d_cost[c], cb));
       HIPCHK(hipMemsetAsync(g->d_cost[c], 0, cb, st));
       g->cost_bytes[c] = cb;
     }
     if (g->time_bytes[c] != tb) {
-      if (g->d_time[c]) (void)hipFree(g->d_time[c]);
+      if (g->d_time[c]) {
+        HIPCHK(hipStreamSynchronize(st));
+        cache_free(g->d_time[c]);
+      }
       g->d_time[c] = nullptr;
       g->time_bytes[c] = 0;
       if (tb) {
@@ -753,7 +852,7 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
                             rl_alpha, rl_c, rl_pwr, (double)maxSO3Val, (double)maxFeatDiff, g->d_energy, st));
   HIPCHK(hipStreamSynchronize(st));
   for (void* p : {(void*)dOff, (void*)dList, (void*)dPre, (void*)dQuat, (void*)dMax})
-    if (p) (void)hipFree(p);
+    if (p) cache_free(p);
   // numNodes = max(SRC) + 1, numLayers = numNodes / npl as run() derives them
   int rc = finish_graph_meta(g.get());
   if (rc && rc != PGC_ERR_UNSUPPORTED) return rc;
@@ -894,13 +993,13 @@ int pgc_graph_generate(pgc_context* ctx, const double* VX, int64_t nVX, const in
   const int rc = generate_impl(ctx, dVX, nVX, dEX, nEX, dVY, nVY, dEY, nEY, dFD, conjugate, regularising, prune,
                                cost_mode, time_mode, dNX, dNY, rl_alpha, rl_c, rl_pwr, EX, out);
   HIPCHK(hipStreamSynchronize(st));
-  if (dNX) (void)hipFree(dNX);
-  if (dNY) (void)hipFree(dNY);
-  (void)hipFree(dEX);
-  (void)hipFree(dEY);
-  (void)hipFree(dVX);
-  (void)hipFree(dVY);
-  (void)hipFree(dFD);
+  if (dNX) cache_free(dNX);
+  if (dNY) cache_free(dNY);
+  cache_free(dEX);
+  cache_free(dEY);
+  cache_free(dVX);
+  cache_free(dVY);
+  cache_free(dFD);
   return rc;
 }
 
@@ -993,7 +1092,7 @@ static int energy_from(pgc_graph* g, const double* feat_diff, int on_device) {
   g->cost_kind = 0;
   if (tmp) {
     HIPCHK(hipStreamSynchronize(st));
-    (void)hipFree(tmp);
+    cache_free(tmp);
   }
   return PGC_OK;
 }
@@ -1203,7 +1302,7 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
     HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, s.eids_io, n, d_out, st));
     HIPCHK(hipMemcpyAsync(rows, d_out, n * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    (void)hipFree(d_out);
+    cache_free(d_out);
   }
   res->passes_layers_once = 1;
   res->objective = (float)(cs / ts);
@@ -1224,7 +1323,10 @@ extern "C++" {
 template <typename T>
 static int grow(T** p, int64_t& cap, int64_t need) {
   if (need <= cap && *p) return PGC_OK;
-  if (*p) (void)hipFree(*p);
+  if (*p) {
+    (void)hipDeviceSynchronize();  // queued work may still use the arena
+    cache_free(*p);
+  }
   *p = nullptr;
   cap = 0;
   const int64_t n = std::max<int64_t>(need, 64) * 3 / 2;
@@ -1549,8 +1651,8 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
     HIPCHK(launch_gather_rows(g->d_src, 1, d_ids, (int64_t)ids.size(), d_out, st));
     HIPCHK(hipMemcpyAsync(bgSrc.data(), d_out, ids.size() * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    (void)hipFree(d_ids);
-    (void)hipFree(d_out);
+    cache_free(d_ids);
+    cache_free(d_out);
     for (size_t i = 0; i < ids.size(); ++i)
       if (!(g->bg_val[i] >= 0 && g->bg_val[i] < g->E)) bgSrc[i] = -1;
   }
@@ -1744,7 +1846,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
     HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, s.eids_io, n, d_out, st));
     HIPCHK(hipMemcpyAsync(rows, d_out, n * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    (void)hipFree(d_out);
+    cache_free(d_out);
   }
   res->passes_layers_once = loops < 2 ? 1 : 0;
   res->objective = (float)upperBound;

@@ -48,6 +48,13 @@ __device__ __forceinline__ SlotClass uni_class(const PGC_GLOBAL SlotClass& c) {
   return r;
 }
 
+// Running minimum of the first-minimum scans: one v_min_f64 instead of a
+// compare-and-select pair. The slot choice still uses the strict nv < best of
+// the scan (ties keep the earlier slot). The values agree with the select
+// except possibly in the sign of a zero minimum, which no comparison, sum or
+// output distinguishes (distances are never NaN).
+__device__ __forceinline__ double pgc_min(double a, double b) { return __builtin_fmin(a, b); }
+
 struct SweepState {
   int complete;  // 0 when a layer 0<l<L had no improvement
   int p0, p1, odd;  // barrier-or state (see BarrierOr)
@@ -139,7 +146,7 @@ struct Sweep {
     for (int j = 0; j < W; ++j) {
       const double nv = dv[j] + weight(c[j], t[j], ratio, rt);
       const bool better = nv < best;
-      best = better ? nv : best;
+      best = pgc_min(best, nv);
       bs = better ? s[j] : bs;
       if (!K1) {
         bc = better ? c[j] : bc;
@@ -187,7 +194,7 @@ struct Sweep {
     for (int j = 0; j < W; ++j) {
       const double nv = dv[j] + weight(c[j], t[j], ratio, rt);
       const bool better = nv < best;
-      best = better ? nv : best;
+      best = pgc_min(best, nv);
       bs = better ? s[j] : bs;
     }
     const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
@@ -501,7 +508,7 @@ struct SweepG {
       for (int j = 0; j < W; ++j) {
         const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
         const bool better = nv < best;
-        best = better ? nv : best;
+        best = pgc_min(best, nv);
         bs = better ? s[j] : bs;
         bc = better ? cc[j] : bc;
         if (HAS_TIME) bt = better ? t[j] : bt;
@@ -551,7 +558,7 @@ struct SweepG {
       for (int j = 0; j < W; ++j) {
         const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
         const bool better = nv < best;
-        best = better ? nv : best;
+        best = pgc_min(best, nv);
         bs = better ? s[j] : bs;
       }
       const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
