@@ -439,6 +439,17 @@ __global__ void k_pad_src(int32_t* __restrict__ s, int64_t n, int32_t npl) {
   if (i < n && s[i] < 0) s[i] = npl;
 }
 
+// Clears *flag when a source slot of layer l > first differs from the same
+// slot of layer `first` (then the class's sources are not layer-invariant).
+__global__ void k_src_invariant(const int32_t* __restrict__ s, int64_t lstride, int64_t first, int64_t nLayers,
+                                int32_t* __restrict__ flag) {
+  const int64_t n = (nLayers - first - 1) * lstride;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = i % lstride;
+    if (s[(first + 1) * lstride + i] != s[first * lstride + o]) *flag = 0;
+  }
+}
+
 // per-slot cost (float q-units or raw double) and time gathered by edge id
 template <typename CT>
 __global__ void k_fill_costs(const int32_t* __restrict__ eid, const int32_t* __restrict__ s, int64_t nSlots,
@@ -549,6 +560,15 @@ hipError_t launch_layout_fill(const int32_t* src, const int32_t* trgt, int64_t E
   const int64_t nLayers = numNodes / npl + 1;
   hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nLayers * npl, 256)), dim3(256), 0, st, cnt, npl, nLayers, nChunks,
                      width, s0, s1, e0, e1);
+  return hipGetLastError();
+}
+
+hipError_t launch_src_invariant(const int32_t* s, int64_t lstride, int64_t first, int64_t nLayers, int32_t* flag,
+                                hipStream_t st) {
+  const int64_t n = (nLayers - first - 1) * lstride;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_src_invariant, dim3((unsigned)std::min<int64_t>(nb(n, 256), 4096)), dim3(256), 0, st, s,
+                     lstride, first, nLayers, flag);
   return hipGetLastError();
 }
 

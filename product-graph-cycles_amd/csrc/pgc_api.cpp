@@ -60,6 +60,7 @@ hipError_t launch_conj_energy(const double*, const double*, int64_t, const int32
                               double*, hipStream_t);
 hipError_t launch_max_f64(const double*, int64_t, double*, hipStream_t);
 hipError_t launch_cycle_grad(const CycleItem*, int, hipStream_t);
+hipError_t launch_src_invariant(const int32_t*, int64_t, int64_t, int64_t, int32_t*, hipStream_t);
 // solver.hip
 size_t solver_lds_bytes(int npl, int K);
 hipError_t launch_solver(const SolverJob*, int, int, int*, int, int, bool, bool, bool, size_t, hipStream_t);
@@ -257,6 +258,7 @@ struct pgc_graph {
   int32_t *d_s[2] = {nullptr, nullptr}, *d_e[2] = {nullptr, nullptr};
   int32_t width[2] = {0, 0};
   int32_t lstride[2] = {0, 0};
+  int32_t src_inv[2] = {0, 0};  // per class: source slots are the same on every layer
   int32_t maxw = 0;
   int32_t max_in_degree = 0;
   int32_t* d_cnt = nullptr;
@@ -377,7 +379,7 @@ int build_layout(pgc_graph* g) {
   int32_t *d_dropped = nullptr, *d_fill = nullptr;
   HIPCHK(dalloc(&g->d_cnt, 2 * nTot));
   HIPCHK(dalloc(&d_fill, 2 * nTot));
-  HIPCHK(dalloc(&d_dropped, 1));
+  HIPCHK(dalloc(&d_dropped, 2));
   HIPCHK(dalloc(&g->d_width, 4));
   HIPCHK(hipMemsetAsync(g->d_cnt, 0, 2 * nTot * 4, st));
   HIPCHK(hipMemsetAsync(d_fill, 0, 2 * nTot * 4, st));
@@ -419,7 +421,16 @@ int build_layout(pgc_graph* g) {
   HIPCHK(launch_layout_fill(g->d_src, g->d_trgt, g->E, npl, g->numNodes, nC, d_fill, g->d_cnt, g->d_width,
                             g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
   for (int c = 0; c < 2; ++c) HIPCHK(launch_pad_src(g->d_s[c], g->nslots[c] + kSlotPad, (int32_t)npl, st));
+  // layer-invariant sources: every layer repeats the X adjacency (the
+  // interlayer class from layer 1 on; layer 0 has no interlayer in-edges)
+  HIPCHK(hipMemsetAsync(d_dropped, 0xff, 8, st));
+  HIPCHK(launch_src_invariant(g->d_s[0], g->lstride[0], 0, g->nLayers, d_dropped, st));
+  HIPCHK(launch_src_invariant(g->d_s[1], g->lstride[1], 1, g->nLayers, d_dropped + 1, st));
+  int32_t inv[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(inv, d_dropped, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  g->src_inv[0] = inv[0] != 0;
+  g->src_inv[1] = inv[1] != 0;
   cache_free(d_dropped);
   cache_free(d_fill);
   g->layout = true;
@@ -601,6 +612,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
     SlotClass& S = c == 0 ? j.intra : j.inter;
     S.w = g->width[c];
     S.lstride = g->lstride[c];
+    S.src_inv = g->src_inv[c];
     S.src = g->d_s[c];
     S.cost = g->d_cost[c];
     S.time = g->d_time[c];
@@ -1133,6 +1145,7 @@ int pgc_graph_derive(pgc_graph* base, const double* feat_diff, int on_device, pg
     g->d_e[c] = b->d_e[c];
     g->width[c] = b->width[c];
     g->lstride[c] = b->lstride[c];
+    g->src_inv[c] = b->src_inv[c];
   }
   g->d_width = b->d_width;
   g->maxw = b->maxw;

@@ -284,8 +284,12 @@ struct Solver {
                                                               predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
                                                               ratio, ub);
     else
-      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,
-                                                       bor.prev1, bor.odd ? 1 : 0, ratio, ub);
+      r = K == 1 ? sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W, true>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+                                                                     predLds ? 1 : 0, bor.prev0, bor.prev1,
+                                                                     bor.odd ? 1 : 0, ratio, ub)
+                 : sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+                                                               predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
+                                                               ratio, ub);
     bor.prev0 = r.p0;
     bor.prev1 = r.p1;
     bor.odd = r.odd != 0;

@@ -45,6 +45,7 @@ __device__ __forceinline__ SlotClass uni_class(const PGC_GLOBAL SlotClass& c) {
   r.eid = uni(c.eid);
   r.w = uni(c.w);
   r.lstride = uni(c.lstride);
+  r.src_inv = uni(c.src_inv);
   return r;
 }
 
@@ -60,7 +61,10 @@ struct SweepState {
   int p0, p1, odd;  // barrier-or state (see BarrierOr)
 };
 
-template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
+// ONLY_K1: the instance for jobs with K == 1 (interlayer blocks only). The K == 1
+// loops and the general loop are separate functions, so that each gets its
+// own register allocation.
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool ONLY_K1 = false>
 struct Sweep {
   const PGC_GLOBAL SolverJob* J;
   PGC_LDS double* D;     // [2K][ds] distance rows; entry npl of every row is +inf
@@ -276,6 +280,75 @@ struct Sweep {
   // The prefetch is unconditional (past the last layer it reloads layer L,
   // in bounds, unused): with a branch around it the wait analysis would
   // merge the two paths and wait for the prefetch as well.
+  // run_layers_k1 for layer-invariant interlayer sources: the sources of the
+  // owned chunk are loaded once (from layer 1) and held in registers, and a
+  // block loads only its costs and times.
+  struct SlotsCT {
+    CT c[MAXC][W];
+    float t[MAXC][W];
+  };
+  __device__ __forceinline__ void load_ct(SlotsCT& b, int layer) const {
+    const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)Sx.cost);
+    const PGC_GLOBAL float* __restrict__ tim = gp(Sx.time);
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int ci = chunk_of(i);
+      const int base = layer * Sx.lstride + lane + (ci < nC ? ci * W * 64 : 0);
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        b.c[i][j] = cost[base + j * 64];
+        if (HAS_TIME) b.t[i][j] = tim[base + j * 64];
+      }
+    }
+  }
+  __device__ __forceinline__ bool compute_inv(const int (&s)[MAXC][W], const SlotsCT& b, int layer, double ratio,
+                                              double rt, double ub, double (&dmin_r)[MAXC]) {
+    bool imp = false;
+    const int cur = layer & 1, prev = 1 - cur;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      const int v = vert(i);
+      if (v >= 0) imp |= inter_vertex<true>(i, v, layer, cur, prev, s[i], b.c[i], b.t[i], ratio, rt, ub, dmin_r);
+    }
+    return imp;
+  }
+  __device__ __forceinline__ int run_layers_k1_inv(double ratio, double rt, double ub, double (&dmin_r)[MAXC]) {
+    const int nb = L;
+    int s[MAXC][W];
+    {
+      const PGC_GLOBAL int* __restrict__ src = gp(Sx.src);
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) {
+        const int ci = chunk_of(i);
+        const int base = Sx.lstride + lane + (ci < nC ? ci * W * 64 : 0);
+#pragma unroll
+        for (int j = 0; j < W; ++j) s[i][j] = src[base + j * 64];
+      }
+    }
+    SlotsCT A, B;
+    load_ct(A, 1);
+    for (int b = 0; b < nb; b += 2) {
+      int layer = b + 1;
+      load_ct(B, min(layer + 1, L));
+      bool imp = compute_inv(s, A, layer, ratio, rt, ub, dmin_r);
+      if (layer == L - 1) {
+        if (!bor(imp)) return 0;
+      } else {
+        __syncthreads();
+      }
+      if (b + 1 >= nb) break;
+      layer = b + 2;
+      load_ct(A, min(layer + 1, L));
+      imp = compute_inv(s, B, layer, ratio, rt, ub, dmin_r);
+      if (layer == L - 1) {
+        if (!bor(imp)) return 0;
+      } else {
+        __syncthreads();
+      }
+    }
+    return 1;
+  }
+
   __device__ __forceinline__ int run_layers_k1(Slots& A, Slots& B, double ratio, double rt, double ub,
                                                double (&dmin_r)[MAXC]) {
     const int nb = L;
@@ -321,7 +394,11 @@ struct Sweep {
     double dmin_r[MAXC];
     const int nb = K > 1 ? 2 * L + 1 : L;
     Slots A, B;
-    if (PF && K == 1 && nb > 0) return run_layers_k1(A, B, ratio, rt, ub, dmin_r);
+    if constexpr (ONLY_K1 && PF) {
+      if (nb <= 0) return 1;
+      if (Sx.src_inv) return run_layers_k1_inv(ratio, rt, ub, dmin_r);
+      return run_layers_k1(A, B, ratio, rt, ub, dmin_r);
+    }
     bool inter;
     int layer;
     if (PF && nb > 0) {
@@ -386,11 +463,11 @@ struct Sweep {
 
 };
 
-template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool ONLY_K1 = false>
 __device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                 int32_t* predLg, int predLds, int p0, int p1, int odd, double ratio,
                                                 double ub) {
-  Sweep<CT, HAS_TIME, MINCOST, MAXC, W> w;
+  Sweep<CT, HAS_TIME, MINCOST, MAXC, W, ONLY_K1> w;
   w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
   w.D = (PGC_LDS double*)uni(Dg);
   w.Dmin = (PGC_LDS double*)uni(Dming);
