@@ -422,9 +422,10 @@ int build_layout(pgc_graph* g) {
                             g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
   for (int c = 0; c < 2; ++c) HIPCHK(launch_pad_src(g->d_s[c], g->nslots[c] + kSlotPad, (int32_t)npl, st));
   // layer-invariant sources: every layer repeats the X adjacency (the
-  // interlayer class from layer 1 on; layer 0 has no interlayer in-edges)
+  // interlayer class from layer 1 on, as layer 0 has no interlayer in-edges;
+  // the intralayer class up to layer L-1, as the closing layer L has none)
   HIPCHK(hipMemsetAsync(d_dropped, 0xff, 8, st));
-  HIPCHK(launch_src_invariant(g->d_s[0], g->lstride[0], 0, g->nLayers, d_dropped, st));
+  HIPCHK(launch_src_invariant(g->d_s[0], g->lstride[0], 0, g->nLayers - 1, d_dropped, st));
   HIPCHK(launch_src_invariant(g->d_s[1], g->lstride[1], 1, g->nLayers, d_dropped + 1, st));
   int32_t inv[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(inv, d_dropped, 8, hipMemcpyDeviceToHost, st));

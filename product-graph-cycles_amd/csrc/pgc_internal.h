@@ -30,8 +30,9 @@ struct SlotClass {
   const int32_t* eid;    // product-graph edge id (-1 for padding)
   int32_t w;             // slots per lane of every chunk (the class's maximum in-degree)
   int32_t lstride;       // slots per layer = nChunks * w * 64
-  int32_t src_inv;       // 1: every layer's source slots equal the first layer's (layer 0
-                         // intralayer, layer 1 interlayer): the same X adjacency on every layer
+  int32_t src_inv;       // 1: the source slots repeat on every layer: the intralayer class on
+                         // layers 0..L-1 (layer L has no intralayer edges), the interlayer
+                         // class on layers 1..L (the same X adjacency on every layer)
 };
 
 // Conjugate energy (cost, time) kinds of the device kernel.

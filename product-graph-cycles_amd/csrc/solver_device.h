@@ -280,7 +280,7 @@ struct Solver {
       r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, nullptr, 0, bor.prev0,
                                                         bor.prev1, bor.odd ? 1 : 0, ratio, ub);
     else if constexpr (!UNROLLED)
-      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true, (W == 8 && MAXC <= (HAS_TIME ? 2 : 3) ? MAXC : 0)>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
                                                               predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
                                                               ratio, ub);
     else

@@ -515,7 +515,10 @@ struct DistPtr<true> {
 
 // LD: the rows live in LDS (graphs with many chunks per wave, for which the
 // unrolled sweep would hold too many registers), else in global memory.
-template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false>
+// MC > 0: at most MC chunks per wave (rows in LDS); with layer-invariant
+// sources in both classes, the sources of the owned chunks are loaded once
+// per pass and held in registers, and a step loads only costs and times.
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0>
 struct SweepG {
   typedef typename DistPtr<LD>::type DP;
   const PGC_GLOBAL SolverJob* J;
@@ -663,8 +666,173 @@ struct SweepG {
     return imp;
   }
 
+  // --- steps with the sources held in registers (MC > 0)
+  __device__ __forceinline__ void load_ct(const SlotClass& S, int b, CT (&c)[W], float (&t)[W]) const {
+    const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)S.cost);
+    const PGC_GLOBAL float* __restrict__ tim = gp(S.time);
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      c[j] = cost[b + j * 64];
+      if (HAS_TIME) t[j] = tim[b + j * 64];
+    }
+  }
+  __device__ __forceinline__ bool inter_cached(int layer, int cur, int prev, const int (&sX)[MC > 0 ? MC : 1][W],
+                                               double ratio, double rt, double ub) {
+    bool imp = false;
+    const DP Dp = (K == 1) ? D + prev * ds : Dmin;
+#pragma unroll
+    for (int i = 0; i < MC; ++i) {
+      const int c = wave + i * nw;
+      if (c >= nC) break;
+      CT cc[W];
+      float t[W];
+      load_ct(Sx, layer * Sx.lstride + c * W * 64 + lane, cc, t);
+      const int v = c * 64 + lane;
+      if (v >= npl) continue;
+      const int(&s)[W] = sX[i];
+      double dv[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
+      double best = PGC_INF;
+      int bs = npl;
+      CT bc = 0;
+      float bt = 0;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
+        const bool better = nv < best;
+        best = pgc_min(best, nv);
+        bs = better ? s[j] : bs;
+        bc = better ? cc[j] : bc;
+        if (HAS_TIME) bt = better ? t[j] : bt;
+      }
+      double res = PGC_INF;
+      if (best < PGC_INF && best < ub) {
+        int ks = 0;
+        if (K > 1) {
+          const double bw = weight(bc, bt, ratio, rt);
+          for (int kk = 0; kk < K; ++kk) {
+            if (D[(prev + kk) * ds + bs] + bw == best) {
+              ks = kk;
+              break;
+            }
+          }
+        }
+        const int64_t pi = (int64_t)layer * npl * K + v;
+        const int pv = bs + (layer - 1) * npl * K + ks * npl;
+        if (LD && predLds)
+          predL[pi] = pv;
+        else
+          pred[pi] = pv;
+        res = best;
+        imp = true;
+      }
+      D[cur * ds + v] = res;
+    }
+    return imp;
+  }
+  __device__ __forceinline__ bool intra_cached(int layer, int cur, int k, const int (&sI)[MC > 0 ? MC : 1][W],
+                                               double ratio, double rt, double ub) {
+    bool imp = false;
+    const DP Dk = D + (cur + k - 1) * ds;
+#pragma unroll
+    for (int i = 0; i < MC; ++i) {
+      const int c = wave + i * nw;
+      if (c >= nC) break;
+      CT cc[W];
+      float t[W];
+      load_ct(Sa, layer * Sa.lstride + c * W * 64 + lane, cc, t);
+      const int v = c * 64 + lane;
+      if (v >= npl) continue;
+      const int(&s)[W] = sI[i];
+      double dv[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) dv[j] = Dk[s[j]];
+      double best = PGC_INF;
+      int bs = npl;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
+        const bool better = nv < best;
+        best = pgc_min(best, nv);
+        bs = better ? s[j] : bs;
+      }
+      const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
+      double res = curv;
+      if (best < curv && best < ub) {
+        const int64_t pi = (int64_t)layer * npl * K + (int64_t)k * npl + v;
+        const int pv = bs + layer * npl * K + (k - 1) * npl;
+        if (LD && predLds)
+          predL[pi] = pv;
+        else
+          pred[pi] = pv;
+        res = best;
+        imp = true;
+      }
+      D[(cur + k) * ds + v] = res;
+      if (k == K - 1) {
+        double m = D[cur * ds + v];
+        for (int kk = 1; kk < K; ++kk) {
+          const double x = D[(cur + kk) * ds + v];
+          m = x < m ? x : m;
+        }
+        Dmin[v] = m;
+      }
+    }
+    return imp;
+  }
+  __device__ __forceinline__ int run_layers_cached(double ratio, double rt, double ub) {
+    constexpr int M = MC > 0 ? MC : 1;
+    int sX[M][W], sI[M][W];
+    {
+      const PGC_GLOBAL int* __restrict__ srcX = gp(Sx.src);
+      const PGC_GLOBAL int* __restrict__ srcI = gp(Sa.src);
+#pragma unroll
+      for (int i = 0; i < M; ++i) {
+        const int c = wave + i * nw;
+        const int o = (c < nC ? c : 0) * W * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          sX[i][j] = srcX[Sx.lstride + o + j * 64];  // interlayer pattern of layer 1
+          sI[i][j] = K > 1 ? srcI[o + j * 64] : 0;   // intralayer pattern of layer 0
+        }
+      }
+    }
+    for (int layer = 0; layer <= L; ++layer) {
+      const int cur = (layer & 1) * K;
+      const int prev = K - cur;
+      bool imp = false;
+      const bool check = layer == L - 1 && layer > 0;
+      if (layer > 0) {
+        imp = inter_cached(layer, cur, prev, sX, ratio, rt, ub);
+        if (K == 1) {
+          if (check) {
+            if (!bor(imp)) return 0;
+          } else {
+            __syncthreads();
+          }
+          continue;
+        }
+        __syncthreads();
+      }
+      for (int k = 1; k < K; ++k) {
+        // the closing layer L has its own (empty) intralayer slots
+        imp |= layer < L ? intra_cached(layer, cur, k, sI, ratio, rt, ub) : intra_step(layer, cur, k, ratio, rt, ub);
+        if (k == K - 1 && check) {
+          if (!bor(imp)) return 0;
+        } else {
+          __syncthreads();
+        }
+      }
+    }
+    return 1;
+  }
+
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     const double rt = __dmul_rn(ratio, uni(J->time_const));
+    if constexpr (MC > 0) {
+      if (Sx.src_inv && (K == 1 || Sa.src_inv)) return run_layers_cached(ratio, rt, ub);
+    }
     for (int layer = 0; layer <= L; ++layer) {
       const int cur = (layer & 1) * K;
       const int prev = K - cur;
@@ -696,14 +864,15 @@ struct SweepG {
   }
 };
 
-template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false>
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0>
 __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                        int32_t* predLg, int predLds, int p0, int p1, int odd,
                                                        double ratio, double ub) {
-  SweepG<CT, HAS_TIME, MINCOST, W, LD> w;
+  typedef SweepG<CT, HAS_TIME, MINCOST, W, LD, MC> SW;
+  SW w;
   w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
-  w.D = (typename SweepG<CT, HAS_TIME, MINCOST, W, LD>::DP)uni(Dg);
-  w.Dmin = (typename SweepG<CT, HAS_TIME, MINCOST, W, LD>::DP)uni(Dming);
+  w.D = (typename SW::DP)uni(Dg);
+  w.Dmin = (typename SW::DP)uni(Dming);
   w.sh = (PGC_LDS Shared*)uni(shg);
   w.pred = (PGC_GLOBAL int32_t*)uni(w.J->pred);
   w.predL = (PGC_LDS int32_t*)uni(predLg);
