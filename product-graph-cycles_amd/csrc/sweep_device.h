@@ -567,103 +567,131 @@ struct SweepG {
     }
   }
 
-  __device__ __forceinline__ bool inter_step(int layer, int cur, int prev, double ratio, double rt, double ub) {
+  // First-minimum scan and write-back of one interlayer vertex v whose slots
+  // and gathered source distances are loaded.
+  __device__ __forceinline__ bool inter_relax(int layer, int cur, int prev, int v, const int (&s)[W],
+                                              const CT (&cc)[W], const float (&t)[W], const double (&dv)[W],
+                                              double ratio, double rt, double ub) {
+    double best = PGC_INF;
+    int bs = npl;
+    CT bc = 0;
+    float bt = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
+      const bool better = nv < best;
+      best = pgc_min(best, nv);
+      bs = better ? s[j] : bs;
+      bc = better ? cc[j] : bc;
+      if (HAS_TIME) bt = better ? t[j] : bt;
+    }
+    double res = PGC_INF;
     bool imp = false;
-    const DP Dp = (K == 1) ? D + prev * ds : Dmin;
-    for (int c = wave; c < nC; c += nw) {
-      int s[W];
-      CT cc[W];
-      float t[W];
-      load(Sx, layer * Sx.lstride + c * W * 64 + lane, s, cc, t);
-      const int v = c * 64 + lane;
-      if (v >= npl) continue;
-      double dv[W];
-#pragma unroll
-      for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
-      double best = PGC_INF;
-      int bs = npl;
-      CT bc = 0;
-      float bt = 0;
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
-        const bool better = nv < best;
-        best = pgc_min(best, nv);
-        bs = better ? s[j] : bs;
-        bc = better ? cc[j] : bc;
-        if (HAS_TIME) bt = better ? t[j] : bt;
-      }
-      double res = PGC_INF;
-      if (best < PGC_INF && best < ub) {
-        int ks = 0;
-        if (K > 1) {
-          const double bw = weight(bc, bt, ratio, rt);
-          for (int kk = 0; kk < K; ++kk) {
-            if (D[(prev + kk) * ds + bs] + bw == best) {
-              ks = kk;
-              break;
-            }
+    if (best < PGC_INF && best < ub) {
+      int ks = 0;
+      if (K > 1) {
+        const double bw = weight(bc, bt, ratio, rt);
+        for (int kk = 0; kk < K; ++kk) {
+          if (D[(prev + kk) * ds + bs] + bw == best) {
+            ks = kk;
+            break;
           }
         }
-        const int64_t pi = (int64_t)layer * npl * K + v;
-        const int pv = bs + (layer - 1) * npl * K + ks * npl;
-        if (LD && predLds)
-          predL[pi] = pv;
-        else
-          pred[pi] = pv;
-        res = best;
-        imp = true;
       }
-      D[cur * ds + v] = res;
+      const int64_t pi = (int64_t)layer * npl * K + v;
+      const int pv = bs + (layer - 1) * npl * K + ks * npl;
+      if (LD && predLds)
+        predL[pi] = pv;
+      else
+        pred[pi] = pv;
+      res = best;
+      imp = true;
+    }
+    D[cur * ds + v] = res;
+    return imp;
+  }
+
+  // The same for an intralayer vertex at depth k (and Dmin at the last depth).
+  __device__ __forceinline__ bool intra_relax(int layer, int cur, int k, int v, const int (&s)[W],
+                                              const CT (&cc)[W], const float (&t)[W], const double (&dv)[W],
+                                              double ratio, double rt, double ub) {
+    double best = PGC_INF;
+    int bs = npl;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
+      const bool better = nv < best;
+      best = pgc_min(best, nv);
+      bs = better ? s[j] : bs;
+    }
+    const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
+    double res = curv;
+    bool imp = false;
+    if (best < curv && best < ub) {
+      const int64_t pi = (int64_t)layer * npl * K + (int64_t)k * npl + v;
+      const int pv = bs + layer * npl * K + (k - 1) * npl;
+      if (LD && predLds)
+        predL[pi] = pv;
+      else
+        pred[pi] = pv;
+      res = best;
+      imp = true;
+    }
+    D[(cur + k) * ds + v] = res;
+    if (k == K - 1) {
+      double m = D[cur * ds + v];
+      for (int kk = 1; kk < K; ++kk) {
+        const double x = D[(cur + kk) * ds + v];
+        m = x < m ? x : m;
+      }
+      Dmin[v] = m;
     }
     return imp;
   }
 
-  __device__ __forceinline__ bool intra_step(int layer, int cur, int k, double ratio, double rt, double ub) {
+  // Chunks per loop iteration. One: with two chunks' loads and gathers in
+  // flight together the global-table sweep (C2-size graphs) measured no
+  // faster, and one chunk keeps the registers free.
+  static constexpr int NG = 1;
+
+  // One interlayer (k == 0) or intralayer (depth k) step over the wave's chunks.
+  __device__ __forceinline__ bool step_n(int layer, int cur, int prev, int k, double ratio, double rt, double ub) {
     bool imp = false;
-    const DP Dk = D + (cur + k - 1) * ds;
-    for (int c = wave; c < nC; c += nw) {
-      int s[W];
-      CT cc[W];
-      float t[W];
-      load(Sa, layer * Sa.lstride + c * W * 64 + lane, s, cc, t);
-      const int v = c * 64 + lane;
-      if (v >= npl) continue;
-      double dv[W];
+    const bool inter = k == 0;
+    const SlotClass& S = inter ? Sx : Sa;
+    const DP Dg = inter ? ((K == 1) ? D + prev * ds : Dmin) : D + (cur + k - 1) * ds;
+    for (int c0 = wave; c0 < nC; c0 += NG * nw) {
+      int s[NG][W];
+      CT cc[NG][W];
+      float t[NG][W];
 #pragma unroll
-      for (int j = 0; j < W; ++j) dv[j] = Dk[s[j]];
-      double best = PGC_INF;
-      int bs = npl;
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
-        const bool better = nv < best;
-        best = pgc_min(best, nv);
-        bs = better ? s[j] : bs;
+      for (int g = 0; g < NG; ++g) {
+        const int c = c0 + g * nw;
+        load(S, layer * S.lstride + (c < nC ? c : c0) * W * 64 + lane, s[g], cc[g], t[g]);
       }
-      const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
-      double res = curv;
-      if (best < curv && best < ub) {
-        const int64_t pi = (int64_t)layer * npl * K + (int64_t)k * npl + v;
-        const int pv = bs + layer * npl * K + (k - 1) * npl;
-        if (LD && predLds)
-          predL[pi] = pv;
+      double dv[NG][W];
+#pragma unroll
+      for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int j = 0; j < W; ++j) dv[g][j] = Dg[s[g][j]];
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const int c = c0 + g * nw;
+        const int v = c * 64 + lane;
+        if (c >= nC || v >= npl) continue;
+        if (inter)
+          imp |= inter_relax(layer, cur, prev, v, s[g], cc[g], t[g], dv[g], ratio, rt, ub);
         else
-          pred[pi] = pv;
-        res = best;
-        imp = true;
-      }
-      D[(cur + k) * ds + v] = res;
-      if (k == K - 1) {
-        double m = D[cur * ds + v];
-        for (int kk = 1; kk < K; ++kk) {
-          const double x = D[(cur + kk) * ds + v];
-          m = x < m ? x : m;
-        }
-        Dmin[v] = m;
+          imp |= intra_relax(layer, cur, k, v, s[g], cc[g], t[g], dv[g], ratio, rt, ub);
       }
     }
     return imp;
+  }
+  __device__ __forceinline__ bool inter_step(int layer, int cur, int prev, double ratio, double rt, double ub) {
+    return step_n(layer, cur, prev, 0, ratio, rt, ub);
+  }
+  __device__ __forceinline__ bool intra_step(int layer, int cur, int k, double ratio, double rt, double ub) {
+    return step_n(layer, cur, K - cur, k, ratio, rt, ub);
   }
 
   // --- steps with the sources held in registers (MC > 0)
