@@ -521,12 +521,21 @@ bool wrap_fits_lds(const pgc_graph* g, int K) {
   return b + (size_t)g->npl * 8 <= 160 * 1024 - 1024;
 }
 
+// K == 1 graphs of the unrolled sweep (one chunk per wave, 8 slots) with pred
+// and the wrap map in LDS: two rows of chain origins follow when they fit.
+bool org_fits_lds(const pgc_graph* g, int K) {
+  if (K != 1 || kernel_maxc(g, K) != 1 || g->maxw > 8 || !pred_fits_lds(g, K) || !wrap_fits_lds(g, K)) return false;
+  const size_t b = solver_lds_bytes((int)g->npl, K) + (size_t)(g->L + 1) * K * g->npl * 4 + (size_t)g->npl * 8;
+  return b + (size_t)g->npl * 8 <= 160 * 1024 - 1024;
+}
+
 // dynamic LDS of a job's launch
 size_t job_lds_bytes(const pgc_graph* g, int K) {
   if (kernel_maxc(g, K) == 0) return 0;
   size_t b = solver_lds_bytes((int)g->npl, K);
   if (pred_fits_lds(g, K)) b += (size_t)(g->L + 1) * K * g->npl * 4;
   if (wrap_fits_lds(g, K)) b += (size_t)g->npl * 8;
+  if (org_fits_lds(g, K)) b += (size_t)g->npl * 8;
   return b;
 }
 
@@ -645,6 +654,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.dtab = s.dtab;
   j.pred_lds = pred_fits_lds(g, K) ? 1 : 0;
   j.wrap_lds = wrap_fits_lds(g, K) ? 1 : 0;
+  j.org_lds = org_fits_lds(g, K) ? 1 : 0;
   j.vc_eid = s.vc_eid;
   j.vc_c = s.vc_c;
   j.vc_t = s.vc_t;

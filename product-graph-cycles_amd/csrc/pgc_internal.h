@@ -139,6 +139,8 @@ struct SolverJob {
   int32_t vc_cap, vc_stride, out_cap;
   int32_t pred_lds;       // 1: pred lives in LDS after the distance table (small graphs)
   int32_t wrap_lds;       // 1: the wrap map (wnext, wlen) lives in LDS after the table and pred
+  int32_t org_lds;        // 1 (K == 1, pred and wrap map in LDS): two rows of chain origins in LDS
+                          // after the wrap map, kept by the sweep, replace the wrap chase
   double* dtab;           // [(2K+1)*(npl+1)] distance table in global memory (graphs beyond the LDS)
 };
 

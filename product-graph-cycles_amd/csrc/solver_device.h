@@ -201,6 +201,11 @@ __device__ __forceinline__ bool lookup_edge(const JT& J, int a, int b, int& eid,
 template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
 struct Solver {
   static constexpr bool DYN = MAXC == 0;
+  // The unrolled sweep (slots of the next block prefetched into registers)
+  // pays off with one chunk per wave and 8 slots per vertex; with more, its
+  // register sets spill, and the sweep that takes one chunk at a time (rows
+  // in LDS) is faster.
+  static constexpr bool UNROLLED = MAXC == 1 && W == 8;
   const PGC_CONST SolverJob& J;
   double* D;     // [2K][ds] distance rows; entry npl of every row is +inf (padding slots point there)
   double* Dmin;  // [ds]
@@ -210,6 +215,8 @@ struct Solver {
   PGC_LDS int32_t* WnL;  // wrap map in LDS after pred (J.wrap_lds), else J.wnext/J.wlen
   PGC_LDS int32_t* WlL;
   bool wrapLds;
+  PGC_LDS int32_t* OrgL;  // chain-origin rows after the wrap map (J.org_lds), see sweep_layers
+  bool orgOn;
   FastDiv fdNpl, fdPer;  // division by npl and by npl*K
   int npl, ds, K, L, nC, wave, lane, nw, npass;
   BarrierOr bor;
@@ -222,6 +229,8 @@ struct Solver {
     wrapLds = !DYN && J.wrap_lds != 0;
     WnL = Pl + (predLds ? (int64_t)(J.L + 1) * J.K * J.npl : 0);
     WlL = WnL + J.npl;
+    orgOn = UNROLLED && !MINCOST && J.K == 1 && predLds && wrapLds && J.org_lds != 0;
+    OrgL = WnL + 2 * J.npl;
     npl = J.npl;
     ds = npl + 1;
     K = J.K;
@@ -271,11 +280,6 @@ struct Solver {
   // gets the whole register file; the caller's state is saved once per sweep).
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     SweepState r;
-    // The unrolled sweep (slots of the next block prefetched into registers)
-    // pays off with one chunk per wave and 8 slots per vertex; with more, its
-    // register sets spill, and the sweep that takes one chunk at a time (rows
-    // in LDS) is faster.
-    constexpr bool UNROLLED = MAXC == 1 && W == 8;
     if constexpr (DYN)
       r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, nullptr, 0, bor.prev0,
                                                         bor.prev1, bor.odd ? 1 : 0, ratio, ub);
@@ -286,7 +290,8 @@ struct Solver {
     else
       r = K == 1 ? sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W, true>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
                                                                      predLds ? 1 : 0, bor.prev0, bor.prev1,
-                                                                     bor.odd ? 1 : 0, ratio, ub)
+                                                                     bor.odd ? 1 : 0, ratio, ub, (int32_t*)OrgL,
+                                                                     orgOn ? 1 : 0)
                  : sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
                                                                predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
                                                                ratio, ub);
@@ -382,10 +387,41 @@ struct Solver {
           pp = pget(pp);
         }
       }
+    } else if (orgOn) {
+      chase_from_origins();
     } else {
       chase_wraps_fn<DYN ? 1 : MAXC>(predLds ? (int32_t*)Pl : J.pred, predLds ? 1 : 0,
                                      wrapLds ? (int32_t*)WnL : J.wnext, wrapLds ? (int32_t*)WlL : J.wlen,
                                      wrapLds ? 1 : 0, npl, nC, stride);
+    }
+  }
+
+  // The wrap map from the chain origins the K == 1 sweep kept (the result of
+  // the chase, without walking the chains): the predecessor of layer-0 vertex
+  // v is a node y of layer L-1 (the fold copies it from the closing copy), and
+  // its chain visits one node per layer: it ends at vertex o of layer 0 after
+  // L steps, or after L - m steps at a node of layer m without predecessor.
+  __device__ __forceinline__ void chase_from_origins() {
+    const PGC_LDS int32_t* last = OrgL + ((L - 1) & 1) * npl;
+#pragma unroll
+    for (int i = 0; i < np(); ++i) {
+      const int v = vert(i);
+      if (v < 0) continue;
+      const int p = pget(v);
+      int wn, wl;
+      if (p == -1) {
+        wn = -1;
+        wl = 0;
+      } else if (p < npl) {
+        wn = p;
+        wl = 1;
+      } else {
+        const int o = last[p - (L - 1) * npl];
+        wn = o >= 0 ? o : -1;
+        wl = o >= 0 ? L : L - (-1 - o);
+      }
+      WnL[v] = wn;
+      WlL[v] = wl;
     }
   }
 

@@ -73,6 +73,12 @@ struct Sweep {
   PGC_GLOBAL int32_t* pred;
   PGC_LDS int32_t* predL;  // pred in LDS (small graphs), selected by predLds
   int predLds;
+  // chain origins (K == 1, pred in LDS), or nullptr. Row l & 1 holds, for
+  // every vertex x of layer l, where the predecessor chain from (l, x) ends:
+  // the layer-0 vertex it reaches (>= 0), or -1 - m when the node of layer m
+  // on it has no predecessor.
+  PGC_LDS int32_t* orgL;
+  int orgOn;  // orgL is in use (an explicit flag: null pointers do not survive the cast to LDS)
   SlotClass Sa, Sx;  // intralayer and interlayer slot classes, read once per sweep
   int npl, ds, K, L, nC, wave, lane, nw;
   int p0, p1, odd;
@@ -181,6 +187,19 @@ struct Sweep {
     }
     D[cur * ds + v] = res;
     dmin_r[i] = res;
+    if (K1 && orgOn) {
+      // origin of (layer, v): through its new predecessor, or through the
+      // one it keeps from an earlier sweep (K == 1: always a node of layer-1)
+      const PGC_LDS int32_t* prevRow = orgL + ((layer - 1) & 1) * npl;
+      int o;
+      if (imp) {
+        o = layer == 1 ? bs : prevRow[bs];
+      } else {
+        const int p = predL[(int64_t)layer * npl + v];
+        o = p == -1 ? -1 - layer : (layer == 1 ? p : prevRow[p - (layer - 1) * npl]);
+      }
+      orgL[(layer & 1) * npl + v] = o;
+    }
     return imp;
   }
 
@@ -466,8 +485,10 @@ struct Sweep {
 template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool ONLY_K1 = false>
 __device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                 int32_t* predLg, int predLds, int p0, int p1, int odd, double ratio,
-                                                double ub) {
+                                                double ub, int32_t* orgLg = nullptr, int orgOn = 0) {
   Sweep<CT, HAS_TIME, MINCOST, MAXC, W, ONLY_K1> w;
+  w.orgOn = ONLY_K1 ? uni(orgOn) : 0;
+  w.orgL = (PGC_LDS int32_t*)uni(orgLg);
   w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
   w.D = (PGC_LDS double*)uni(Dg);
   w.Dmin = (PGC_LDS double*)uni(Dming);
