@@ -217,6 +217,7 @@ struct Solver {
   bool wrapLds;
   PGC_LDS int32_t* OrgL;  // chain-origin rows after the wrap map (J.org_lds), see sweep_layers
   bool orgOn;
+  uint32_t inb = 0;  // unrolled variants: bit i set when owned vertex i is in the current branch
   FastDiv fdNpl, fdPer;  // division by npl and by npl*K
   int npl, ds, K, L, nC, wave, lane, nw, npass;
   BarrierOr bor;
@@ -307,11 +308,13 @@ struct Solver {
     for (int i = threadIdx.x; i < ds; i += PGC_BLOCK) Dmin[i] = PGC_INF;
     __syncthreads();
     int cnt = 0, last = -1;
+    inb = 0;
 #pragma unroll
     for (int i = 0; i < np(); ++i) {
       const int v = vert(i);
       if (v < 0) continue;
       const bool in = onlyV >= 0 ? (v == onlyV) : (gp(J.branch)[v] == bid);
+      if (!DYN && in) inb |= 1u << i;
       if (in) {
         D[v] = 0;
         pset(v, -1);
@@ -340,7 +343,7 @@ struct Solver {
 #pragma unroll
     for (int i = 0; i < np(); ++i) {
       const int v = vert(i);
-      if (v < 0 || !in_branch(v, bid, onlyV)) continue;
+      if (v < 0 || !(DYN ? in_branch(v, bid, onlyV) : ((inb >> i) & 1u) != 0)) continue;
       const double dl = D[lastRow * ds + v];
       const double bk = gp(J.backup)[v];
       if (dl < bk) {
@@ -635,8 +638,8 @@ struct Solver {
         candidates(gp((const int32_t*)J.wnext), gp((const int32_t*)J.wlen), bid, onlyV, zeroSearch, bestKey, bestV);
       } else {
         const Candidate c = candidates_fn<DYN ? 1 : MAXC>(
-            D, J.branch, bid, onlyV, zeroSearch ? 1 : 0, J.tol, (J.flags & FLAG_PICK_LAST_VERTEX) != 0 ? 1 : 0,
-            capSteps, wrapLds ? (int32_t*)WnL : J.wnext, wrapLds ? (int32_t*)WlL : J.wlen, wrapLds ? 1 : 0, npl, nC);
+            D, inb, zeroSearch ? 1 : 0, J.tol, (J.flags & FLAG_PICK_LAST_VERTEX) != 0 ? 1 : 0, capSteps,
+            wrapLds ? (int32_t*)WnL : J.wnext, wrapLds ? (int32_t*)WlL : J.wlen, wrapLds ? 1 : 0, npl, nC);
         bestKey = c.key;
         bestV = c.v;
       }

@@ -162,8 +162,7 @@ struct Candidate {
 // searching zero cycles, |D| < tol) layer-0 value whose walk closes within
 // the step cap; key -v (pick last) or D, ties to the lower vertex.
 template <int MAXC, typename WT>
-__device__ __forceinline__ Candidate candidates_run(const WrapGeom& g, const PGC_LDS double* D,
-                                                    const PGC_GLOBAL int32_t* branch, int bid, int onlyV,
+__device__ __forceinline__ Candidate candidates_run(const WrapGeom& g, const PGC_LDS double* D, uint32_t inb,
                                                     int zeroSearch, double tol, int pickLast, int64_t capSteps, WT Wn,
                                                     WT Wl) {
   Candidate c;
@@ -172,8 +171,7 @@ __device__ __forceinline__ Candidate candidates_run(const WrapGeom& g, const PGC
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     const int v = g.vert(i);
-    if (v < 0) continue;
-    if (onlyV >= 0 ? (v != onlyV) : (branch[v] != bid)) continue;
+    if (v < 0 || ((inb >> i) & 1u) == 0) continue;
     const double lb = D[v];
     if (!zeroSearch && lb >= 0) continue;
     if (zeroSearch && (lb <= -tol || lb >= tol)) continue;
@@ -188,15 +186,13 @@ __device__ __forceinline__ Candidate candidates_run(const WrapGeom& g, const PGC
   return c;
 }
 
+// inb: bit i set when the thread's owned vertex i is in the current branch.
 template <int MAXC>
-__device__ __noinline__ Candidate candidates_fn(double* D, int32_t* branch, int bid, int onlyV, int zeroSearch,
-                                                double tol, int pickLast, int64_t capSteps, int32_t* wn, int32_t* wl,
-                                                int wrapLds, int npl, int nC) {
+__device__ __noinline__ Candidate candidates_fn(double* D, uint32_t inb, int zeroSearch, double tol, int pickLast,
+                                                int64_t capSteps, int32_t* wn, int32_t* wl, int wrapLds, int npl,
+                                                int nC) {
   const WrapGeom g = wrap_geom(npl, nC);
   const PGC_LDS double* Dl = (const PGC_LDS double*)uni(D);
-  const PGC_GLOBAL int32_t* br = (const PGC_GLOBAL int32_t*)uni(branch);
-  bid = uni(bid);
-  onlyV = uni(onlyV);
   zeroSearch = uni(zeroSearch);
   tol = uni(tol);
   pickLast = uni(pickLast);
@@ -204,10 +200,10 @@ __device__ __noinline__ Candidate candidates_fn(double* D, int32_t* branch, int 
   wn = uni(wn);
   wl = uni(wl);
   if (uni(wrapLds))
-    return candidates_run<MAXC>(g, Dl, br, bid, onlyV, zeroSearch, tol, pickLast, capSteps,
-                                (const PGC_LDS int32_t*)wn, (const PGC_LDS int32_t*)wl);
-  return candidates_run<MAXC>(g, Dl, br, bid, onlyV, zeroSearch, tol, pickLast, capSteps,
-                              (const PGC_GLOBAL int32_t*)wn, (const PGC_GLOBAL int32_t*)wl);
+    return candidates_run<MAXC>(g, Dl, inb, zeroSearch, tol, pickLast, capSteps, (const PGC_LDS int32_t*)wn,
+                                (const PGC_LDS int32_t*)wl);
+  return candidates_run<MAXC>(g, Dl, inb, zeroSearch, tol, pickLast, capSteps, (const PGC_GLOBAL int32_t*)wn,
+                              (const PGC_GLOBAL int32_t*)wl);
 }
 
 }  // namespace pgc
