@@ -134,6 +134,12 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
  * (product_graph_generators.cpp:21-89) without rebuilding the graph. */
 int pgc_graph_set_feat_diff(pgc_graph* g, const double* feat_diff, int on_device);
 
+/* pgc_graph_set_feat_diff for n graphs of one context in one launch:
+ * feat_diff[i] is graph i's [nVX x nVY] row-major feat_diff in device memory,
+ * float64 (dtype 0) or float32 (dtype 1, widened exactly). No counterpart in
+ * the reference (batched perturbation samples). */
+int pgc_graphs_set_feat_diff(pgc_graph* const* graphs, const void* const* feat_diff, int64_t n, int dtype);
+
 /* A new graph that shares base's product space, ids and in-edge layout
  * (read-only) and owns its energy (from feat_diff, as above), quantized costs
  * and solver scratch: the per-sample graphs of a perturbation batch. base may

@@ -95,6 +95,26 @@ __global__ void k_energy(const double* __restrict__ VX, const double* __restrict
   }
 }
 
+// k_energy for a list of graphs (blockIdx.y = graph), each with its own
+// feat_diff; a float32 feat_diff is widened exactly.
+__global__ void k_benergy(const EItem* __restrict__ items) {
+  const EItem it = items[blockIdx.y];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < it.E; e += (int64_t)gridDim.x * blockDim.x) {
+    const int4 row = reinterpret_cast<const int4*>(it.rows)[e];
+    const int64_t y0 = row.x, y1 = row.y, x0 = row.z, x1 = row.w;
+    const double len = it.line ? edge_length(it.vx, it.vy, y0, y1, x0, x1) : 1.0;
+    const int64_t a = x0 * it.nVY + y0, b = x1 * it.nVY + y1;
+    const double fd = it.f32 ? (double)((const float*)it.fd)[a] + (double)((const float*)it.fd)[b]
+                             : ((const double*)it.fd)[a] + ((const double*)it.fd)[b];
+    if (!it.lengthMode) {
+      it.energy[e] = __dmul_rn(__dmul_rn(0.5, fd), len);
+    } else {
+      it.energy[e] = fd;
+      it.energy[it.E + e] = len;
+    }
+  }
+}
+
 // --------------------------------- cycle ratio in raw units and its gradient
 // lambda = sum_e c_e / sum_e t_e over a cycle's edges (walk order), with c_e
 // the raw energy of k_energy and t_e its time column (1 per edge without
@@ -528,6 +548,12 @@ hipError_t launch_quantize_batch(const QItem* d_items, int n, void* states, hipS
     hipLaunchKernelGGL(k_bselect_step, dim3(n), dim3(256), 0, st, S, shift);
   }
   hipLaunchKernelGGL(k_bquantize, dim3(32, n), dim3(256), 0, st, d_items, S, 100000.0);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_batch(const EItem* d_items, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_benergy, dim3(64, n), dim3(256), 0, st, d_items);
   return hipGetLastError();
 }
 

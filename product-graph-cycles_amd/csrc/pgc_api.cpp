@@ -37,6 +37,7 @@ hipError_t launch_energy(const double*, const double*, const int32_t*, int64_t, 
 hipError_t launch_quantize(double*, int64_t, void*, hipStream_t);
 hipError_t launch_quantize_batch(const QItem*, int, void*, hipStream_t);
 hipError_t launch_fill_costs_batch(const FItem*, int, hipStream_t);
+hipError_t launch_energy_batch(const EItem*, int, hipStream_t);
 size_t select_state_bytes();
 hipError_t launch_layout_count(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int32_t*, int32_t*,
                                hipStream_t);
@@ -205,6 +206,8 @@ struct pgc_context {
   int64_t qitems_cap = 0;
   pgc::FItem* d_fitems = nullptr;
   int64_t fitems_cap = 0;
+  pgc::EItem* d_eitems = nullptr;  // batched energies
+  int64_t eitems_cap = 0;
   uint8_t* d_select = nullptr;  // select states of the batched median
   int64_t select_cap = 0;
 };
@@ -336,7 +339,7 @@ void pgc_context_destroy(pgc_context* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values,
-                c->d_qitems, c->d_fitems, c->d_select};
+                c->d_qitems, c->d_fitems, c->d_eitems, c->d_select};
   for (void* p : ps)
     if (p) cache_free(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1360,6 +1363,38 @@ static int grow(T** p, int64_t& cap, int64_t need) {
 }
 }  // extern "C++"
 
+int pgc_graphs_set_feat_diff(pgc_graph* const* graphs, const void* const* feat_diff, int64_t n, int dtype) {
+  if (n < 0 || (n > 0 && (!graphs || !feat_diff)) || (dtype != 0 && dtype != 1))
+    return fail(PGC_ERR_ARG, "invalid arguments");
+  if (n == 0) return PGC_OK;
+  pgc_context* ctx = graphs[0] ? graphs[0]->ctx : nullptr;
+  if (!ctx) return fail(PGC_ERR_ARG, "graph 0 is null");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<EItem> items((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    pgc_graph* g = graphs[i];
+    if (!g || g->ctx != ctx || !feat_diff[i])
+      return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + ": null, of another context, or no feat_diff");
+    const pgc_graph* b = g->base ? g->base : g;
+    if (b->conjugate)
+      return fail(PGC_ERR_UNSUPPORTED, "feat_diff updates are available for non-conjugate graphs only");
+    if (!b->d_vx) return fail(PGC_ERR_UNSUPPORTED, "the graph was not built by pgc_graph_generate*: no energy inputs");
+    items[i] = {b->d_vx, b->d_vy, b->d_rows, b->E, feat_diff[i], b->nVY, b->line_integral, b->length_mode, dtype,
+                g->d_energy};
+  }
+  int rc;
+  if ((rc = grow(&ctx->d_eitems, ctx->eitems_cap, n))) return rc;
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipMemcpyAsync(ctx->d_eitems, items.data(), n * sizeof(EItem), hipMemcpyHostToDevice, st));
+  // the host table must outlive the copy (the energies themselves stay asynchronous)
+  HIPCHK(hipStreamSynchronize(st));
+  const int64_t chunk = 65535;
+  for (int64_t a = 0; a < n; a += chunk)
+    HIPCHK(launch_energy_batch(ctx->d_eitems + a, (int)std::min<int64_t>(chunk, n - a), st));
+  for (int64_t i = 0; i < n; ++i) graphs[i]->cost_kind = 0;
+  return PGC_OK;
+}
+
 int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int32_t* max_depth, int64_t n,
                           uint32_t flags, int32_t* edge_ids, int32_t* rows, const int64_t* offsets,
                           pgc_result* res) {
@@ -1400,9 +1435,12 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
                         g->ncols > 1 ? g->d_time[c] : nullptr});
     if ((rc = ensure_scratch(g, K))) return rc;
     if ((rc = ensure_bnb(g))) return rc;
-    HIPCHK(hipMemsetAsync(g->js.pred, 0xff, g->js.predN * 4, st));
-    HIPCHK(hipMemsetAsync(g->js.neg_len, 0, 4, st));
+    // a batch job is one launch: the kernel starts it with an empty cycle and,
+    // for a predecessor table in LDS, with -1 predecessors (no global table to
+    // reset, load or write back)
+    if (!pred_fits_lds(g, K)) HIPCHK(hipMemsetAsync(g->js.pred, 0xff, g->js.predN * 4, st));
     jobs[i] = make_job(g, K, MODE_FULL_RATIO, flags);
+    jobs[i].flags |= FLAG_FRESH_JOB;
   }
   {
     // chunks of at most 65535 columns (the launch grid's y limit)

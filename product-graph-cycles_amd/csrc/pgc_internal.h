@@ -63,6 +63,7 @@ enum JobError : int32_t {
 enum : uint32_t {
   FLAG_PICK_LAST_VERTEX = 1u,  // candidate pick of the reference's GPU path (last reporting vertex)
   FLAG_MINCOST_CPU_RULES = 2u, // min-cost: double lower-bound keys
+  FLAG_FRESH_JOB = 1u << 16,   // the solve's only launch: the kernel clears neg_len and an LDS pred table
 };
 
 struct JobResult {
@@ -164,6 +165,18 @@ struct QItem {
 };
 
 // One slot class of a batched fp32 cost fill.
+// One graph of the batched energy kernel (k_benergy).
+struct EItem {
+  const double* vx;
+  const double* vy;
+  const int32_t* rows;
+  int64_t E;
+  const void* fd;  // [nVX x nVY] feat_diff, float64 or float32 (f32)
+  int64_t nVY;
+  int32_t line, lengthMode, f32;
+  double* energy;
+};
+
 struct FItem {
   const int32_t* eid;
   const int32_t* s;

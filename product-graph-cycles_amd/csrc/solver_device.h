@@ -418,10 +418,33 @@ struct Solver {
       } else if (p < npl) {
         wn = p;
         wl = 1;
-      } else {
+      } else if (p >= (L - 1) * npl && p < L * npl) {
         const int o = last[p - (L - 1) * npl];
         wn = o >= 0 ? o : -1;
         wl = o >= 0 ? L : L - (-1 - o);
+      } else {
+        // not a node of layer L-1 (no path sets such a predecessor): walk
+        // the chain as the chase does
+        int pp = p, pos = 0;
+        while (true) {
+          if (pp < npl) {
+            wn = pp;
+            wl = pos + 1;
+            break;
+          }
+          if (pos >= J.chain_stride - 1) {
+            wn = -2;
+            wl = pos;
+            break;
+          }
+          ++pos;
+          pp = pget(pp);
+          if (pp == -1) {
+            wn = -1;
+            wl = pos;
+            break;
+          }
+        }
       }
       WnL[v] = wn;
       WlL[v] = wl;
@@ -1142,11 +1165,16 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __res
     // the current call, and then follow its predecessor from an earlier one).
     int32_t* P = (int32_t*)(lds + (int64_t)(2 * J.K + 1) * (J.npl + 1));
     const bool predLds = MAXC != 0 && J.pred_lds;
+    const bool fresh = (J.flags & FLAG_FRESH_JOB) != 0;
     const int64_t nPred = (int64_t)(J.L + 1) * J.K * J.npl;
     if (predLds) {
-      for (int64_t i = threadIdx.x; i < nPred; i += PGC_BLOCK) ((PGC_LDS int32_t*)P)[i] = gp(J.pred)[i];
-      __syncthreads();
+      if (fresh)
+        for (int64_t i = threadIdx.x; i < nPred; i += PGC_BLOCK) ((PGC_LDS int32_t*)P)[i] = -1;
+      else
+        for (int64_t i = threadIdx.x; i < nPred; i += PGC_BLOCK) ((PGC_LDS int32_t*)P)[i] = gp(J.pred)[i];
     }
+    if (fresh && threadIdx.x == 0) *gp(J.neg_len) = 0;
+    __syncthreads();
     Solver<CT, HAS_TIME, MINCOST, MAXC, W> s(J, D, Dmin, sh, P);
     if (MINCOST) {
       s.min_cost();
@@ -1157,7 +1185,7 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __res
     } else {
       s.lawler();
     }
-    if (predLds) {
+    if (predLds && !fresh) {
       __syncthreads();
       for (int64_t i = threadIdx.x; i < nPred; i += PGC_BLOCK) gp(J.pred)[i] = ((PGC_LDS int32_t*)P)[i];
     }

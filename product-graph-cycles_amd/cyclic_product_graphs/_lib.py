@@ -49,8 +49,8 @@ EXPORTED = [
     "pgc_graph_generate_device",
     "pgc_graph_from_arrays", "pgc_graph_set_branch_graph", "pgc_graph_info", "pgc_graph_get_product_space",
     "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
-    "pgc_solve_ratio_batch", "pgc_graph_set_feat_diff", "pgc_graph_derive", "pgc_cycle_ratio_values",
-    "pgc_cycle_ratio_grad",
+    "pgc_solve_ratio_batch", "pgc_graph_set_feat_diff", "pgc_graphs_set_feat_diff", "pgc_graph_derive",
+    "pgc_cycle_ratio_values", "pgc_cycle_ratio_grad",
 ]
 
 _lib = None
@@ -86,6 +86,7 @@ def load():
     L.pgc_solve_min_cost.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
     L.pgc_solve_ratio_batch.argtypes = [P, P, P, I64, C.c_uint32, P, P, P, P]
     L.pgc_graph_set_feat_diff.argtypes = [P, P, C.c_int]
+    L.pgc_graphs_set_feat_diff.argtypes = [P, P, I64, C.c_int]
     L.pgc_graph_derive.argtypes = [P, P, C.c_int, C.POINTER(P)]
     L.pgc_cycle_ratio_values.argtypes = [P, P, I64, P, P, P, P, P]
     L.pgc_cycle_ratio_grad.argtypes = [P, P, I64, P, P, P, P, P]
@@ -320,24 +321,41 @@ def _cycles(cycles):
     return np.ascontiguousarray(cat, np.int32), off, lengths
 
 
+def _dptr(x):
+    return x if isinstance(x, int) else x.data_ptr()
+
+
+def set_feat_diff_batch(graphs, fds, dtype=0):
+    """pgc_graphs_set_feat_diff: new energies of graphs from feat_diffs fds
+    (contiguous device tensors or device addresses; float64 when dtype is 0,
+    float32 when 1), in one launch."""
+    n = len(graphs)
+    if n == 0:
+        return
+    handles = (C.c_void_p * n)(*[g._h.value for g in graphs])
+    fdp = (C.c_void_p * n)(*[_dptr(f) for f in fds])
+    check(load().pgc_graphs_set_feat_diff(handles, fdp, n, int(dtype)))
+
+
 def cycle_ratio_values(ctx, graphs, fds, cycles):
     """Raw-unit ratios sum c / sum t of cycles (edge-id arrays) of graphs, for
-    feat_diffs fds (contiguous float64 device tensors)."""
+    feat_diffs fds (contiguous float64 device tensors or device addresses)."""
     n = len(graphs)
     cat, off, lengths = _cycles(cycles)
     handles = (C.c_void_p * n)(*[g._h.value for g in graphs])
-    fdp = (C.c_void_p * n)(*[f.data_ptr() for f in fds])
+    fdp = (C.c_void_p * n)(*[_dptr(f) for f in fds])
     vals = np.zeros(n, np.float64)
     check(load().pgc_cycle_ratio_values(ctx.handle, handles, n, fdp, ptr(cat), ptr(off), ptr(lengths), ptr(vals)))
     return vals
 
 
 def cycle_ratio_grad(ctx, graphs, cycles, grad_out, dfds):
-    """dfds[i] += grad_out[i] * d ratio_i / d feat_diff (float64 device tensors)."""
+    """dfds[i] += grad_out[i] * d ratio_i / d feat_diff (float64 device tensors
+    or device addresses)."""
     n = len(graphs)
     cat, off, lengths = _cycles(cycles)
     handles = (C.c_void_p * n)(*[g._h.value for g in graphs])
-    dp = (C.c_void_p * n)(*[d.data_ptr() for d in dfds])
+    dp = (C.c_void_p * n)(*[_dptr(d) for d in dfds])
     go = np.ascontiguousarray(grad_out, np.float64)
     check(load().pgc_cycle_ratio_grad(ctx.handle, handles, n, ptr(cat), ptr(off), ptr(lengths), ptr(go), dp))
 

@@ -72,22 +72,40 @@ class _CycleRatio(torch.autograd.Function):
     @staticmethod
     def forward(ctx, problem, pair_index, *fds):
         n = len(fds)
-        fd64 = [f.detach().to(problem.device, torch.float64).contiguous() for f in fds]
-        for i, f in enumerate(fd64):
+        for i, f in enumerate(fds):
             if tuple(f.shape) != problem.shapes[pair_index[i]]:
                 raise ValueError("feat_diff %d has shape %s, pair %d needs %s"
                                  % (i, tuple(f.shape), pair_index[i], problem.shapes[pair_index[i]]))
-        torch.cuda.current_stream(problem.device).synchronize()  # fd64 is read on libpgc's stream
+        # all feat_diffs in one float64 buffer (one copy and one conversion for
+        # the whole call), item i at offset off[i]
+        sizes = [int(f.numel()) for f in fds]
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(sizes, out=off[1:])
+        flat = torch.cat([f.detach().reshape(-1) for f in fds]).to(problem.device, torch.float64)
+        base = flat.data_ptr()
+        ptrs = [base + 8 * int(off[i]) for i in range(n)]
+        torch.cuda.current_stream(problem.device).synchronize()  # flat is read on libpgc's stream
         # one graph per (position, pair): items of one call never share a graph
-        graphs = [problem.item_graph(i, pair_index[i], fd64[i]) for i in range(n)]
+        graphs, update = [], []
+        for i in range(n):
+            key = (i, pair_index[i])
+            g = problem._items.get(key)
+            if g is None:
+                g = problem.bases[pair_index[i]].derive(flat[off[i]:off[i + 1]].view(problem.shapes[pair_index[i]]))
+                problem._items[key] = g
+            else:
+                update.append(i)
+            graphs.append(g)
+        _lib.set_feat_diff_batch([graphs[i] for i in update], [ptrs[i] for i in update], 0)
         Ks = [problem.K[pair_index[i]] for i in range(n)]
         out = _lib.solve_ratio_batch(problem.ctx, graphs, Ks, problem.flags, with_rows=False)
         cycles = [o[0] for o in out]
-        vals = _lib.cycle_ratio_values(problem.ctx, graphs, fd64, cycles)
+        vals = _lib.cycle_ratio_values(problem.ctx, graphs, ptrs, cycles)
         ctx.problem = problem
         ctx.pair_index = list(pair_index)
         ctx.graphs = graphs
         ctx.cycles = cycles
+        ctx.off = off
         ctx.dtypes = [f.dtype for f in fds]
         problem.last_cycles = cycles
         problem.last_results = [o[2] for o in out]
@@ -97,10 +115,20 @@ class _CycleRatio(torch.autograd.Function):
     def backward(ctx, grad):
         problem = ctx.problem
         go = grad.detach().to(torch.float64).cpu().numpy()
-        dfds = [torch.zeros(problem.shapes[p], dtype=torch.float64, device=problem.device) for p in ctx.pair_index]
+        off = ctx.off
+        n = len(ctx.graphs)
+        # all gradients in one zeroed float64 buffer, converted in one step
+        flat = torch.zeros(int(off[-1]), dtype=torch.float64, device=problem.device)
+        base = flat.data_ptr()
         torch.cuda.current_stream(problem.device).synchronize()
-        _lib.cycle_ratio_grad(problem.ctx, ctx.graphs, ctx.cycles, go, dfds)
-        return (None, None) + tuple(d.to(dt) for d, dt in zip(dfds, ctx.dtypes))
+        _lib.cycle_ratio_grad(problem.ctx, ctx.graphs, ctx.cycles, go, [base + 8 * int(off[i]) for i in range(n)])
+        shapes = [problem.shapes[p] for p in ctx.pair_index]
+        if len(set(ctx.dtypes)) == 1:
+            conv = flat.to(ctx.dtypes[0])
+            outs = [conv[off[i]:off[i + 1]].view(shapes[i]) for i in range(n)]
+        else:
+            outs = [flat[off[i]:off[i + 1]].view(shapes[i]).to(ctx.dtypes[i]) for i in range(n)]
+        return (None, None) + tuple(outs)
 
 
 def cycle_ratio(problem: RatioProblem, fds, pair_index=None):
