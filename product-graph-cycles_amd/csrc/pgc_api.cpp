@@ -208,6 +208,10 @@ struct pgc_context {
   int64_t fitems_cap = 0;
   pgc::EItem* d_eitems = nullptr;  // batched energies
   int64_t eitems_cap = 0;
+  // launch groups of a batch run concurrently on these streams (default priority)
+  hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   uint8_t* d_select = nullptr;  // select states of the batched median
   int64_t select_cap = 0;
 };
@@ -342,6 +346,14 @@ void pgc_context_destroy(pgc_context* c) {
                 c->d_qitems, c->d_fitems, c->d_eitems, c->d_select};
   for (void* p : ps)
     if (p) cache_free(p);
+  for (int k = 0; k < 4; ++k) {
+    if (c->aux[k]) {
+      (void)hipStreamSynchronize(c->aux[k]);
+      (void)hipStreamDestroy(c->aux[k]);
+    }
+    if (c->join[k]) (void)hipEventDestroy(c->join[k]);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1507,11 +1519,33 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   HIPCHK(hipMemcpyAsync(ctx->d_jobs, ordered.data(), n * sizeof(SolverJob), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(ctx->d_queue, 0, 64 * sizeof(int32_t), st));
   HIPCHK(hipEventRecord(ctx->ev0, st));
-  for (size_t li = 0; li < launches.size(); ++li) {
-    const Launch& L = launches[li];
+  if (launches.size() == 1) {
+    const Launch& L = launches[0];
     const int grid = (int)std::min<int64_t>(L.count, std::max(1, ctx->num_cu));
-    HIPCHK(launch_solver(ctx->d_jobs + L.first, (int)L.count, grid, ctx->d_queue + li, L.k.maxc, L.k.maxw, false,
+    HIPCHK(launch_solver(ctx->d_jobs + L.first, (int)L.count, grid, ctx->d_queue, L.k.maxc, L.k.maxw, false,
                          L.k.time, false, L.lds, st));
+  } else {
+    // the groups run concurrently: a group's persistent workgroups hold CUs
+    // only while it has jobs, and the CUs a group leaves (while its longest
+    // jobs finish) take the other groups' workgroups
+    for (int k = 0; k < 4; ++k) {
+      if (!ctx->aux[k]) HIPCHK(hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
+      if (!ctx->join[k]) HIPCHK(hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming));
+    }
+    if (!ctx->fork) HIPCHK(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ctx->fork, st));
+    const int ns = (int)std::min<size_t>(4, launches.size());
+    for (int k = 0; k < ns; ++k) HIPCHK(hipStreamWaitEvent(ctx->aux[k], ctx->fork, 0));
+    for (size_t li = 0; li < launches.size(); ++li) {
+      const Launch& L = launches[li];
+      const int grid = (int)std::min<int64_t>(L.count, std::max(1, ctx->num_cu));
+      HIPCHK(launch_solver(ctx->d_jobs + L.first, (int)L.count, grid, ctx->d_queue + li, L.k.maxc, L.k.maxw, false,
+                           L.k.time, false, L.lds, ctx->aux[li % 4]));
+    }
+    for (int k = 0; k < ns; ++k) {
+      HIPCHK(hipEventRecord(ctx->join[k], ctx->aux[k]));
+      HIPCHK(hipStreamWaitEvent(st, ctx->join[k], 0));
+    }
   }
   HIPCHK(hipEventRecord(ctx->ev1, st));
   std::vector<JobResult> out(n);
