@@ -37,6 +37,11 @@ typedef struct pgc_graph pgc_graph;
                                     without it: lowest (D, vertex) (cost_time_ratio_solver.cpp:430-437) */
 #define PGC_MINCOST_CPU_RULES 2u /* dijkstracpu: double lower-bound keys, branching seed
                                     cycle.front()-numNodes (dijkstra_solver.cpp:318-381) */
+#define PGC_CORRECTED_ODD_L 4u   /* odd layer count with max_depth >= 2: fold (ratio) and
+                                    lower bounds (min-cost) read the closing layer's depth-0
+                                    row, and its depth rows 1..K-1 become layer 0's, as they
+                                    do for even L. Without it: the reference's row L % 2
+                                    (cost_time_ratio_solver.cpp:400, dijkstra_solver.cpp:318) */
 
 typedef struct pgc_result {
   int32_t passes_layers_once; /* the reference's passes_layers_just_once */

@@ -224,6 +224,7 @@ struct CtrSolver {
   std::vector<int> negCycle, hits;
   double lbCurrentBranch = 0;
   bool gpu_pick = false;
+  bool corrected = false;  // fold the closing layer's depth-0 row for odd L (Q1 fixed)
   Stats st;
 
   struct Ratio {
@@ -296,8 +297,15 @@ struct CtrSolver {
         return w < 0 ? FOUND_NEGATIVE_CYCLE : NO_NEGATIVE_CYCLE;
       }
 
-      // fold of the dummy layer into layer 0 (:399-417)
-      const long lastRow = L % 2;
+      // fold of the dummy layer into layer 0 (:399-417). The reference reads
+      // row L % 2, which is the closing layer's depth-0 row only for even L
+      // or K == 1 (Q1); `corrected` reads that row, (L % 2) * K. With even L
+      // the closing layer's rows are layer 0's rows, so its depths 1..K-1
+      // become layer 0's; `corrected` copies them for odd L likewise.
+      const long lastRow = corrected ? (L % 2) * K : L % 2;
+      if (corrected && L % 2 == 1)
+        for (long k = 1; k < K; ++k)
+          for (long vi = 0; vi < npl; ++vi) S.d(k, vi) = S.d(K + k, vi);
       bool any = false;
       for (long vi = 0; vi < npl; ++vi) {
         if (branch[vi] != branchId) continue;
@@ -516,6 +524,7 @@ struct MinCostSolver {
   int bSrc = -1, bTrgt = -1;
   std::vector<std::pair<double, long>> lbs;
   bool gpu_rules = true;
+  bool corrected = false;  // lower bounds from the closing layer's depth-0 row for odd L (Q1 fixed)
   // branch graph: CSR of values (edge ids, or vertex ids used as edge ids)
   std::vector<long> bg_off, bg_val;
   Stats st;
@@ -539,7 +548,7 @@ struct MinCostSolver {
     std::copy(S.D.begin(), S.D.begin() + npl, S.D.begin() + 2 * K * npl);
     st.sweeps++;
     if (!S.sweep_layers()) return false;
-    const long lastRow = L % 2;
+    const long lastRow = corrected ? (L % 2) * K : L % 2;  // Q1, as in the ratio fold
     lbs.clear();
     bool anyFinite = false;
     for (long vi = 0; vi < npl; ++vi) {
@@ -825,8 +834,9 @@ void orc_quantize(double* cost, long E, int ncols) {
 
 // Lawler ratio solve. cost is column-major E x ncols, quantized in place.
 // flags bit0: candidate pick rule of the GPU reference (last reporting
-// vertex) instead of the CPU order. Outputs the cycle's edge ids in walk
-// order. stats: [bf_calls, sweeps, lawler_iters, branches, relaxations].
+// vertex) instead of the CPU order; bit1: corrected odd-L fold (Q1). Outputs
+// the cycle's edge ids in walk order. stats: [bf_calls, sweeps, lawler_iters,
+// branches, relaxations].
 int orc_ctr_solve(const int32_t* SRC, const int32_t* TRGT, long E, double* cost, int ncols, long npl, int K,
                   int flags, int32_t* out_edges, long cap, long* out_len, float* out_obj, int* passes,
                   long* stats) {
@@ -843,6 +853,7 @@ int orc_ctr_solve(const int32_t* SRC, const int32_t* TRGT, long E, double* cost,
     s.S.t = s.time.data();
     s.tol = (double)(float)1e-8;
     s.gpu_pick = (flags & 1) != 0;
+    s.corrected = (flags & 2) != 0;
     s.branch.assign(npl, 0);
     std::vector<long> edges;
     float obj = 0;
@@ -877,7 +888,8 @@ int orc_ctr_solve(const int32_t* SRC, const int32_t* TRGT, long E, double* cost,
 // Minimum-cost solve. cost column 0 is used raw. Branch graph given as CSR
 // over layer-0 nodes (bg_off[npl+1], bg_val); pass bg_off == nullptr to use
 // the in-edges of layer-0 nodes (the reference's default). flags bit0: CPU
-// rules (float-free keys, negative branching seed) instead of the GPU ones.
+// rules (float-free keys, negative branching seed) instead of the GPU ones;
+// bit1: corrected odd-L lower bounds (Q1).
 int orc_dijkstra_solve(const int32_t* SRC, const int32_t* TRGT, long E, const double* cost, long npl, int K,
                        const long* bg_off, const long* bg_val, int flags, int32_t* out_edges, long cap,
                        long* out_len, float* out_obj, int* passes, long* stats) {
@@ -887,6 +899,7 @@ int orc_dijkstra_solve(const int32_t* SRC, const int32_t* TRGT, long E, const do
     s.S.c = cost;
     s.S.t = nullptr;
     s.gpu_rules = (flags & 1) == 0;
+    s.corrected = (flags & 2) != 0;
     s.branch.assign(npl, 0);
     if (bg_off) {
       s.bg_off.assign(bg_off, bg_off + npl + 1);

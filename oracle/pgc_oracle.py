@@ -104,8 +104,9 @@ def quantize(cost):
     return cm.T.copy()
 
 
-def ctr_solve(src, trgt, cost, npl, K, gpu_pick=False):
-    """Lawler ratio solve. Returns (edge_ids, objective float32, passes, stats, quantized cost)."""
+def ctr_solve(src, trgt, cost, npl, K, gpu_pick=False, corrected=False):
+    """Lawler ratio solve. Returns (edge_ids, objective float32, passes, stats, quantized cost).
+    corrected: fold the closing layer's depth-0 row for odd L (Q1 fixed)."""
     src = np.ascontiguousarray(src, np.int32)
     trgt = np.ascontiguousarray(trgt, np.int32)
     cost = np.asarray(cost, np.float64)
@@ -120,15 +121,17 @@ def ctr_solve(src, trgt, cost, npl, K, gpu_pick=False):
     obj = C.c_float(0)
     passes = C.c_int(0)
     stats = np.zeros(5, np.int64)
-    rc = lib().orc_ctr_solve(_p(src), _p(trgt), E, _p(cm), cm.shape[0], npl, K, int(gpu_pick), _p(out), cap,
+    flags = int(gpu_pick) | (2 if corrected else 0)
+    rc = lib().orc_ctr_solve(_p(src), _p(trgt), E, _p(cm), cm.shape[0], npl, K, flags, _p(out), cap,
                              C.byref(n), C.byref(obj), C.byref(passes), _p(stats))
     if rc != 0:
         raise RuntimeError("oracle ctr solve failed: %s" % lib().orc_last_error().decode())
     return out[: n.value].copy(), np.float32(obj.value), bool(passes.value), stats, cm.T.copy()
 
 
-def dijkstra_solve(src, trgt, cost0, npl, K, branch_graph=None, cpu_rules=False):
-    """Minimum-cost solve on raw column-0 costs. branch_graph: list of value lists or None."""
+def dijkstra_solve(src, trgt, cost0, npl, K, branch_graph=None, cpu_rules=False, corrected=False):
+    """Minimum-cost solve on raw column-0 costs. branch_graph: list of value lists or None.
+    corrected: lower bounds from the closing layer's depth-0 row for odd L (Q1 fixed)."""
     src = np.ascontiguousarray(src, np.int32)
     trgt = np.ascontiguousarray(trgt, np.int32)
     cost0 = np.ascontiguousarray(np.asarray(cost0, np.float64).reshape(-1))
@@ -151,7 +154,8 @@ def dijkstra_solve(src, trgt, cost0, npl, K, branch_graph=None, cpu_rules=False)
         bo, bv = _p(off), _p(val)
     else:
         bo = bv = None
-    rc = lib().orc_dijkstra_solve(_p(src), _p(trgt), E, _p(cost0), npl, K, bo, bv, int(cpu_rules), _p(out), cap,
+    flags = int(cpu_rules) | (2 if corrected else 0)
+    rc = lib().orc_dijkstra_solve(_p(src), _p(trgt), E, _p(cost0), npl, K, bo, bv, flags, _p(out), cap,
                                   C.byref(n), C.byref(obj), C.byref(passes), _p(stats))
     if rc != 0:
         raise RuntimeError("oracle dijkstra solve failed: %s" % lib().orc_last_error().decode())

@@ -63,6 +63,7 @@ enum JobError : int32_t {
 enum : uint32_t {
   FLAG_PICK_LAST_VERTEX = 1u,  // candidate pick of the reference's GPU path (last reporting vertex)
   FLAG_MINCOST_CPU_RULES = 2u, // min-cost: double lower-bound keys
+  FLAG_CORRECTED_ODD_L = 4u,   // odd L, K >= 2: the closing layer's depth-0 row (not row L % 2)
   FLAG_FRESH_JOB = 1u << 16,   // the solve's only launch: the kernel clears neg_len and an LDS pred table
 };
 

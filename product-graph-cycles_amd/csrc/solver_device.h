@@ -337,8 +337,24 @@ struct Solver {
   }
 
   // Fold of the closing layer into layer 0; returns whether any value improved.
+  // The reference reads row L % 2: the closing layer's depth-0 row only for
+  // even L or K == 1 (for odd L it is row 1 of layer L-1). The corrected
+  // semantics read row (L % 2) * K and, for odd L, make the closing layer's
+  // depth rows 1..K-1 layer 0's, as the shared rows do for even L.
+  __device__ __forceinline__ int closing_row() const {
+    return (J.flags & FLAG_CORRECTED_ODD_L) != 0 ? (L % 2) * K : L % 2;
+  }
+
   __device__ __forceinline__ bool fold(int bid, int onlyV) {
-    const int lastRow = L % 2;
+    const int lastRow = closing_row();
+    if ((J.flags & FLAG_CORRECTED_ODD_L) != 0 && (L % 2) == 1) {
+#pragma unroll
+      for (int i = 0; i < np(); ++i) {
+        const int v = vert(i);
+        if (v < 0) continue;
+        for (int k = 1; k < K; ++k) D[k * ds + v] = D[(K + k) * ds + v];
+      }
+    }
     bool imp = false;
 #pragma unroll
     for (int i = 0; i < np(); ++i) {
@@ -1074,7 +1090,7 @@ struct Solver {
     }
     const uint64_t tc0 = wclock();
     if (status == ST_CAN_CONTAIN) {
-      const int lastRow = L % 2;
+      const int lastRow = closing_row();
       const bool cpuRules = (J.flags & FLAG_MINCOST_CPU_RULES) != 0;
       const int stride = J.chain_stride;
       const int64_t maxLen = (int64_t)L * K;

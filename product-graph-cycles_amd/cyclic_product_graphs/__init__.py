@@ -146,18 +146,24 @@ class product_graph_generator:
         return False, -1.0, np.zeros((4, 1), np.int32)
 
     # ------------------------------------------------------------- solvers
-    def solve_with_cost_time_ratio(self, solvername, max_depth=None):
+    def solve_with_cost_time_ratio(self, solvername, max_depth=None, corrected=False):
+        """corrected (an addition to the reference's signature): the odd-L fold
+        reads the closing layer's depth-0 row (PGC_CORRECTED_ODD_L); the
+        default keeps the reference's row L % 2."""
         if solvername == "export":
             return self._export()
         if solvername not in _LAWLER:
             raise ValueError("Solver >>%s<< is not implemented" % solvername)
         g = self._ensure()
         K = self.max_depth if max_depth is None else int(max_depth)
-        _, rows, res = g.solve_ratio(K, _LAWLER[solvername])
+        flags = _LAWLER[solvername] | (_lib.PGC_CORRECTED_ODD_L if corrected else 0)
+        _, rows, res = g.solve_ratio(K, flags)
         self.last_result = res.as_dict()
         return bool(res.passes_layers_once), float(np.float32(res.objective)), rows
 
-    def solve_with_minimum_cost(self, solvername, max_depth=None):
+    def solve_with_minimum_cost(self, solvername, max_depth=None, corrected=False):
+        """corrected: odd-L lower bounds from the closing layer's depth-0 row
+        (PGC_CORRECTED_ODD_L), an addition to the reference's signature."""
         if solvername == "export":
             return self._export()
         if solvername not in _MINCOST:
@@ -165,6 +171,8 @@ class product_graph_generator:
             flags = _MINCOST["dijkstracpu"]
         else:
             flags = _MINCOST[solvername]
+        if corrected:
+            flags |= _lib.PGC_CORRECTED_ODD_L
         g = self._ensure()
         K = self.max_depth if max_depth is None else int(max_depth)
         _, rows, res = g.solve_min_cost(K, flags)

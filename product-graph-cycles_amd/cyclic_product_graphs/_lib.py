@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libpgc.so")
 
 PGC_LAWLER_PICK_LAST = 1
 PGC_MINCOST_CPU_RULES = 2
+PGC_CORRECTED_ODD_L = 4
 
 
 class PgcError(RuntimeError):

@@ -35,12 +35,13 @@ def _oracle_graph(p, reg=False, cost="vanilla", time="", prune=False):
     return rows, src, trgt, en
 
 
-def _check_ratio(p, K, name, reg=False, cost="vanilla", time="", prune=False):
-    print("case ratio", name, "nx", len(p["VX"]), "ny", len(p["VY"]), "K", K, flush=True)
+def _check_ratio(p, K, name, reg=False, cost="vanilla", time="", prune=False, corrected=False):
+    print("case ratio", name, "nx", len(p["VX"]), "ny", len(p["VY"]), "K", K, "corrected", corrected, flush=True)
     g = _gen(p, reg, cost, time, prune)
-    passes, obj, rows_gpu = g.solve_with_cost_time_ratio(name, K)
+    passes, obj, rows_gpu = g.solve_with_cost_time_ratio(name, K, corrected=corrected)
     rows, src, trgt, en = _oracle_graph(p, reg, cost, time, prune)
-    e, oobj, opasses, stats, q = O.ctr_solve(src, trgt, en, len(p["VX"]), K, gpu_pick=LAWLER[name])
+    e, oobj, opasses, stats, q = O.ctr_solve(src, trgt, en, len(p["VX"]), K, gpu_pick=LAWLER[name],
+                                             corrected=corrected)
     assert passes == opasses
     assert np.array_equal(rows_gpu, rows[e]), "matching differs"
     assert abs(obj - float(oobj)) <= OBJ_RTOL * abs(float(oobj))
@@ -51,8 +52,8 @@ def _check_ratio(p, K, name, reg=False, cost="vanilla", time="", prune=False):
     return g
 
 
-def _check_mincost(p, K, name, prune=False):
-    print("case mincost", name, "nx", len(p["VX"]), "ny", len(p["VY"]), "K", K, flush=True)
+def _check_mincost(p, K, name, prune=False, corrected=False):
+    print("case mincost", name, "nx", len(p["VX"]), "ny", len(p["VY"]), "K", K, "corrected", corrected, flush=True)
     g = _gen(p, prune=prune)
     rows, src, trgt, en = _oracle_graph(p, prune=prune)
     bg = None
@@ -62,13 +63,13 @@ def _check_mincost(p, K, name, prune=False):
             bg[a].append(b)
     try:
         e, oobj, opasses, stats = O.dijkstra_solve(src, trgt, en[:, 0], len(p["VX"]), K, branch_graph=bg,
-                                                   cpu_rules=MINCOST[name])
+                                                   cpu_rules=MINCOST[name], corrected=corrected)
     except RuntimeError as ex:
         assert "no nodes in branch" in str(ex)
         with pytest.raises(cpg.PgcError):
-            g.solve_with_minimum_cost(name, K)
+            g.solve_with_minimum_cost(name, K, corrected=corrected)
         return
-    passes, obj, rows_gpu = g.solve_with_minimum_cost(name, K)
+    passes, obj, rows_gpu = g.solve_with_minimum_cost(name, K, corrected=corrected)
     assert passes == opasses
     assert np.array_equal(rows_gpu, rows[e]), "matching differs"
     assert obj == float(oobj)
@@ -91,6 +92,30 @@ def test_ratio_tiny(seed, nx, ny, ring, K, name):
 @pytest.mark.parametrize("name", list(MINCOST))
 def test_mincost_tiny(seed, nx, ny, ring, K, name):
     _check_mincost(S.tiny_pair(nx, ny, seed=seed, ring=ring), K, name)
+
+
+# odd layer counts (ring pairs with odd |VY|; open pairs with even |VY|) and
+# K >= 2: the only cases where the corrected fold differs from the reference's
+ODD_L = [(s, nx, ny, ring, K) for s, (nx, ny, ring) in enumerate([(4, 5, True), (5, 5, True), (3, 7, True),
+                                                                  (4, 6, False), (6, 3, True)])
+         for K in (2, 3)]
+
+
+@pytest.mark.parametrize("seed,nx,ny,ring,K", ODD_L)
+@pytest.mark.parametrize("name", list(LAWLER))
+def test_ratio_corrected_odd_L(seed, nx, ny, ring, K, name):
+    _check_ratio(S.tiny_pair(nx, ny, seed=seed + 40, ring=ring), K, name, corrected=True)
+
+
+@pytest.mark.parametrize("seed,nx,ny,ring,K", ODD_L)
+@pytest.mark.parametrize("name", list(MINCOST))
+def test_mincost_corrected_odd_L(seed, nx, ny, ring, K, name):
+    _check_mincost(S.tiny_pair(nx, ny, seed=seed + 40, ring=ring), K, name, corrected=True)
+
+
+@pytest.mark.parametrize("sub,ny,K,seed", [(2, 33, 2, 1), (2, 41, 3, 4)])
+def test_ratio_corrected_meshes(sub, ny, K, seed):
+    _check_ratio(S.make_pair(sub, ny, seed=seed), K, "lawlergpu", corrected=True)
 
 
 @pytest.mark.parametrize("sub,ny,K,seed", [(1, 8, 2, 0), (2, 33, 2, 1), (2, 33, 3, 2), (2, 20, 1, 3),
