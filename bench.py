@@ -22,7 +22,10 @@ lawlergpu solves, each step one forward (energy from the perturbed feat_diff,
 quantization, solve, lambda*) and backward (dL/dfeat_diff) of all 2304 items
 through cyclic_product_graphs.autograd; against the plain per-item loop
 over the same 2304 items (energy + one pgc_solve_ratio call per item), with
-the cycles of both compared item by item.
+the cycles of both compared item by item. Its "corrected" object times the
+same steps with the corrected odd-layer fold (PGC_CORRECTED_ODD_L) and counts
+the odd-layer items whose cycle and lambda* change; the reference's fold is
+the one timed everywhere else.
 """
 import argparse
 import json
@@ -113,13 +116,16 @@ def batched(args):
     torch.cuda.synchronize()
     b0 = time.perf_counter()
     for _ in range(args.batch_steps):
-        step()
+        lam = step()
     b_el = time.perf_counter() - b0
     res = prob.last_results
-    cycles = prob.last_cycles
+    cycles = [c.copy() for c in prob.last_cycles]
     kms = res[0].kernel_ms
     relax = sum(r.relaxations for r in res)
     sweeps = sum(r.sweeps for r in res)
+    corrected = None
+    if not args.no_corrected:
+        corrected = batched_corrected(args, prob, step, cycles, lam.detach().cpu().numpy(), pairs, owner)
 
     # plain loop over the same items: energy from the item's feat_diff, then
     # one pgc_solve_ratio call per item, on one graph per pair
@@ -150,7 +156,46 @@ def batched(args):
         "batched_vs_plain": (n * args.batch_steps / b_el) / (n / p_el),
         "same_cycles_as_plain_loop": same,
         "setup_s": setup_s,
+        "corrected": corrected,
     }
+
+
+def batched_corrected(args, prob, step, cycles, lam, pairs, owner):
+    """The same items with the corrected odd-layer fold (PGC_CORRECTED_ODD_L):
+    time per forward+backward step, and, over the items with an odd layer
+    count and max_depth >= 2 (the only ones whose fold differs), how many
+    cycles and lambda* change and by how much lambda* drops on average."""
+    import numpy as np
+    import torch
+    from cyclic_product_graphs import _lib
+
+    prob.flags |= _lib.PGC_CORRECTED_ODD_L
+    try:
+        step()
+        torch.cuda.synchronize()
+        c0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            lam_c = step()
+        c_el = time.perf_counter() - c0
+        lam_c = lam_c.detach().cpu().numpy()
+        odd = [i for i in range(len(owner))
+               if len(pairs[owner[i]][0]["EY"]) % 2 == 1 and pairs[owner[i]][1] >= 2]
+        changed = [i for i in odd if not np.array_equal(prob.last_cycles[i], cycles[i])]
+        odd_set = set(odd)
+        even_same = all(np.array_equal(prob.last_cycles[i], cycles[i])
+                        for i in range(len(owner)) if i not in odd_set)
+        rel = [(lam_c[i] - lam[i]) / abs(lam[i]) for i in changed if lam[i] != 0]
+        return {
+            "ms_per_step": c_el / args.batch_steps * 1e3,
+            "odd_L_items": len(odd),
+            "odd_L_cycles_changed": len(changed),
+            "odd_L_lambda_lower": int(sum(1 for i in changed if lam_c[i] < lam[i])),
+            "odd_L_lambda_higher": int(sum(1 for i in changed if lam_c[i] > lam[i])),
+            "mean_rel_lambda_change_of_changed": float(np.mean(rel)) if rel else 0.0,
+            "other_items_same_cycle": bool(even_same),
+        }
+    finally:
+        prob.flags &= ~_lib.PGC_CORRECTED_ODD_L
 
 
 def conjugate(steps=2):
@@ -196,6 +241,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched serving-scale workload")
+    ap.add_argument("--no-corrected", action="store_true",
+                    help="skip the batched workload's second pass with the corrected odd-layer fold")
     ap.add_argument("--batch-pairs", type=int, default=256)
     ap.add_argument("--batch-samples", type=int, default=8, help="perturbations per pair (plus the unperturbed one)")
     ap.add_argument("--batch-steps", type=int, default=2)

@@ -94,10 +94,10 @@ def test_mincost_tiny(seed, nx, ny, ring, K, name):
     _check_mincost(S.tiny_pair(nx, ny, seed=seed, ring=ring), K, name)
 
 
-# odd layer counts (ring pairs with odd |VY|; open pairs with even |VY|) and
-# K >= 2: the only cases where the corrected fold differs from the reference's
+# odd layer counts (the Y curve is closed, so L = |VY|) and K >= 2: the only
+# cases where the corrected fold differs from the reference's
 ODD_L = [(s, nx, ny, ring, K) for s, (nx, ny, ring) in enumerate([(4, 5, True), (5, 5, True), (3, 7, True),
-                                                                  (4, 6, False), (6, 3, True)])
+                                                                  (4, 7, False), (6, 3, True)])
          for K in (2, 3)]
 
 

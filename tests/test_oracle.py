@@ -179,7 +179,7 @@ def _ring(seed, nx, ny):
 
 
 @pytest.mark.parametrize("seed,nx,ny,ring,K",
-                         [c for c in CASES if (c[2] if c[3] else c[2] - 1) % 2 == 0 or c[4] == 1][:12])
+                         [c for c in CASES if c[2] % 2 == 0 or c[4] == 1][:12])
 def test_corrected_is_compat_for_even_L_or_K1(seed, nx, ny, ring, K):
     # the two semantics differ only for odd L with K >= 2
     p, rows, src, trgt, en = _tiny(seed, nx, ny, ring)
