@@ -233,6 +233,29 @@ def conjugate(steps=2):
             "distance_table": "global memory (46080 nodes per layer exceed the LDS)"}
 
 
+def cap_case(reps=2):
+    """The C4 item that sets the batch makespan: pair 111 of the uneven batch
+    (icosphere s=3, |VY| = 41, max_depth 1) with the feat_diff perturbation of
+    tests/golden/bf_cap_case.npz, whose Bellman-Ford calls run to the sweep cap
+    (26,363 sweeps). One lawlergpu solve per repetition on a fresh graph; prints
+    one JSON line per solve with the solver's phase times and whether the cycle
+    equals the fixture's."""
+    import numpy as np
+    from cyclic_product_graphs import _lib, synthetic as S
+    z = np.load(os.path.join(ROOT, "tests", "golden", "bf_cap_case.npz"))
+    p, K = S.make_uneven_pairs(256, seed=0)[int(z["pair_index"])]
+    ctx = _lib.default_context(0)
+    for _ in range(reps):
+        g = _lib.Graph.generate(ctx, p["VX"], p["EX"], p["VY"], p["EY"], z["feat_diff"].astype(np.float64), False,
+                                False, False, "vanilla", "")
+        e, _, res = g.solve_ratio(K)
+        d = res.as_dict()
+        print(json.dumps({"workload": "C4 sweep-cap item (pair 111, npl 642, L 41, K 1)", "kernel_ms": res.kernel_ms,
+                          "sweeps": res.sweeps, "phase_ms": d["phase_ms"],
+                          "same_cycle_as_fixture": bool(np.array_equal(e, z["edges"]))}), flush=True)
+        g.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,9 +272,14 @@ def main():
     ap.add_argument("--no-conj", action="store_true", help="skip the conjugate-graph measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--cap-case", action="store_true",
+                    help="run only the batch item that runs to the sweep cap (for profiling its solver launch)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.cap_case:
+        cap_case()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path in this build "

@@ -134,6 +134,32 @@ __device__ __forceinline__ int block_max(Shared& sh, int x) {
   return s;
 }
 
+// Ascending sort of N (key, value) pairs in registers by Batcher's odd-even
+// merge network (N a power of two; keys distinct, or equal keys whose order
+// does not matter).
+template <int N>
+__device__ __forceinline__ void sort_pairs(int (&k)[N], double (&x)[N]) {
+#pragma unroll
+  for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+    for (int q = p; q >= 1; q >>= 1)
+#pragma unroll
+      for (int j = q % p; j + q < N; j += 2 * q)
+#pragma unroll
+        for (int i = 0; i < q; ++i) {
+          const int a = i + j, b = i + j + q;
+          if (b < N && a / (2 * p) == b / (2 * p)) {
+            const bool sw = k[b] < k[a];
+            const int ka = k[a], kb = k[b];
+            const double xa = x[a], xb = x[b];
+            k[a] = sw ? kb : ka;
+            k[b] = sw ? ka : kb;
+            x[a] = sw ? xb : xa;
+            x[b] = sw ? xa : xb;
+          }
+        }
+}
+
 // lexicographic minimum of (key, v); v = INT_MAX means none
 __device__ __forceinline__ void block_min_key(Shared& sh, double& key, int& v) {
   for (int o = 32; o > 0; o >>= 1) {
@@ -551,6 +577,9 @@ struct Solver {
     cs = 0;
     ts = 0;
     ws = 0;
+    // unrolled so that the loads of eight pairs are in flight together; the
+    // additions stay in index order
+#pragma unroll 8
     for (int i = 0; i < m; ++i) {
       const double ci = c[i], ti = t[i];
       cs += ci;
@@ -638,15 +667,17 @@ struct Solver {
             gp(J.tmp_nodes)[cnt++] = orig(p);
             if (p == start) break;
           }
-          for (int i = 0, j = cnt - 1; i < j; ++i, --j) {
-            const int tmp = gp(J.tmp_nodes)[i];
-            gp(J.tmp_nodes)[i] = gp(J.tmp_nodes)[j];
-            gp(J.tmp_nodes)[j] = tmp;
-          }
           sh.i0 = cnt;
         }
         __syncthreads();
         const int cnt = sh.i0;
+        // reverse the path in place: each pair (i, cnt-1-i) belongs to one thread
+        for (int i = threadIdx.x; i < cnt / 2; i += PGC_BLOCK) {
+          const int a = gp(J.tmp_nodes)[i], b = gp(J.tmp_nodes)[cnt - 1 - i];
+          gp(J.tmp_nodes)[i] = b;
+          gp(J.tmp_nodes)[cnt - 1 - i] = a;
+        }
+        __syncthreads();
         if (!lookup_pairs(cnt)) err = ERR_EDGE_LOOKUP;
         copy_tmp_to_neg(cnt);
         if (threadIdx.x == 0) {
@@ -761,20 +792,31 @@ struct Solver {
     numWraps = 0;
     nHits = 0;
     bool entered = true, left = false;
-    for (int i = 0; i < n; ++i) {
-      const int v = gp(J.neg_nodes)[i];
-      if (left && v < npl) {
-        entered = true;
-        left = false;
-      }
-      if (entered) {
-        if (nHits < npl) gp(J.hits)[nHits] = v;
-        nHits++;
-        entered = false;
-      }
-      if (v >= npl && !left) {
-        numWraps++;
-        left = true;
+    const PGC_GLOBAL int32_t* nodes = gp(J.neg_nodes);
+    // eight nodes are loaded before they are scanned (hits is a different
+    // array, so its stores cannot change them): one load latency per eight
+    for (int i0 = 0; i0 < n; i0 += 8) {
+      int vv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) vv[k] = nodes[i0 + k < n ? i0 + k : i0];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int v = vv[k];
+        if (i0 + k < n) {
+          if (left && v < npl) {
+            entered = true;
+            left = false;
+          }
+          if (entered) {
+            if (nHits < npl) gp(J.hits)[nHits] = v;
+            nHits++;
+            entered = false;
+          }
+          if (v >= npl && !left) {
+            numWraps++;
+            left = true;
+          }
+        }
       }
     }
   }
@@ -854,8 +896,41 @@ struct Solver {
       const int node_layer = 1;
       float mean = 0.0f;
       int n = 0;
+      if constexpr (W == 8) {
+        // the 2 x 8 slots of the vertex's layer-1 copy, loaded once; a class's
+        // valid slots are a prefix (padding holds the sentinel npl), the
+        // others get key INT_MAX and sort last; then the mean in ascending
+        // edge id, as the scan below computes it
+        int es[16];
+        double cv[16];
+        const int ch = v >> 6, ln = v & 63;
+#pragma unroll
+        for (int cls = 0; cls < 2; ++cls) {
+          const auto& S = cls == 0 ? J.intra : J.inter;
+          const int base = node_layer * S.lstride + ch * 8 * 64 + ln;
+          bool ok = true;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int idx = base + j * 64;
+            const int sv = gp(S.src)[idx];
+            const int e = gp(S.eid)[idx];
+            const double c = (double)gp((const CT*)S.cost)[idx];
+            ok = ok && sv >= 0 && sv < npl;
+            es[cls * 8 + j] = ok ? e : 0x7fffffff;
+            cv[cls * 8 + j] = c;
+          }
+        }
+        sort_pairs<16>(es, cv);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          if (es[j] != 0x7fffffff) {
+            mean = (float)((double)mean + cv[j]);
+            n++;
+          }
+        }
+      }
       int lastE = -1;
-      while (true) {
+      while (W != 8) {
         int nextE = 0x7fffffff;
         double nextC = 0;
         for (int cls = 0; cls < 2; ++cls) {
@@ -891,17 +966,15 @@ struct Solver {
     startV = start;
     double ratio = 10.0;
     bellman_ford(ratio, true, true, 0, start, sweeps, err);
+    // The reference then doubles currentRatio until the cycle's reweighted
+    // cost is <= 0 (:690-695) and takes lambda_UB from the last
+    // computeRatioForCurrentCycle: cs / ts of the same cycle, which does not
+    // depend on the doubled ratio. The loop only decides when to stop, so
+    // lambda_UB is that quotient, computed once.
     if (threadIdx.x == 0) {
       const int n = *gp(J.neg_len);
       double cs, ts, ws;
       pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
-      double cycleCost = (double)(float)ws;
-      int guard = 0;
-      while ((cycleCost > 0 || isnan(cycleCost)) && guard++ < 4096) {
-        ratio += ratio;
-        pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
-        cycleCost = (double)(float)ws;
-      }
       sh.d1 = (double)(float)(cs / ts);
     }
     __syncthreads();
@@ -1057,6 +1130,7 @@ struct Solver {
       for (int i = threadIdx.x; i < m; i += PGC_BLOCK) gp(J.out_eid)[i] = se[i];
     if (threadIdx.x == 0) {
       double cs = 0, ts = 0;
+#pragma unroll 8
       for (int i = 0; i < m; ++i) {
         cs += sc[i];
         ts += stt[i];
