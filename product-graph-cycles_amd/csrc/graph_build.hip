@@ -496,6 +496,13 @@ __global__ void k_gather_rows(const int32_t* __restrict__ rows, int cols, const 
   out[i] = rows[(int64_t)eids[r] * cols + c];
 }
 
+// out[j] = base[j * stride], j < n (one column of a column-strided table)
+__global__ void k_gather_strided(const int32_t* __restrict__ base, int64_t stride, int64_t n,
+                                 int32_t* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = base[j * stride];
+}
+
 // layer-0 intralayer in-edges (branch graph of the min-cost branching)
 __global__ void k_int_max(const int32_t* __restrict__ x, int64_t n, int32_t* __restrict__ out) {
   int m = -2147483647 - 1;
@@ -620,6 +627,12 @@ hipError_t launch_gather_rows(const int32_t* rows, int cols, const int32_t* eids
                               hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_gather_rows, dim3(nb(n * cols, 256)), dim3(256), 0, st, rows, cols, eids, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_strided(const int32_t* base, int64_t stride, int64_t n, int32_t* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_strided, dim3(nb(n, 256)), dim3(256), 0, st, base, stride, n, out);
   return hipGetLastError();
 }
 

@@ -47,6 +47,7 @@ hipError_t launch_layout_fill(const int32_t*, const int32_t*, int64_t, int64_t, 
 hipError_t launch_fill_costs(const int32_t*, const int32_t*, int64_t, const double*, int64_t, int, bool, void*,
                              float*, hipStream_t);
 hipError_t launch_gather_rows(const int32_t*, int, const int32_t*, int64_t, int32_t*, hipStream_t);
+hipError_t launch_gather_strided(const int32_t*, int64_t, int64_t, int32_t*, hipStream_t);
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
 hipError_t launch_cycle_value(const CycleItem*, int, double*, hipStream_t);
@@ -1707,51 +1708,59 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   if ((rc = ensure_scratch(g, K))) return rc;
   JobScratch& s = g->js;
   HIPCHK(hipMemsetAsync(s.pred, 0xff, s.predN * 4, st));
-  // branch graph: the given one, else the in-edges of layer-0 nodes (ascending edge id)
-  if (!g->has_bg) {
-    const int64_t n0 = g->lstride[0];  // layer-0 block of the intralayer class
-    std::vector<int32_t> sl(n0), el(n0);
-    if (n0) {
-      HIPCHK(hipMemcpy(sl.data(), g->d_s[0], n0 * 4, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(el.data(), g->d_e[0], n0 * 4, hipMemcpyDeviceToHost));
-    }
-    g->bg_off.assign(npl + 1, 0);
-    g->bg_val.clear();
-    for (int64_t v = 0; v < npl; ++v) {
-      const int c = (int)(v >> 6), lane = (int)(v & 63);
-      const int w = g->width[0];
-      std::vector<int64_t> ids;
-      for (int j = 0; j < w; ++j) {
-        const int64_t idx = ((int64_t)c * w + j) * 64 + lane;
-        if (sl[idx] < 0 || sl[idx] >= npl) break;
-        ids.push_back(el[idx]);
+  // branch graph: the given one, else the in-edges of layer-0 nodes (ascending
+  // edge id), and the SRC ids of its values (the BFS reads SRCIds(value)).
+  // Only branch() reads them, so they are built at the first split.
+  std::vector<int32_t> bgSrc;
+  bool bgReady = false;
+  auto ensure_bg = [&]() -> int {
+    if (bgReady) return PGC_OK;
+    if (!g->has_bg) {
+      const int64_t n0 = g->lstride[0];  // layer-0 block of the intralayer class
+      std::vector<int32_t> sl(n0), el(n0);
+      if (n0) {
+        HIPCHK(hipMemcpy(sl.data(), g->d_s[0], n0 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(el.data(), g->d_e[0], n0 * 4, hipMemcpyDeviceToHost));
       }
-      std::sort(ids.begin(), ids.end());
-      g->bg_val.insert(g->bg_val.end(), ids.begin(), ids.end());
-      g->bg_off[v + 1] = (int64_t)g->bg_val.size();
+      g->bg_off.assign(npl + 1, 0);
+      g->bg_val.clear();
+      for (int64_t v = 0; v < npl; ++v) {
+        const int c = (int)(v >> 6), lane = (int)(v & 63);
+        const int w = g->width[0];
+        std::vector<int64_t> ids;
+        for (int j = 0; j < w; ++j) {
+          const int64_t idx = ((int64_t)c * w + j) * 64 + lane;
+          if (sl[idx] < 0 || sl[idx] >= npl) break;
+          ids.push_back(el[idx]);
+        }
+        std::sort(ids.begin(), ids.end());
+        g->bg_val.insert(g->bg_val.end(), ids.begin(), ids.end());
+        g->bg_off[v + 1] = (int64_t)g->bg_val.size();
+      }
+      g->has_bg = true;
+      g->bg_val.push_back(0);
+      g->bg_val.pop_back();
     }
-    g->has_bg = true;
-    g->bg_val.push_back(0);
-    g->bg_val.pop_back();
-  }
-  // SRC ids of branch-graph values (the BFS reads SRCIds(value))
-  std::vector<int32_t> bgSrc(g->bg_val.size(), -1);
-  if (!g->bg_val.empty()) {
-    std::vector<int32_t> ids(g->bg_val.size());
-    for (size_t i = 0; i < ids.size(); ++i)
-      ids[i] = (g->bg_val[i] >= 0 && g->bg_val[i] < g->E) ? (int32_t)g->bg_val[i] : 0;
-    int32_t *d_ids = nullptr, *d_out = nullptr;
-    HIPCHK(dalloc(&d_ids, ids.size()));
-    HIPCHK(dalloc(&d_out, ids.size()));
-    HIPCHK(hipMemcpyAsync(d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
-    HIPCHK(launch_gather_rows(g->d_src, 1, d_ids, (int64_t)ids.size(), d_out, st));
-    HIPCHK(hipMemcpyAsync(bgSrc.data(), d_out, ids.size() * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    cache_free(d_ids);
-    cache_free(d_out);
-    for (size_t i = 0; i < ids.size(); ++i)
-      if (!(g->bg_val[i] >= 0 && g->bg_val[i] < g->E)) bgSrc[i] = -1;
-  }
+    bgSrc.assign(g->bg_val.size(), -1);
+    if (!g->bg_val.empty()) {
+      std::vector<int32_t> ids(g->bg_val.size());
+      for (size_t i = 0; i < ids.size(); ++i)
+        ids[i] = (g->bg_val[i] >= 0 && g->bg_val[i] < g->E) ? (int32_t)g->bg_val[i] : 0;
+      int32_t *d_ids = nullptr, *d_out = nullptr;
+      HIPCHK(dalloc(&d_ids, ids.size()));
+      HIPCHK(dalloc(&d_out, ids.size()));
+      HIPCHK(hipMemcpyAsync(d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
+      HIPCHK(launch_gather_rows(g->d_src, 1, d_ids, (int64_t)ids.size(), d_out, st));
+      HIPCHK(hipMemcpyAsync(bgSrc.data(), d_out, ids.size() * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      cache_free(d_ids);
+      cache_free(d_out);
+      for (size_t i = 0; i < ids.size(); ++i)
+        if (!(g->bg_val[i] >= 0 && g->bg_val[i] < g->E)) bgSrc[i] = -1;
+    }
+    bgReady = true;
+    return PGC_OK;
+  };
 
   double kms = 0;
   int64_t sweeps = 0, bfc = 0, branches = 0;
@@ -1826,8 +1835,13 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
         if (newUB < upperBound) {
           upperBound = newUB;
           cycle.resize(clen[v]);
-          if (clen[v])
-            HIPCHK(hipMemcpy2D(cycle.data(), 4, s.chain + v, (size_t)npl * 4, 4, clen[v], hipMemcpyDeviceToHost));
+          if (clen[v]) {
+            // column v of the chain table (stride npl), gathered on the device:
+            // a strided copy of 4-byte rows costs the copy engine ~2 us per row
+            HIPCHK(launch_gather_strided(s.chain + v, npl, clen[v], s.nodes_io, st));
+            HIPCHK(hipMemcpyAsync(cycle.data(), s.nodes_io, (size_t)clen[v] * 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+          }
         }
         break;
       } else if (first) {
@@ -1851,6 +1865,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
       continue;
     }
     // branch(): split by a multi-source unit-weight BFS (dijkstra_solver_cuda.cpp:30-117)
+    if ((rc = ensure_bg())) return rc;
     maxBranchId++;
     {
       BnbHeap bh(npl);
