@@ -257,57 +257,176 @@ __global__ void k_quantize(double* __restrict__ col, int64_t n, const SelectStat
 }
 
 // ---- batched forms: one launch over many columns (grid.y = column), the
-// same arithmetic per column as the single-column kernels above
-__global__ void k_bselect_init(const QItem* __restrict__ items, SelectState* __restrict__ states) {
+// same arithmetic per column as the single-column kernels above.
+//
+// Batched median by a sampled bracket. Per column: sort kMedianSample
+// strided keys (all keys when the column is that short: then the median is
+// read off the sorted sample), take the keys kMedianMargin sample positions
+// below and above the median's expected position as a bracket [lo, hi], count
+// the keys below lo and gather the keys inside; when the counts show the two
+// median ranks inside the bracket, radix-select them among the gathered keys,
+// else (or when the gathered keys overflow the column's capacity) run the
+// full radix select on the column. Either way the result is the exact
+// k-th smallest key, as the full select computes it.
+struct BracketState {
+  uint64_t lo, hi;
+  unsigned long long nlo, cnt;
+  int32_t done, miss;
+};
+
+__device__ __forceinline__ void median_ranks(int64_t n, int64_t& k0, int64_t& k1) {
+  k0 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
+  k1 = (n % 2 == 0) ? k0 + 1 : k0;
+}
+
+__global__ void __launch_bounds__(1024) k_bsample(const QItem* __restrict__ items, SelectState* __restrict__ states,
+                                                 BracketState* __restrict__ brs) {
+  __shared__ uint64_t key[kMedianSample];
+  const QItem it = items[blockIdx.x];
   SelectState* st = states + blockIdx.x;
-  const int64_t n = items[blockIdx.x].n;
-  const int t = threadIdx.x;
-  if (t < 256) {
-    st->hist[0][t] = 0;
-    st->hist[1][t] = 0;
+  BracketState* b = brs + blockIdx.x;
+  const int64_t n = it.n;
+  const bool all = n <= kMedianSample;
+  for (int j = threadIdx.x; j < kMedianSample; j += blockDim.x) {
+    const bool in = all ? j < n : true;
+    const int64_t i = all ? j : (int64_t)j * n / kMedianSample;
+    key[j] = in ? dkey(it.col[i]) : ~0ull;
   }
-  if (t == 0) {
-    const int64_t k1 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
+  __syncthreads();
+  // bitonic sort, ascending
+  for (int size = 2; size <= kMedianSample; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < kMedianSample / 2; t += blockDim.x) {
+        const int i = 2 * t - (t & (stride - 1));
+        const int j = i + stride;
+        const bool up = (i & size) == 0;
+        const uint64_t a = key[i], c = key[j];
+        if ((a > c) == up) {
+          key[i] = c;
+          key[j] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) (&st->hist[0][0])[i] = 0;
+  if (threadIdx.x == 0) {
+    int64_t k0, k1;
+    median_ranks(n, k0, k1);
     st->prefix[0] = st->prefix[1] = 0;
     st->mask[0] = st->mask[1] = 0;
-    st->k[0] = k1;
-    st->k[1] = (n % 2 == 0) ? k1 + 1 : k1;
+    st->k[0] = k0;
+    st->k[1] = k1;
+    b->nlo = 0;
+    b->cnt = 0;
+    b->miss = 0;
+    b->lo = 0;
+    b->hi = ~0ull;
+    if (all) {
+      b->done = 1;
+      st->value[0] = n > 0 ? dkey_inv(key[k0]) : 0.0;
+      st->value[1] = n > 0 ? dkey_inv(key[k1]) : 0.0;
+    } else {
+      b->done = 0;
+      const int64_t lo = k0 * kMedianSample / n - kMedianMargin;
+      const int64_t hi = k1 * kMedianSample / n + kMedianMargin;
+      b->lo = lo <= 0 ? 0ull : key[lo];
+      b->hi = hi >= kMedianSample - 1 ? ~0ull : key[hi];
+    }
   }
 }
 
-__global__ void k_bselect_hist(const QItem* __restrict__ items, SelectState* __restrict__ states, int shift) {
+// keys below the bracket counted, keys inside gathered (wave-aggregated appends)
+__global__ void k_bbracket(const QItem* __restrict__ items, BracketState* __restrict__ brs, double* __restrict__ cand) {
+  const QItem it = items[blockIdx.y];
+  BracketState* b = brs + blockIdx.y;
+  if (b->done) return;
+  const uint64_t lo = b->lo, hi = b->hi;
+  const int lane = threadIdx.x & 63;
+  unsigned long long below = 0;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < it.n; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    const bool valid = i < it.n;
+    const double x = it.col[valid ? i : 0];
+    const uint64_t k = dkey(x);
+    below += (unsigned long long)__popcll(__ballot(valid && k < lo));
+    const bool in = valid && k >= lo && k <= hi;
+    const unsigned long long m = __ballot(in);
+    if (m) {
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&b->cnt, (unsigned long long)__popcll(m));
+      const uint32_t blo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+      const uint32_t bhi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+      const int64_t pos = (int64_t)(((uint64_t)bhi << 32) | blo) + __popcll(m & ((1ull << lane) - 1));
+      if (in && pos < it.cand_cap) cand[it.cand_off + pos] = x;
+    }
+  }
+  if (lane == 0 && below) atomicAdd(&b->nlo, below);
+}
+
+// ranks inside the bracket, or the full select when the bracket missed
+__global__ void k_bcand_init(const QItem* __restrict__ items, SelectState* __restrict__ states,
+                             BracketState* __restrict__ brs) {
+  if (threadIdx.x != 0) return;
+  const QItem it = items[blockIdx.x];
+  SelectState* st = states + blockIdx.x;
+  BracketState* b = brs + blockIdx.x;
+  if (b->done) return;
+  int64_t k0, k1;
+  median_ranks(it.n, k0, k1);
+  const int64_t nlo = (int64_t)b->nlo, cnt = (int64_t)b->cnt;
+  if (cnt > it.cand_cap || k0 < nlo || k1 >= nlo + cnt) {
+    b->miss = 1;
+  } else {
+    st->k[0] = k0 - nlo;
+    st->k[1] = k1 - nlo;
+  }
+}
+
+// histogram pass over the gathered keys (CAND) or over the whole column of a
+// column whose bracket missed (!CAND); other columns return at once
+template <bool CAND>
+__global__ void k_bselect_hist_br(const QItem* __restrict__ items, SelectState* __restrict__ states,
+                                  const BracketState* __restrict__ brs, const double* __restrict__ cand, int shift) {
+  const BracketState* b = brs + blockIdx.y;
+  if (b->done || (CAND ? b->miss != 0 : b->miss == 0)) return;
   __shared__ uint32_t h[2][256];
   const QItem it = items[blockIdx.y];
   SelectState* st = states + blockIdx.y;
+  const double* col = CAND ? cand + it.cand_off : it.col;
+  const int64_t n = CAND ? (int64_t)b->cnt : it.n;
   h[0][threadIdx.x] = 0;
   h[1][threadIdx.x] = 0;
   __syncthreads();
   const uint64_t p0 = st->prefix[0], m0 = st->mask[0], p1 = st->prefix[1], m1 = st->mask[1];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.n; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t kk = dkey(it.col[i]);
-    const uint32_t b = (uint32_t)(kk >> shift) & 255u;
-    if ((kk & m0) == p0) atomicAdd(&h[0][b], 1u);
-    if ((kk & m1) == p1) atomicAdd(&h[1][b], 1u);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t kk = dkey(col[i]);
+    const uint32_t bk = (uint32_t)(kk >> shift) & 255u;
+    if ((kk & m0) == p0) atomicAdd(&h[0][bk], 1u);
+    if ((kk & m1) == p1) atomicAdd(&h[1][bk], 1u);
   }
   __syncthreads();
   if (h[0][threadIdx.x]) atomicAdd(&st->hist[0][threadIdx.x], h[0][threadIdx.x]);
   if (h[1][threadIdx.x]) atomicAdd(&st->hist[1][threadIdx.x], h[1][threadIdx.x]);
 }
 
-__global__ void k_bselect_step(SelectState* __restrict__ states, int shift) {
+template <bool CAND>
+__global__ void k_bselect_step_br(SelectState* __restrict__ states, const BracketState* __restrict__ brs, int shift) {
+  const BracketState* b = brs + blockIdx.x;
+  if (b->done || (CAND ? b->miss != 0 : b->miss == 0)) return;
   SelectState* st = states + blockIdx.x;
   if (threadIdx.x < 2) {
     const int t = threadIdx.x;
     int64_t cum = 0;
-    int b = 0;
-    for (; b < 256; ++b) {
-      const int64_t c = st->hist[t][b];
+    int bk = 0;
+    for (; bk < 256; ++bk) {
+      const int64_t c = st->hist[t][bk];
       if (st->k[t] < cum + c) break;
       cum += c;
     }
-    if (b > 255) b = 255;
+    if (bk > 255) bk = 255;
     st->k[t] -= cum;
-    st->prefix[t] |= ((uint64_t)b) << shift;
+    st->prefix[t] |= ((uint64_t)bk) << shift;
     st->mask[t] |= ((uint64_t)255) << shift;
     if (shift == 0) st->value[t] = dkey_inv(st->prefix[t]);
   }
@@ -545,14 +664,27 @@ hipError_t launch_quantize(double* col, int64_t n, void* selectState, hipStream_
 
 size_t select_state_bytes() { return sizeof(SelectState); }
 
-// Quantization of n columns (the median of each, then q-units) in 18 launches.
-hipError_t launch_quantize_batch(const QItem* d_items, int n, void* states, hipStream_t st) {
+size_t bracket_state_bytes() { return sizeof(BracketState); }
+
+// Quantization of n columns (the median of each, then q-units): one full pass
+// over the keys for the bracket, eight radix passes over the gathered keys,
+// eight full passes only for columns whose bracket missed (the others return
+// at once), then the quantization pass.
+hipError_t launch_quantize_batch(const QItem* d_items, int n, void* states, void* brackets, double* cand,
+                                 hipStream_t st) {
   if (n <= 0) return hipSuccess;
   SelectState* S = (SelectState*)states;
-  hipLaunchKernelGGL(k_bselect_init, dim3(n), dim3(256), 0, st, d_items, S);
+  BracketState* B = (BracketState*)brackets;
+  hipLaunchKernelGGL(k_bsample, dim3(n), dim3(1024), 0, st, d_items, S, B);
+  hipLaunchKernelGGL(k_bbracket, dim3(32, n), dim3(256), 0, st, d_items, B, cand);
+  hipLaunchKernelGGL(k_bcand_init, dim3(n), dim3(64), 0, st, d_items, S, B);
   for (int shift = 56; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(k_bselect_hist, dim3(32, n), dim3(256), 0, st, d_items, S, shift);
-    hipLaunchKernelGGL(k_bselect_step, dim3(n), dim3(256), 0, st, S, shift);
+    hipLaunchKernelGGL(k_bselect_hist_br<true>, dim3(8, n), dim3(256), 0, st, d_items, S, B, cand, shift);
+    hipLaunchKernelGGL(k_bselect_step_br<true>, dim3(n), dim3(256), 0, st, S, B, shift);
+  }
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(k_bselect_hist_br<false>, dim3(32, n), dim3(256), 0, st, d_items, S, B, cand, shift);
+    hipLaunchKernelGGL(k_bselect_step_br<false>, dim3(n), dim3(256), 0, st, S, B, shift);
   }
   hipLaunchKernelGGL(k_bquantize, dim3(32, n), dim3(256), 0, st, d_items, S, 100000.0);
   return hipGetLastError();

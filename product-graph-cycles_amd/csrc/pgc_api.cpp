@@ -35,10 +35,11 @@ hipError_t launch_product_space(const int32_t*, int64_t, const int32_t*, int64_t
 hipError_t launch_energy(const double*, const double*, const int32_t*, int64_t, const double*, int64_t, int, int,
                          double*, hipStream_t);
 hipError_t launch_quantize(double*, int64_t, void*, hipStream_t);
-hipError_t launch_quantize_batch(const QItem*, int, void*, hipStream_t);
+hipError_t launch_quantize_batch(const QItem*, int, void*, void*, double*, hipStream_t);
 hipError_t launch_fill_costs_batch(const FItem*, int, hipStream_t);
 hipError_t launch_energy_batch(const EItem*, int, hipStream_t);
 size_t select_state_bytes();
+size_t bracket_state_bytes();
 hipError_t launch_layout_count(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int32_t*, int32_t*,
                                hipStream_t);
 hipError_t launch_layout_width(const int32_t*, int64_t, int32_t*, hipStream_t);
@@ -215,6 +216,10 @@ struct pgc_context {
   hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   uint8_t* d_select = nullptr;  // select states of the batched median
   int64_t select_cap = 0;
+  uint8_t* d_bracket = nullptr;  // its per-column brackets
+  int64_t bracket_cap = 0;
+  double* d_cand = nullptr;  // and the keys gathered inside the brackets
+  int64_t cand_cap = 0;
 };
 
 namespace {
@@ -344,7 +349,7 @@ void pgc_context_destroy(pgc_context* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values,
-                c->d_qitems, c->d_fitems, c->d_eitems, c->d_select};
+                c->d_qitems, c->d_fitems, c->d_eitems, c->d_select, c->d_bracket, c->d_cand};
   for (void* p : ps)
     if (p) cache_free(p);
   for (int k = 0; k < 4; ++k) {
@@ -1442,7 +1447,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     if ((rc = build_layout(g))) return rc;
     if ((rc = check_width(g))) return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
     if ((rc = cost_buffers(g, 1))) return rc;
-    for (int c = 0; c < g->ncols; ++c) qitems.push_back({g->d_energy + (int64_t)c * g->E, g->E});
+    for (int c = 0; c < g->ncols; ++c) qitems.push_back({g->d_energy + (int64_t)c * g->E, g->E, 0, 0});
     for (int c = 0; c < 2; ++c)
       fitems.push_back({g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, (float*)g->d_cost[c],
                         g->ncols > 1 ? g->d_time[c] : nullptr});
@@ -1460,14 +1465,22 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     const int64_t chunk = 65535;
     if ((rc = grow(&ctx->d_qitems, ctx->qitems_cap, (int64_t)qitems.size()))) return rc;
     if ((rc = grow(&ctx->d_fitems, ctx->fitems_cap, (int64_t)fitems.size()))) return rc;
-    if ((rc = grow(&ctx->d_select, ctx->select_cap,
-                   (int64_t)std::min<size_t>(qitems.size(), (size_t)chunk) * (int64_t)select_state_bytes())))
-      return rc;
+    const int64_t nsel = (int64_t)std::min<size_t>(qitems.size(), (size_t)chunk);
+    if ((rc = grow(&ctx->d_select, ctx->select_cap, nsel * (int64_t)select_state_bytes()))) return rc;
+    if ((rc = grow(&ctx->d_bracket, ctx->bracket_cap, nsel * (int64_t)bracket_state_bytes()))) return rc;
+    // the median brackets' candidate regions, one per column
+    int64_t candN = 0;
+    for (QItem& q : qitems) {
+      q.cand_off = candN;
+      q.cand_cap = median_cand_cap(q.n);
+      candN += q.cand_cap;
+    }
+    if ((rc = grow(&ctx->d_cand, ctx->cand_cap, std::max<int64_t>(candN, 1)))) return rc;
     HIPCHK(hipMemcpyAsync(ctx->d_qitems, qitems.data(), qitems.size() * sizeof(QItem), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(ctx->d_fitems, fitems.data(), fitems.size() * sizeof(FItem), hipMemcpyHostToDevice, st));
     for (int64_t a = 0; a < (int64_t)qitems.size(); a += chunk)
       HIPCHK(launch_quantize_batch(ctx->d_qitems + a, (int)std::min<int64_t>(chunk, (int64_t)qitems.size() - a),
-                                   ctx->d_select, st));
+                                   ctx->d_select, ctx->d_bracket, ctx->d_cand, st));
     for (int64_t a = 0; a < (int64_t)fitems.size(); a += chunk)
       HIPCHK(launch_fill_costs_batch(ctx->d_fitems + a, (int)std::min<int64_t>(chunk, (int64_t)fitems.size() - a),
                                      st));

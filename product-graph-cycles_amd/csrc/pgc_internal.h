@@ -159,11 +159,29 @@ struct CycleItem {
   double gout;          // backward: upstream gradient
 };
 
-// One column of a batched quantization.
+// One column of a batched quantization. Its median bracket's candidates go
+// to cand[cand_off, cand_off + cand_cap) of the batch's candidate buffer.
 struct QItem {
   double* col;
   int64_t n;
+  int64_t cand_off;
+  int64_t cand_cap;
 };
+
+// Keys sampled per column, and the bracket's margin in sample positions on
+// each side of the median's expected position (about four standard
+// deviations of the sample rank of the median).
+constexpr int kMedianSample = 8192;
+constexpr int kMedianMargin = 192;
+
+// Candidate capacity of a column of n keys: the bracket holds about
+// (2 * margin + 1) / sample = 4.7% of the keys; a column whose bracket
+// overflows this falls back to the full radix select.
+inline int64_t median_cand_cap(int64_t n) {
+  if (n <= kMedianSample) return 0;
+  const int64_t c = n / 8 > 16384 ? n / 8 : 16384;
+  return c < n ? c : n;
+}
 
 // One slot class of a batched fp32 cost fill.
 // One graph of the batched energy kernel (k_benergy).

@@ -129,3 +129,35 @@ def test_sweep_cap_case_golden():
         assert [res.bf_calls, res.sweeps, res.lawler_iters, res.branches] == list(z["stats"])
     g1.close()
     gb.close()
+
+
+def test_batch_median_bracket_miss_and_overflow():
+    # the batched median samples 8192 strided keys per column and brackets the
+    # median's expected position; these columns defeat the sample (every
+    # sampled key is the smallest) or overflow the bracket (all keys equal),
+    # so they take the full radix select; the quantized costs must still equal
+    # the oracle's quantization (numpy-pinned median and rounding)
+    ctx = _lib.default_context(0)
+    p = S.make_pair(2, 33, seed=5)
+    rows, src, trgt = O.product_space(p["EX"], p["EY"])
+    npl = len(p["VX"])
+    E = len(src)
+    assert E > 8192
+    rng = np.random.default_rng(3)
+    sampled = (np.arange(8192, dtype=np.int64) * E) // 8192
+    miss = 2.0 + np.arange(E) * 1e-6
+    miss[sampled] = 1.0
+    flat = np.full(E, 3.0)
+    flat[rng.integers(0, E, 50)] = 7.0
+    normal = rng.uniform(0.1, 20.0, E)
+    cols = {"miss": miss, "overflow": flat, "normal": normal}
+    graphs = [_lib.Graph.from_arrays(ctx, rows, src, trgt, c, npl) for c in cols.values()]
+    # one more column of odd length (the two median ranks coincide)
+    graphs.append(_lib.Graph.from_arrays(ctx, rows[:-1], src[:-1], trgt[:-1], normal[:-1], npl))
+    print("batch of", len(graphs), "array graphs, E", E, flush=True)
+    _lib.solve_ratio_batch(ctx, graphs, 1)
+    expect = [O.quantize(c[:, None])[:, 0] for c in cols.values()] + [O.quantize(normal[:-1, None])[:, 0]]
+    for name, g, q in zip(list(cols) + ["odd"], graphs, expect):
+        assert np.array_equal(g.cost()[:, 0], q), "quantized costs differ (%s column)" % name
+    for g in graphs:
+        g.close()
