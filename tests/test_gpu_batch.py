@@ -161,3 +161,31 @@ def test_batch_median_bracket_miss_and_overflow():
         assert np.array_equal(g.cost()[:, 0], q), "quantized costs differ (%s column)" % name
     for g in graphs:
         g.close()
+
+
+def test_batch_corrected_odd_L_matches_single_and_oracle():
+    # PGC_CORRECTED_ODD_L through the batch path: odd layer counts with K >= 2
+    # (where the corrected fold differs from the reference's), against
+    # pgc_solve_ratio with the same flag and the oracle's corrected solver
+    ctx = _lib.default_context(0)
+    flags = _lib.PGC_LAWLER_PICK_LAST | _lib.PGC_CORRECTED_ODD_L
+    cases = [(S.tiny_pair(nx, ny, seed=s + 60, ring=ring), K)
+             for s, (nx, ny, ring) in enumerate([(4, 5, True), (5, 7, True), (4, 5, False), (6, 3, True)])
+             for K in (2, 3)]
+    cases += [(S.make_pair(2, 33, seed=1), 2), (S.make_pair(2, 41, seed=4), 3)]
+    print("batch of", len(cases), "odd-L graphs, corrected", flush=True)
+    graphs = [_graph(ctx, p) for p, K in cases]
+    outs = _lib.solve_ratio_batch(ctx, graphs, [K for _, K in cases], flags)
+    for (p, K), (eids, rows, res) in zip(cases, outs):
+        orow, src, trgt = O.product_space(p["EX"], p["EY"])
+        en = O.energy(p["VX"], p["VY"], orow, p["feat_diff"].astype(np.float64))
+        oe, oobj, _, stats, _ = O.ctr_solve(src, trgt, en, len(p["VX"]), K, gpu_pick=True, corrected=True)
+        assert np.array_equal(rows, orow[oe]), "matching differs from the oracle"
+        assert res.objective == float(oobj)
+        assert (res.bf_calls, res.sweeps, res.branches) == (stats[0], stats[1], stats[3])
+        g1 = _graph(ctx, p)
+        e1, r1, res1 = g1.solve_ratio(K, flags)
+        assert np.array_equal(eids, e1) and res.objective == res1.objective
+        g1.close()
+    for g in graphs:
+        g.close()
