@@ -713,7 +713,11 @@ struct Solver {
         bestKey = c.key;
         bestV = c.v;
       }
-      block_min_key(sh, bestKey, bestV);
+      // the block minimum (shuffles, three barriers and a scan of the wave
+      // minima) only when some thread has a candidate: in long Bellman-Ford
+      // calls no walk closes in most sweeps, and then every thread holds
+      // (+inf, none) already, which is the minimum
+      if (bor(sh, bestV != 0x7fffffff)) block_min_key(sh, bestKey, bestV);
       t1 = wclock();
       tWalk += t1 - t0;
       if (bestV != 0x7fffffff) {
