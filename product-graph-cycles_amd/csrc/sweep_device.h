@@ -152,16 +152,8 @@ struct Sweep {
     double dv[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
-    // slot 0 opens the scan (the winner is read only when the minimum is
-    // finite, and then it is the first slot reaching it either way)
-    best = dv[0] + weight(c[0], t[0], ratio, rt);
-    bs = s[0];
-    if (!K1) {
-      bc = c[0];
-      if (HAS_TIME) bt = t[0];
-    }
 #pragma unroll
-    for (int j = 1; j < W; ++j) {
+    for (int j = 0; j < W; ++j) {
       const double nv = dv[j] + weight(c[j], t[j], ratio, rt);
       const bool better = nv < best;
       best = pgc_min(best, nv);
@@ -221,10 +213,8 @@ struct Sweep {
     double dv[W];
 #pragma unroll
     for (int j = 0; j < W; ++j) dv[j] = Dk[s[j]];
-    best = dv[0] + weight(c[0], t[0], ratio, rt);
-    bs = s[0];
 #pragma unroll
-    for (int j = 1; j < W; ++j) {
+    for (int j = 0; j < W; ++j) {
       const double nv = dv[j] + weight(c[j], t[j], ratio, rt);
       const bool better = nv < best;
       best = pgc_min(best, nv);
@@ -605,12 +595,10 @@ struct SweepG {
                                               double ratio, double rt, double ub) {
     double best = PGC_INF;
     int bs = npl;
-    CT bc = cc[0];
-    float bt = HAS_TIME ? t[0] : 0.0f;
-    best = dv[0] + weight(cc[0], t[0], ratio, rt);
-    bs = s[0];
+    CT bc = 0;
+    float bt = 0;
 #pragma unroll
-    for (int j = 1; j < W; ++j) {
+    for (int j = 0; j < W; ++j) {
       const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
       const bool better = nv < best;
       best = pgc_min(best, nv);
@@ -648,10 +636,10 @@ struct SweepG {
   __device__ __forceinline__ bool intra_relax(int layer, int cur, int k, int v, const int (&s)[W],
                                               const CT (&cc)[W], const float (&t)[W], const double (&dv)[W],
                                               double ratio, double rt, double ub) {
-    double best = dv[0] + weight(cc[0], t[0], ratio, rt);
-    int bs = s[0];
+    double best = PGC_INF;
+    int bs = npl;
 #pragma unroll
-    for (int j = 1; j < W; ++j) {
+    for (int j = 0; j < W; ++j) {
       const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
       const bool better = nv < best;
       best = pgc_min(best, nv);
@@ -756,12 +744,10 @@ struct SweepG {
       for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
       double best = PGC_INF;
       int bs = npl;
-      CT bc = cc[0];
-      float bt = HAS_TIME ? t[0] : 0.0f;
-      best = dv[0] + weight(cc[0], t[0], ratio, rt);
-      bs = s[0];
+      CT bc = 0;
+      float bt = 0;
 #pragma unroll
-      for (int j = 1; j < W; ++j) {
+      for (int j = 0; j < W; ++j) {
         const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
         const bool better = nv < best;
         best = pgc_min(best, nv);
@@ -811,10 +797,10 @@ struct SweepG {
       double dv[W];
 #pragma unroll
       for (int j = 0; j < W; ++j) dv[j] = Dk[s[j]];
-      double best = dv[0] + weight(cc[0], t[0], ratio, rt);
-      int bs = s[0];
+      double best = PGC_INF;
+      int bs = npl;
 #pragma unroll
-      for (int j = 1; j < W; ++j) {
+      for (int j = 0; j < W; ++j) {
         const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
         const bool better = nv < best;
         best = pgc_min(best, nv);
