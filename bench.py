@@ -274,12 +274,17 @@ def main():
                     help="run only a device copy of known size (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--cap-case", action="store_true",
                     help="run only the batch item that runs to the sweep cap (for profiling its solver launch)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched workload (its steps and the plain loop), print its JSON object")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
         return
     if args.cap_case:
         cap_case()
+        return
+    if args.batch_only:
+        print(json.dumps(batched(args)))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path in this build "

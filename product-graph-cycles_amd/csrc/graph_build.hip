@@ -336,32 +336,68 @@ __global__ void __launch_bounds__(1024) k_bsample(const QItem* __restrict__ item
   }
 }
 
-// keys below the bracket counted, keys inside gathered (wave-aggregated appends)
-__global__ void k_bbracket(const QItem* __restrict__ items, BracketState* __restrict__ brs, double* __restrict__ cand) {
+// Keys below the bracket counted, keys inside gathered. A block takes tiles
+// of 4096 keys (16 per thread, coalesced); a block-wide prefix sum of the
+// threads' counts places them, so a tile costs one atomic on the column's
+// counter (not one per wave: 55 million contended atomics for the batch).
+__global__ void __launch_bounds__(256) k_bbracket(const QItem* __restrict__ items, BracketState* __restrict__ brs,
+                                                  double* __restrict__ cand) {
+  constexpr int PER = 16;
+  constexpr int TILE = 256 * PER;
+  __shared__ int wtot[4];
+  __shared__ unsigned long long tbase;
   const QItem it = items[blockIdx.y];
   BracketState* b = brs + blockIdx.y;
   if (b->done) return;
   const uint64_t lo = b->lo, hi = b->hi;
-  const int lane = threadIdx.x & 63;
-  unsigned long long below = 0;
-  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < it.n; i0 += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = i0 + threadIdx.x;
-    const bool valid = i < it.n;
-    const double x = it.col[valid ? i : 0];
-    const uint64_t k = dkey(x);
-    below += (unsigned long long)__popcll(__ballot(valid && k < lo));
-    const bool in = valid && k >= lo && k <= hi;
-    const unsigned long long m = __ballot(in);
-    if (m) {
-      unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&b->cnt, (unsigned long long)__popcll(m));
-      const uint32_t blo = __builtin_amdgcn_readfirstlane((uint32_t)base);
-      const uint32_t bhi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
-      const int64_t pos = (int64_t)(((uint64_t)bhi << 32) | blo) + __popcll(m & ((1ull << lane) - 1));
-      if (in && pos < it.cand_cap) cand[it.cand_off + pos] = x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t below = 0;
+  for (int64_t t0 = (int64_t)blockIdx.x * TILE; t0 < it.n; t0 += (int64_t)gridDim.x * TILE) {
+    double x[PER];
+    uint32_t inm = 0;
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int64_t i = t0 + q * 256 + threadIdx.x;
+      const bool valid = i < it.n;
+      x[q] = it.col[valid ? i : 0];
+      const uint64_t k = dkey(x[q]);
+      below += (valid && k < lo) ? 1u : 0u;
+      const bool in = valid && k >= lo && k <= hi;
+      inm |= (in ? 1u : 0u) << q;
+      cnt += in ? 1 : 0;
     }
+    // inclusive scan of the counts in the wave, wave totals through LDS
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(inc, o);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    int wbase = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int t = wtot[k];
+      wbase += k < w ? t : 0;
+      total += t;
+    }
+    if (threadIdx.x == 0) tbase = total ? atomicAdd(&b->cnt, (unsigned long long)total) : 0ull;
+    __syncthreads();
+    int64_t pos = (int64_t)tbase + wbase + inc - cnt;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      if ((inm >> q) & 1u) {
+        if (pos < it.cand_cap) cand[it.cand_off + pos] = x[q];
+        ++pos;
+      }
+    }
+    __syncthreads();  // wtot and tbase are rewritten by the next tile
   }
-  if (lane == 0 && below) atomicAdd(&b->nlo, below);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) below += __shfl_xor(below, o);
+  if (lane == 0 && below) atomicAdd(&b->nlo, (unsigned long long)below);
 }
 
 // ranks inside the bracket, or the full select when the bracket missed
