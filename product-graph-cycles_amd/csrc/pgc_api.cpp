@@ -1475,7 +1475,17 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
       q.cand_cap = median_cand_cap(q.n);
       candN += q.cand_cap;
     }
-    if ((rc = grow(&ctx->d_cand, ctx->cand_cap, std::max<int64_t>(candN, 1)))) return rc;
+    // exact size (no growth margin): it is 1/8 of the batch's energies
+    if (candN > ctx->cand_cap || !ctx->d_cand) {
+      if (ctx->d_cand) {
+        (void)hipDeviceSynchronize();  // queued work may still use the buffer
+        cache_free(ctx->d_cand);
+        ctx->d_cand = nullptr;
+        ctx->cand_cap = 0;
+      }
+      HIPCHK(dalloc(&ctx->d_cand, std::max<int64_t>(candN, 1)));
+      ctx->cand_cap = std::max<int64_t>(candN, 1);
+    }
     HIPCHK(hipMemcpyAsync(ctx->d_qitems, qitems.data(), qitems.size() * sizeof(QItem), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(ctx->d_fitems, fitems.data(), fitems.size() * sizeof(FItem), hipMemcpyHostToDevice, st));
     for (int64_t a = 0; a < (int64_t)qitems.size(); a += chunk)
