@@ -97,10 +97,35 @@ __global__ void k_energy(const double* __restrict__ VX, const double* __restrict
 
 // k_energy for a list of graphs (blockIdx.y = graph), each with its own
 // feat_diff; a float32 feat_diff is widened exactly.
+// Row (y0, y1, x0, x1) of edge e of a non-conjugate product space, as
+// k_product_space writes it: layer i = e / B with B edges per layer (per X
+// edge an intralayer and an interlayer edge, or only the interlayer one when
+// pruned, then one stay edge per X vertex).
+__device__ __forceinline__ int4 decode_row(const EItem& it, int64_t e) {
+  const int64_t per_x = it.prune ? 1 : 2;
+  const int64_t B = per_x * it.nEX + it.nVX;
+  const int64_t i = (e < 0xffffffffll && B < 0xffffffffll) ? (int64_t)((uint32_t)e / (uint32_t)B) : e / B;
+  const int64_t r = e - i * B;
+  const int32_t y0 = it.ey[2 * i], y1 = it.ey[2 * i + 1];
+  int4 row;
+  row.x = y0;
+  if (r < per_x * it.nEX) {
+    const int64_t j = r / per_x;
+    const bool intra = !it.prune && (r % 2 == 0);
+    row.y = intra ? y0 : y1;
+    row.z = it.ex[2 * j];
+    row.w = it.ex[2 * j + 1];
+  } else {
+    row.y = y1;
+    row.z = row.w = (int32_t)(r - per_x * it.nEX);
+  }
+  return row;
+}
+
 __global__ void k_benergy(const EItem* __restrict__ items) {
   const EItem it = items[blockIdx.y];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < it.E; e += (int64_t)gridDim.x * blockDim.x) {
-    const int4 row = reinterpret_cast<const int4*>(it.rows)[e];
+    const int4 row = it.ex ? decode_row(it, e) : reinterpret_cast<const int4*>(it.rows)[e];
     const int64_t y0 = row.x, y1 = row.y, x0 = row.z, x1 = row.w;
     const double len = it.line ? edge_length(it.vx, it.vy, y0, y1, x0, x1) : 1.0;
     const int64_t a = x0 * it.nVY + y0, b = x1 * it.nVY + y1;

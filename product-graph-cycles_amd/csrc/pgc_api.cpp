@@ -287,6 +287,11 @@ struct pgc_graph {
   // energy inputs, kept to recompute the energy from another feat_diff
   double *d_vx = nullptr, *d_vy = nullptr;
   int64_t nVX = 0, nVY = 0;
+  // the X and Y edge lists (non-conjugate graphs): the batched energy decodes
+  // an edge's row from its id with them instead of reading the rows
+  int32_t *d_ex = nullptr, *d_ey = nullptr;
+  int64_t nEX = 0, nEY = 0;
+  int prune = 0;
   bool conjugate = false;
   int line_integral = 0, length_mode = 0;
   // a derived graph shares base's product space, ids, layout and energy
@@ -304,9 +309,12 @@ struct pgc_graph {
       d_width = d_cnt = nullptr;
       d_s[0] = d_s[1] = d_e[0] = d_e[1] = nullptr;
       d_vx = d_vy = nullptr;
+      d_ex = d_ey = nullptr;
     }
     if (d_vx) cache_free(d_vx);
     if (d_vy) cache_free(d_vy);
+    if (d_ex) cache_free(d_ex);
+    if (d_ey) cache_free(d_ey);
     void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_width, d_s[0], d_s[1], d_e[0], d_e[1],
                   d_cnt, d_cost[0], d_cost[1], d_time[0], d_time[1]};
     for (void* p : ps)
@@ -978,6 +986,13 @@ int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   HIPCHK(dalloc(&g->d_vy, 3 * nVY));
   HIPCHK(hipMemcpyAsync(g->d_vx, dVX, 3 * nVX * 8, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(g->d_vy, dVY, 3 * nVY * 8, hipMemcpyDeviceToDevice, st));
+  HIPCHK(dalloc(&g->d_ex, std::max<int64_t>(2 * nEX, 1)));
+  HIPCHK(dalloc(&g->d_ey, std::max<int64_t>(2 * nEY, 1)));
+  if (nEX) HIPCHK(hipMemcpyAsync(g->d_ex, dEX, 2 * nEX * 4, hipMemcpyDeviceToDevice, st));
+  if (nEY) HIPCHK(hipMemcpyAsync(g->d_ey, dEY, 2 * nEY * 4, hipMemcpyDeviceToDevice, st));
+  g->nEX = nEX;
+  g->nEY = nEY;
+  g->prune = prune ? 1 : 0;
   HIPCHK(launch_energy(dVX, dVY, g->d_rows, E, dFD, nVY, g->line_integral, g->length_mode, g->d_energy, st));
   // numNodes / numLayers as the solver derives them (run(), :870-871)
   g->numNodes = nEY * nVXe;
@@ -1398,7 +1413,7 @@ int pgc_graphs_set_feat_diff(pgc_graph* const* graphs, const void* const* feat_d
       return fail(PGC_ERR_UNSUPPORTED, "feat_diff updates are available for non-conjugate graphs only");
     if (!b->d_vx) return fail(PGC_ERR_UNSUPPORTED, "the graph was not built by pgc_graph_generate*: no energy inputs");
     items[i] = {b->d_vx, b->d_vy, b->d_rows, b->E, feat_diff[i], b->nVY, b->line_integral, b->length_mode, dtype,
-                g->d_energy};
+                g->d_energy, b->d_ex, b->d_ey, b->nEX, b->nEY, b->nVX, b->prune};
   }
   int rc;
   if ((rc = grow(&ctx->d_eitems, ctx->eitems_cap, n))) return rc;

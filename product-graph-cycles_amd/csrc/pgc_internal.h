@@ -194,6 +194,12 @@ struct EItem {
   int64_t nVY;
   int32_t line, lengthMode, f32;
   double* energy;
+  // the X and Y edge lists of the product space (non-conjugate): an edge's
+  // row is decoded from its id instead of read from rows (nullptr: read it)
+  const int32_t* ex;
+  const int32_t* ey;
+  int64_t nEX, nEY, nVX;
+  int32_t prune;
 };
 
 struct FItem {

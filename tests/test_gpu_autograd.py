@@ -149,3 +149,44 @@ def test_sixteen_pair_batch_and_smoothed_gradient():
         ref = torch.stack(grads[32 * i:32 * i + 32]).mean(0)
         torch.testing.assert_close(f2[i].grad.double(), ref, rtol=1e-5, atol=1e-9)
     prob.close()
+
+
+@pytest.mark.parametrize("reg,prune,cost,time", [(False, False, "vanilla", ""), (True, False, "vanilla", ""),
+                                                 (False, True, "vanilla", ""), (True, True, "vanilla", ""),
+                                                 (False, False, "feature", "lengthnormalisation")])
+def test_batched_energies_equal_single_graph_energies(reg, prune, cost, time):
+    # pgc_graphs_set_feat_diff decodes each edge's row from its id (X and Y
+    # edge lists) instead of reading the product space; its energies must be
+    # bit-exact with the single-graph energy (which reads the rows), for
+    # float64 and float32 feat_diff
+    from cyclic_product_graphs import _lib
+    ctx = _lib.default_context(0)
+    dev = torch.device("cuda:0")
+    pairs = [S.make_pair(2, 33, seed=3), S.make_pair(3, 40, seed=4)]
+    gen = torch.Generator().manual_seed(5)
+    for p in pairs:
+        print("pair nx", len(p["VX"]), "ny", len(p["VY"]), "reg", reg, "prune", prune, cost, time, flush=True)
+        base = _lib.Graph.generate(ctx, p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"].astype(np.float64), False,
+                                   reg, prune, cost, time)
+        fds = [torch.rand(p["feat_diff"].shape, generator=gen, dtype=torch.float64) * 3.0 for _ in range(3)]
+        single = []
+        for f in fds:
+            g = base.derive(f.numpy())
+            single.append(g.cost())
+            g.close()
+        items = [base.derive(p["feat_diff"].astype(np.float64)) for _ in fds]
+        d64 = [f.to(dev).contiguous() for f in fds]
+        _lib.set_feat_diff_batch(items, d64, 0)
+        torch.cuda.synchronize()
+        for g, want in zip(items, single):
+            assert np.array_equal(g.cost(), want), "float64 feat_diff: batched energy differs"
+        d32 = [f.to(torch.float32).to(dev).contiguous() for f in fds]
+        _lib.set_feat_diff_batch(items, d32, 1)
+        torch.cuda.synchronize()
+        for g, f in zip(items, fds):
+            h = base.derive(f.to(torch.float32).double().numpy())
+            assert np.array_equal(g.cost(), h.cost()), "float32 feat_diff: batched energy differs"
+            h.close()
+        for g in items:
+            g.close()
+        base.close()
