@@ -183,3 +183,31 @@ def test_c1_full_size(name):
         _check_ratio(p, 3, name)
     else:
         _check_mincost(p, 3, name)
+
+
+def _random_cases(n, seed):
+    # seeded draws over the configuration space (deterministic case list)
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        sub = int(rng.integers(1, 3))
+        ny = int(rng.integers(6, 41))
+        K = int(rng.integers(1, 4))
+        mode = int(rng.integers(0, 3))  # 0 vanilla, 1 vanilla + regulariser, 2 feature / lengthnormalisation
+        prune = bool(rng.integers(0, 4) == 0)
+        corrected = bool(rng.integers(0, 2))
+        out.append((i, sub, ny, K, mode, prune, corrected, int(rng.integers(1 << 30))))
+    return out
+
+
+@pytest.mark.parametrize("i,sub,ny,K,mode,prune,corrected,seed", _random_cases(16, 2024))
+@pytest.mark.parametrize("name", list(LAWLER))
+def test_ratio_random_configurations(i, sub, ny, K, mode, prune, corrected, seed, name):
+    kw = [{}, {"reg": True}, {"cost": "feature", "time": "lengthnormalisation"}][mode]
+    _check_ratio(S.make_pair(sub, ny, seed=seed), K, name, prune=prune, corrected=corrected, **kw)
+
+
+@pytest.mark.parametrize("i,sub,ny,K,mode,prune,corrected,seed", _random_cases(12, 77))
+@pytest.mark.parametrize("name", list(MINCOST))
+def test_mincost_random_configurations(i, sub, ny, K, mode, prune, corrected, seed, name):
+    _check_mincost(S.make_pair(sub, ny, seed=seed), K, name, prune=prune, corrected=corrected)
