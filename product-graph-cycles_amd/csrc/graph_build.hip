@@ -513,12 +513,12 @@ __global__ void k_bfill_costs(const FItem* __restrict__ items) {
   const FItem it = items[blockIdx.y];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.nslots;
        i += (int64_t)gridDim.x * blockDim.x) {
-    if (it.s[i] < 0 || it.eid[i] < 0) {
+    const int64_t e = it.eid[i];  // -1: a padding slot (see k_fill_costs)
+    if (e < 0) {
       it.cost[i] = 0.f;
       if (it.time) it.time[i] = 0.f;
       continue;
     }
-    const int64_t e = it.eid[i];
     it.cost[i] = (float)it.energy[e];
     if (it.time) it.time[i] = (float)it.energy[it.E + e];
   }
@@ -657,12 +657,14 @@ __global__ void k_fill_costs(const int32_t* __restrict__ eid, const int32_t* __r
                              float* __restrict__ time) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nSlots) return;
-  if (s[i] < 0 || eid[i] < 0) {
+  // padding slots are the ones without an edge (their source is the
+  // sentinel npl, never negative once k_pad_src has run)
+  const int64_t e = eid[i];
+  if (e < 0) {
     cost[i] = (CT)0;
     if (time) time[i] = 0.f;
     return;
   }
-  const int64_t e = eid[i];
   cost[i] = (CT)energy[e];
   if (time) time[i] = (float)energy[E + e];
 }
