@@ -15,6 +15,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "fastdiv.h"
 #include "pgc_internal.h"
 
 namespace pgc {
@@ -528,23 +529,31 @@ __global__ void k_bfill_costs(const FItem* __restrict__ items) {
 // class 0: source on the target's layer (intralayer), class 1: source on the
 // previous layer (interlayer). Targets on layer 0 whose source id is > npl
 // move to the closing copy numNodes + target (the reference's dummy nodes).
-__device__ __forceinline__ void classify(int64_t s, int64_t t, int64_t npl, int64_t numNodes, int64_t& node,
-                                         int& cls) {
+// Node ids are int32 and numNodes = max(src) + 1 <= 2^31, so every id here,
+// the closing copies included (< numNodes + npl <= 2^32), fits 32 unsigned
+// bits and the layer of an id is one FastDiv by npl. Returns false for an
+// edge that is neither (or names a negative id): it is dropped.
+__device__ __forceinline__ bool classify(int32_t s_, int32_t t_, uint32_t npl, uint32_t numNodes, const FastDiv& fd,
+                                         uint32_t& node, uint32_t& layer, int& cls) {
+  if (s_ < 0 || t_ < 0) return false;
+  const uint32_t s = (uint32_t)s_;
+  uint32_t t = (uint32_t)t_;
   if (t < npl && s > npl) t += numNodes;
   node = t;
-  const int64_t lt = t / npl, ls = s / npl;
-  cls = (ls == lt) ? 0 : ((ls == lt - 1) ? 1 : -1);
+  layer = fd.div(t);
+  const uint32_t ls = fd.div(s);
+  cls = (ls == layer) ? 0 : ((ls + 1 == layer) ? 1 : -1);
+  return cls >= 0;
 }
 
-__global__ void k_count(const int32_t* __restrict__ src, const int32_t* __restrict__ trgt, int64_t E, int64_t npl,
-                        int64_t numNodes, int32_t* __restrict__ cnt, int32_t* __restrict__ dropped) {
+__global__ void k_count(const int32_t* __restrict__ src, const int32_t* __restrict__ trgt, int64_t E, uint32_t npl,
+                        uint32_t numNodes, FastDiv fd, int32_t* __restrict__ cnt, int32_t* __restrict__ dropped) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
-  int64_t node;
+  uint32_t node, layer;
   int cls;
-  classify(src[e], trgt[e], npl, numNodes, node, cls);
-  const int64_t nTot = numNodes + npl;
-  if (cls < 0 || node >= nTot) {
+  const int64_t nTot = (int64_t)numNodes + npl;
+  if (!classify(src[e], trgt[e], npl, numNodes, fd, node, layer, cls) || node >= nTot) {
     atomicAdd(dropped, 1);
     return;
   }
@@ -578,45 +587,43 @@ __global__ void k_class_width(const int32_t* __restrict__ cnt, int64_t nTot, int
 }
 
 __global__ void k_fill_slots(const int32_t* __restrict__ src, const int32_t* __restrict__ trgt, int64_t E,
-                             int64_t npl, int64_t numNodes, int nChunks, int32_t* __restrict__ fill,
+                             uint32_t npl, uint32_t numNodes, FastDiv fd, int nChunks, int32_t* __restrict__ fill,
                              const int32_t* __restrict__ width, int32_t* __restrict__ s0, int32_t* __restrict__ s1,
                              int32_t* __restrict__ e0, int32_t* __restrict__ e1) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
-  int64_t node;
+  uint32_t node, layer;
   int cls;
-  const int64_t s = src[e];
-  classify(s, trgt[e], npl, numNodes, node, cls);
-  const int64_t nTot = numNodes + npl;
-  if (cls < 0 || node >= nTot) return;
+  const int32_t s = src[e];
+  const int64_t nTot = (int64_t)numNodes + npl;
+  if (!classify(s, trgt[e], npl, numNodes, fd, node, layer, cls) || node >= nTot) return;
   const int pos = atomicAdd(&fill[cls * nTot + node], 1);
-  const int64_t layer = node / npl;
-  const int64_t v = node - layer * npl;
+  const uint32_t v = node - layer * npl;
   const int c = (int)(v >> 6), lane = (int)(v & 63);
   const int64_t w = width[cls];
-  const int64_t slot = (layer * nChunks + c) * w * 64 + (int64_t)pos * 64 + lane;
-  const int64_t sl = s - (cls == 0 ? layer : layer - 1) * npl;
-  (cls == 0 ? s0 : s1)[slot] = (int32_t)sl;
+  const int64_t slot = ((int64_t)layer * nChunks + c) * w * 64 + (int64_t)pos * 64 + lane;
+  const int32_t sl = (int32_t)((uint32_t)s - (cls == 0 ? layer : layer - 1) * npl);
+  (cls == 0 ? s0 : s1)[slot] = sl;
   (cls == 0 ? e0 : e1)[slot] = (int32_t)e;
 }
 
-// order the slots of every node by (source, edge id)
-__global__ void k_sort_slots(const int32_t* __restrict__ cnt, int64_t npl, int64_t nLayers, int nChunks,
+// order the slots of every node by (source, edge id); nTot = nLayers * npl
+// < 2^32 (the count table's node ids)
+__global__ void k_sort_slots(const int32_t* __restrict__ cnt, uint32_t npl, int64_t nTot, FastDiv fd, int nChunks,
                              const int32_t* __restrict__ width, int32_t* __restrict__ s0, int32_t* __restrict__ s1,
                              int32_t* __restrict__ e0, int32_t* __restrict__ e1) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nTot = nLayers * npl;
   if (g >= 2 * nTot) return;
-  const int cls = (int)(g / nTot);
-  const int64_t node = g - cls * nTot;
   const int n = cnt[g];
   if (n <= 1) return;
-  const int64_t layer = node / npl;
-  const int64_t v = node - layer * npl;
+  const int cls = g >= nTot ? 1 : 0;
+  const uint32_t node = (uint32_t)(g - cls * nTot);
+  const uint32_t layer = fd.div(node);
+  const uint32_t v = node - layer * npl;
   const int c = (int)(v >> 6), lane = (int)(v & 63);
   int32_t* S = cls == 0 ? s0 : s1;
   int32_t* Ee = cls == 0 ? e0 : e1;
-  const int64_t base = (layer * nChunks + c) * (int64_t)width[cls] * 64 + lane;
+  const int64_t base = ((int64_t)layer * nChunks + c) * (int64_t)width[cls] * 64 + lane;
   for (int i = 1; i < n; ++i) {
     const int32_t ks = S[base + (int64_t)i * 64], ke = Ee[base + (int64_t)i * 64];
     int j = i - 1;
@@ -767,7 +774,11 @@ hipError_t launch_fill_costs_batch(const FItem* d_items, int n, hipStream_t st) 
 
 hipError_t launch_layout_count(const int32_t* src, const int32_t* trgt, int64_t E, int64_t npl, int64_t numNodes,
                                int32_t* cnt, int32_t* dropped, hipStream_t st) {
-  hipLaunchKernelGGL(k_count, dim3(nb(E, 256)), dim3(256), 0, st, src, trgt, E, npl, numNodes, cnt, dropped);
+  if (numNodes + npl > ((int64_t)1 << 32) || numNodes > ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  FastDiv fd;
+  fd.init((uint32_t)npl);
+  hipLaunchKernelGGL(k_count, dim3(nb(E, 256)), dim3(256), 0, st, src, trgt, E, (uint32_t)npl, (uint32_t)numNodes, fd,
+                     cnt, dropped);
   return hipGetLastError();
 }
 
@@ -783,10 +794,13 @@ hipError_t launch_layout_width(const int32_t* cnt, int64_t nTot, int32_t* width,
 hipError_t launch_layout_fill(const int32_t* src, const int32_t* trgt, int64_t E, int64_t npl, int64_t numNodes,
                               int nChunks, int32_t* fill, const int32_t* cnt, const int32_t* width, int32_t* s0,
                               int32_t* s1, int32_t* e0, int32_t* e1, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_slots, dim3(nb(E, 256)), dim3(256), 0, st, src, trgt, E, npl, numNodes, nChunks, fill,
-                     width, s0, s1, e0, e1);
-  const int64_t nLayers = numNodes / npl + 1;
-  hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nLayers * npl, 256)), dim3(256), 0, st, cnt, npl, nLayers, nChunks,
+  if (numNodes + npl > ((int64_t)1 << 32) || numNodes > ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  FastDiv fd;
+  fd.init((uint32_t)npl);
+  hipLaunchKernelGGL(k_fill_slots, dim3(nb(E, 256)), dim3(256), 0, st, src, trgt, E, (uint32_t)npl,
+                     (uint32_t)numNodes, fd, nChunks, fill, width, s0, s1, e0, e1);
+  const int64_t nTot = (numNodes / npl + 1) * npl;
+  hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nTot, 256)), dim3(256), 0, st, cnt, (uint32_t)npl, nTot, fd, nChunks,
                      width, s0, s1, e0, e1);
   return hipGetLastError();
 }
