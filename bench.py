@@ -123,6 +123,15 @@ def batched(args):
     kms = res[0].kernel_ms
     relax = sum(r.relaxations for r in res)
     sweeps = sum(r.sweeps for r in res)
+    # algorithmic bytes of every item's sweeps, per sweep as for the single
+    # pair: 4 B source + 4 B cost per product edge, 4 B parent per node
+    infos = [b.info() for b in prob.bases]
+    alg_bytes = 0.0
+    for r, i in zip(res, owner):
+        inf = infos[i]
+        vx = inf["nodes_per_layer"] * (inf["n_layers"] + 1) * int(pairs[i][1])
+        alg_bytes += r.sweeps * (inf["n_edges"] * 8 + vx * 4)
+    b_achieved = alg_bytes / (kms * 1e-3) / 1e9
     corrected = None
     if not args.no_corrected:
         corrected = batched_corrected(args, prob, step, cycles, lam.detach().cpu().numpy(), pairs, owner)
@@ -151,6 +160,10 @@ def batched(args):
         "batched_solver_kernel_ms": kms,
         "batched_gteps": relax / (kms * 1e-3) / 1e9,
         "batched_sweeps": sweeps,
+        # the solver launches of one step against the HBM roof; their time is
+        # the makespan of the longest item, while most CUs idle near the end
+        "batched_roofline": {"bound": "hbm", "achieved": b_achieved, "peak": 8000.0, "unit": "GB/s",
+                             "frac": b_achieved / 8000.0, "algorithmic_bytes": alg_bytes},
         "plain_loop_solves_per_s": n / p_el,
         "plain_loop_s": p_el,
         "batched_vs_plain": (n * args.batch_steps / b_el) / (n / p_el),
