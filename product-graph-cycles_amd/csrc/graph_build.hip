@@ -232,41 +232,76 @@ __global__ void k_select_init(SelectState* st, int64_t k0, int64_t k1) {
   }
 }
 
+// Histograms of the next digit of the keys that match each rank's prefix.
+// In the first passes most keys share their high digits: when every active
+// lane of a wave has the same digit, the wave adds its two match counts with
+// one LDS atomic each instead of one per lane (the same totals).
 __global__ void k_select_hist(const double* __restrict__ x, int64_t n, SelectState* st, int shift) {
   __shared__ uint32_t h[2][256];
   h[0][threadIdx.x] = 0;
   h[1][threadIdx.x] = 0;
   __syncthreads();
   const uint64_t p0 = st->prefix[0], m0 = st->mask[0], p1 = st->prefix[1], m1 = st->mask[1];
+  const int lane = threadIdx.x & 63;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t kk = dkey(x[i]);
     const uint32_t b = (uint32_t)(kk >> shift) & 255u;
-    if ((kk & m0) == p0) atomicAdd(&h[0][b], 1u);
-    if ((kk & m1) == p1) atomicAdd(&h[1][b], 1u);
+    const bool in0 = (kk & m0) == p0, in1 = (kk & m1) == p1;
+    const uint64_t act = __ballot(1);
+    const uint32_t bf = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+    if (__ballot(b == bf) == act) {
+      const uint64_t v0 = __ballot(in0), v1 = __ballot(in1);
+      if (lane == __ffsll((unsigned long long)act) - 1) {
+        if (v0) atomicAdd(&h[0][bf], (uint32_t)__popcll(v0));
+        if (v1) atomicAdd(&h[1][bf], (uint32_t)__popcll(v1));
+      }
+    } else {
+      if (in0) atomicAdd(&h[0][b], 1u);
+      if (in1) atomicAdd(&h[1][b], 1u);
+    }
   }
   __syncthreads();
   if (h[0][threadIdx.x]) atomicAdd(&st->hist[0][threadIdx.x], h[0][threadIdx.x]);
   if (h[1][threadIdx.x]) atomicAdd(&st->hist[1][threadIdx.x], h[1][threadIdx.x]);
 }
 
+// Picks each rank's digit: the first bin b with k < (keys in bins <= b), and
+// k becomes the rank inside that bin (bin 255 and k minus all the keys when
+// no bin qualifies, which a consistent histogram never gives). One thread per
+// bin: an inclusive scan of the 256 counts in LDS, then the one bin whose
+// range holds k writes the state. Launched with 256 threads.
 __global__ void k_select_step(SelectState* st, int shift) {
-  if (threadIdx.x < 2) {
-    const int t = threadIdx.x;
-    int64_t cum = 0;
-    int b = 0;
-    for (; b < 256; ++b) {
-      const int64_t c = st->hist[t][b];
-      if (st->k[t] < cum + c) break;
-      cum += c;
-    }
-    if (b > 255) b = 255;
-    st->k[t] -= cum;
-    st->prefix[t] |= ((uint64_t)b) << shift;
-    st->mask[t] |= ((uint64_t)255) << shift;
-    if (shift == 0) st->value[t] = dkey_inv(st->prefix[t]);
-  }
+  __shared__ int64_t sc[2][256];
+  const int b = threadIdx.x;
+  const int64_t c0 = st->hist[0][b], c1 = st->hist[1][b];
+  const int64_t k0 = st->k[0], k1 = st->k[1];
+  sc[0][b] = c0;
+  sc[1][b] = c1;
   __syncthreads();
-  for (int i = threadIdx.x; i < 512; i += blockDim.x) (&st->hist[0][0])[i] = 0;
+  for (int o = 1; o < 256; o <<= 1) {
+    const int64_t a0 = b >= o ? sc[0][b - o] : 0, a1 = b >= o ? sc[1][b - o] : 0;
+    __syncthreads();
+    sc[0][b] += a0;
+    sc[1][b] += a1;
+    __syncthreads();
+  }
+  // every read of st->k and st->hist is done (the scan's barriers) before
+  // the writes below
+  st->hist[0][b] = 0;
+  st->hist[1][b] = 0;
+  for (int t = 0; t < 2; ++t) {
+    const int64_t k = t ? k1 : k0;
+    const int64_t incl = sc[t][b], excl = incl - (t ? c1 : c0);
+    const bool hit = excl <= k && k < incl;
+    const bool none = b == 255 && k >= incl;
+    if (hit || none) {
+      const int64_t cum = hit ? excl : incl;
+      st->k[t] = k - cum;
+      st->prefix[t] |= ((uint64_t)b) << shift;
+      st->mask[t] |= ((uint64_t)255) << shift;
+      if (shift == 0) st->value[t] = dkey_inv(st->prefix[t]);
+    }
+  }
 }
 
 // col <- min(round(s*col), 100*scaling) with s = scaling / (2 max(median, 0.001))
