@@ -1263,6 +1263,28 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   struct Cyc {
     std::vector<int32_t> eid;
     std::vector<double> c, t;
+    std::vector<int32_t> rows;  // product-space rows, when the caller asked for them
+  };
+  // the m edges of the cycle in the job's buffers, with their rows gathered
+  // on the device from the same ids: one synchronization for all of it
+  auto read_cycle = [&](int m, Cyc& cy) -> int {
+    cy.eid.resize(m);
+    cy.c.resize(m);
+    cy.t.resize(m);
+    cy.rows.resize(rows ? (size_t)m * g->ps_cols : 0);
+    if (!m) return PGC_OK;
+    int32_t* d_out = nullptr;
+    if (rows) {
+      HIPCHK(dalloc(&d_out, (int64_t)m * g->ps_cols));
+      HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, s.neg_eid, m, d_out, st));
+      HIPCHK(hipMemcpyAsync(cy.rows.data(), d_out, (size_t)m * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipMemcpyAsync(cy.eid.data(), s.neg_eid, m * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cy.c.data(), s.neg_c, m * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(cy.t.data(), s.neg_t, m * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (d_out) cache_free(d_out);
+    return PGC_OK;
   };
   std::map<int, Cyc> validCycles;
   Cyc finalCyc;
@@ -1297,16 +1319,7 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
       heap.push(lbBranch, bid);
       valid[bid] = 1;
       Cyc cy;
-      const int m = std::max(0, r.neg_len - 1);
-      cy.eid.resize(m);
-      cy.c.resize(m);
-      cy.t.resize(m);
-      if (m) {
-        HIPCHK(hipMemcpyAsync(cy.eid.data(), s.neg_eid, m * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(cy.c.data(), s.neg_c, m * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(cy.t.data(), s.neg_t, m * 8, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-      }
+      if ((rc = read_cycle(std::max(0, r.neg_len - 1), cy))) return rc;
       validCycles.emplace(bid, std::move(cy));
       continue;
     }
@@ -1330,21 +1343,10 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
       if (h >= 0 && h < npl) heap.push(lbBranch, branch[h]);
   }
   if (!fromValid) {
-    const int m = std::max(0, (int)0);
-    (void)m;
     int32_t nl = 0;
     HIPCHK(hipMemcpyAsync(&nl, s.neg_len, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const int mm = std::max(0, nl - 1);
-    finalCyc.eid.resize(mm);
-    finalCyc.c.resize(mm);
-    finalCyc.t.resize(mm);
-    if (mm) {
-      HIPCHK(hipMemcpyAsync(finalCyc.eid.data(), s.neg_eid, mm * 4, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(finalCyc.c.data(), s.neg_c, mm * 8, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(finalCyc.t.data(), s.neg_t, mm * 8, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-    }
+    if ((rc = read_cycle(std::max(0, nl - 1), finalCyc))) return rc;
   }
   // output (:857-862): ratio of the cycle and its product-space rows
   double cs = 0, ts = 0;
@@ -1355,15 +1357,7 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   const int64_t n = (int64_t)finalCyc.eid.size();
   if (n > capacity) return fail(PGC_ERR_CAPACITY, "output capacity too small: need " + std::to_string(n));
   if (edge_ids) std::copy(finalCyc.eid.begin(), finalCyc.eid.end(), edge_ids);
-  if (rows && n) {
-    HIPCHK(hipMemcpyAsync(s.eids_io, finalCyc.eid.data(), n * 4, hipMemcpyHostToDevice, st));
-    int32_t* d_out = nullptr;
-    HIPCHK(dalloc(&d_out, n * g->ps_cols));
-    HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, s.eids_io, n, d_out, st));
-    HIPCHK(hipMemcpyAsync(rows, d_out, n * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    cache_free(d_out);
-  }
+  if (rows) std::copy(finalCyc.rows.begin(), finalCyc.rows.end(), rows);
   res->passes_layers_once = 1;
   res->objective = (float)(cs / ts);
   res->n_edges = n;
