@@ -272,6 +272,7 @@ struct pgc_graph {
   int32_t width[2] = {0, 0};
   int32_t lstride[2] = {0, 0};
   int32_t src_inv[2] = {0, 0};  // per class: source slots are the same on every layer
+  int32_t cls_deg[2] = {1 << 30, 1 << 30};  // per class: largest in-degree (unknown: above every width)
   int32_t maxw = 0;
   int32_t max_in_degree = 0;
   int32_t* d_cnt = nullptr;
@@ -427,6 +428,8 @@ int build_layout(pgc_graph* g) {
   g->width[0] = W;
   g->width[1] = W;
   g->max_in_degree = mx;
+  g->cls_deg[0] = wn[0];
+  g->cls_deg[1] = wn[1];
   g->intra_slots_valid = wn[2];
   g->inter_slots_valid = wn[3];
   HIPCHK(hipMemcpyAsync(g->d_width, g->width, 2 * 4, hipMemcpyHostToDevice, st));
@@ -652,6 +655,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
     S.w = g->width[c];
     S.lstride = g->lstride[c];
     S.src_inv = g->src_inv[c];
+    S.deg = g->cls_deg[c];
     S.src = g->d_s[c];
     S.cost = g->d_cost[c];
     S.time = g->d_time[c];
@@ -1193,6 +1197,7 @@ int pgc_graph_derive(pgc_graph* base, const double* feat_diff, int on_device, pg
     g->width[c] = b->width[c];
     g->lstride[c] = b->lstride[c];
     g->src_inv[c] = b->src_inv[c];
+    g->cls_deg[c] = b->cls_deg[c];
   }
   g->d_width = b->d_width;
   g->maxw = b->maxw;

@@ -33,6 +33,8 @@ struct SlotClass {
   int32_t src_inv;       // 1: the source slots repeat on every layer: the intralayer class on
                          // layers 0..L-1 (layer L has no intralayer edges), the interlayer
                          // class on layers 1..L (the same X adjacency on every layer)
+  int32_t deg;           // the largest in-degree of the class: slots deg..w-1 are padding on
+                         // every node
 };
 
 // Conjugate energy (cost, time) kinds of the device kernel.

@@ -314,14 +314,19 @@ struct Solver {
       r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true, (W == 8 && MAXC <= (HAS_TIME ? 2 : 3) ? MAXC : 0)>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
                                                               predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
                                                               ratio, ub);
+    else if (K != 1)
+      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+                                                       predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0, ratio,
+                                                       ub);
+    else if (J.inter.deg <= W - 1)  // the last interlayer slot is padding on every node
+      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W, true, W - 1>((const SolverJob*)&J, D, Dmin, &sh,
+                                                                    (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,
+                                                                    bor.prev1, bor.odd ? 1 : 0, ratio, ub,
+                                                                    (int32_t*)OrgL, orgOn ? 1 : 0);
     else
-      r = K == 1 ? sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W, true>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
-                                                                     predLds ? 1 : 0, bor.prev0, bor.prev1,
-                                                                     bor.odd ? 1 : 0, ratio, ub, (int32_t*)OrgL,
-                                                                     orgOn ? 1 : 0)
-                 : sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
-                                                               predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
-                                                               ratio, ub);
+      r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W, true>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+                                                             predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
+                                                             ratio, ub, (int32_t*)OrgL, orgOn ? 1 : 0);
     bor.prev0 = r.p0;
     bor.prev1 = r.p1;
     bor.odd = r.odd != 0;

@@ -63,9 +63,12 @@ struct SweepState {
 
 // ONLY_K1: the instance for jobs with K == 1 (interlayer blocks only). The K == 1
 // loops and the general loop are separate functions, so that each gets its
-// own register allocation.
-template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool ONLY_K1 = false>
+// own register allocation. WE: interlayer slots read per node (< W when the
+// class's largest in-degree leaves the last slots padding on every node; a
+// padding slot never wins, so skipping it changes no result).
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool ONLY_K1 = false, int WE = W>
 struct Sweep {
+  static_assert(WE == W || ONLY_K1, "short interlayer slots: K == 1 loops only");
   const PGC_GLOBAL SolverJob* J;
   PGC_LDS double* D;     // [2K][ds] distance rows; entry npl of every row is +inf
   PGC_LDS double* Dmin;  // [ds] minimum over the depths of the previous layer
@@ -122,7 +125,7 @@ struct Sweep {
     const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)cost_);
     const PGC_GLOBAL float* __restrict__ tim = gp(time_);
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < WE; ++j) {
       s[j] = src[b + j * 64];
       c[j] = cost[b + j * 64];
       if (HAS_TIME) t[j] = tim[b + j * 64];
@@ -151,9 +154,9 @@ struct Sweep {
     // first-minimum scan in slot order
     double dv[W];
 #pragma unroll
-    for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
+    for (int j = 0; j < WE; ++j) dv[j] = Dp[s[j]];
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < WE; ++j) {
       const double nv = dv[j] + weight(c[j], t[j], ratio, rt);
       const bool better = nv < best;
       best = pgc_min(best, nv);
@@ -314,7 +317,7 @@ struct Sweep {
       const int ci = chunk_of(i);
       const int base = layer * Sx.lstride + lane + (ci < nC ? ci * W * 64 : 0);
 #pragma unroll
-      for (int j = 0; j < W; ++j) {
+      for (int j = 0; j < WE; ++j) {
         b.c[i][j] = cost[base + j * 64];
         if (HAS_TIME) b.t[i][j] = tim[base + j * 64];
       }
@@ -341,7 +344,7 @@ struct Sweep {
         const int ci = chunk_of(i);
         const int base = Sx.lstride + lane + (ci < nC ? ci * W * 64 : 0);
 #pragma unroll
-        for (int j = 0; j < W; ++j) s[i][j] = src[base + j * 64];
+        for (int j = 0; j < WE; ++j) s[i][j] = src[base + j * 64];
       }
     }
     SlotsCT A, B;
@@ -482,11 +485,11 @@ struct Sweep {
 
 };
 
-template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool ONLY_K1 = false>
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool ONLY_K1 = false, int WE = W>
 __device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                 int32_t* predLg, int predLds, int p0, int p1, int odd, double ratio,
                                                 double ub, int32_t* orgLg = nullptr, int orgOn = 0) {
-  Sweep<CT, HAS_TIME, MINCOST, MAXC, W, ONLY_K1> w;
+  Sweep<CT, HAS_TIME, MINCOST, MAXC, W, ONLY_K1, WE> w;
   w.orgOn = ONLY_K1 ? uni(orgOn) : 0;
   w.orgL = (PGC_LDS int32_t*)uni(orgLg);
   w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
