@@ -310,10 +310,19 @@ struct Solver {
     if constexpr (DYN)
       r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, nullptr, 0, bor.prev0,
                                                         bor.prev1, bor.odd ? 1 : 0, ratio, ub);
-    else if constexpr (!UNROLLED)
-      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true, (W == 8 && MAXC <= (HAS_TIME ? 2 : 3) ? MAXC : 0)>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
-                                                              predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
-                                                              ratio, ub);
+    else if constexpr (!UNROLLED) {
+      constexpr int MC = W == 8 && MAXC <= (HAS_TIME ? 2 : 3) ? MAXC : 0;
+      // icosphere-like graphs: interlayer in-degree <= 7 and intralayer <= 6
+      // leave the last one and two slots padding on every node
+      if (MC > 0 && J.inter.deg <= W - 1 && J.intra.deg <= W - 2)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true, MC, (MC > 0 ? W - 1 : W), (MC > 0 ? W - 2 : W)>(
+            (const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl, predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
+            ratio, ub);
+      else
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true, MC>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+                                                                    predLds ? 1 : 0, bor.prev0, bor.prev1,
+                                                                    bor.odd ? 1 : 0, ratio, ub);
+    }
     else if (K != 1)
       r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
                                                        predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0, ratio,

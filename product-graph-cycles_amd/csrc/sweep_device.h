@@ -542,8 +542,12 @@ struct DistPtr<true> {
 // MC > 0: at most MC chunks per wave (rows in LDS); with layer-invariant
 // sources in both classes, the sources of the owned chunks are loaded once
 // per pass and held in registers, and a step loads only costs and times.
-template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0>
+// WX, WI: interlayer and intralayer slots that step reads per node (below W
+// when the class's largest in-degree leaves the last slots padding on every
+// node; a padding slot never wins, so skipping it changes no result).
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W>
 struct SweepG {
+  static_assert((WX == W && WI == W) || MC > 0, "short slots: cached-source steps only");
   typedef typename DistPtr<LD>::type DP;
   const PGC_GLOBAL SolverJob* J;
   DP D;
@@ -719,16 +723,17 @@ struct SweepG {
   }
 
   // --- steps with the sources held in registers (MC > 0)
-  __device__ __forceinline__ void load_ct(const SlotClass& S, int b, CT (&c)[W], float (&t)[W]) const {
+  template <int N>
+  __device__ __forceinline__ void load_ct(const SlotClass& S, int b, CT (&c)[N], float (&t)[N]) const {
     const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)S.cost);
     const PGC_GLOBAL float* __restrict__ tim = gp(S.time);
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < N; ++j) {
       c[j] = cost[b + j * 64];
       if (HAS_TIME) t[j] = tim[b + j * 64];
     }
   }
-  __device__ __forceinline__ bool inter_cached(int layer, int cur, int prev, const int (&sX)[MC > 0 ? MC : 1][W],
+  __device__ __forceinline__ bool inter_cached(int layer, int cur, int prev, const int (&sX)[MC > 0 ? MC : 1][WX],
                                                double ratio, double rt, double ub) {
     bool imp = false;
     const DP Dp = (K == 1) ? D + prev * ds : Dmin;
@@ -736,21 +741,21 @@ struct SweepG {
     for (int i = 0; i < MC; ++i) {
       const int c = wave + i * nw;
       if (c >= nC) break;
-      CT cc[W];
-      float t[W];
+      CT cc[WX];
+      float t[WX];
       load_ct(Sx, layer * Sx.lstride + c * W * 64 + lane, cc, t);
       const int v = c * 64 + lane;
       if (v >= npl) continue;
-      const int(&s)[W] = sX[i];
-      double dv[W];
+      const int(&s)[WX] = sX[i];
+      double dv[WX];
 #pragma unroll
-      for (int j = 0; j < W; ++j) dv[j] = Dp[s[j]];
+      for (int j = 0; j < WX; ++j) dv[j] = Dp[s[j]];
       double best = PGC_INF;
       int bs = npl;
       CT bc = 0;
       float bt = 0;
 #pragma unroll
-      for (int j = 0; j < W; ++j) {
+      for (int j = 0; j < WX; ++j) {
         const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
         const bool better = nv < best;
         best = pgc_min(best, nv);
@@ -783,7 +788,7 @@ struct SweepG {
     }
     return imp;
   }
-  __device__ __forceinline__ bool intra_cached(int layer, int cur, int k, const int (&sI)[MC > 0 ? MC : 1][W],
+  __device__ __forceinline__ bool intra_cached(int layer, int cur, int k, const int (&sI)[MC > 0 ? MC : 1][WI],
                                                double ratio, double rt, double ub) {
     bool imp = false;
     const DP Dk = D + (cur + k - 1) * ds;
@@ -791,19 +796,19 @@ struct SweepG {
     for (int i = 0; i < MC; ++i) {
       const int c = wave + i * nw;
       if (c >= nC) break;
-      CT cc[W];
-      float t[W];
+      CT cc[WI];
+      float t[WI];
       load_ct(Sa, layer * Sa.lstride + c * W * 64 + lane, cc, t);
       const int v = c * 64 + lane;
       if (v >= npl) continue;
-      const int(&s)[W] = sI[i];
-      double dv[W];
+      const int(&s)[WI] = sI[i];
+      double dv[WI];
 #pragma unroll
-      for (int j = 0; j < W; ++j) dv[j] = Dk[s[j]];
+      for (int j = 0; j < WI; ++j) dv[j] = Dk[s[j]];
       double best = PGC_INF;
       int bs = npl;
 #pragma unroll
-      for (int j = 0; j < W; ++j) {
+      for (int j = 0; j < WI; ++j) {
         const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
         const bool better = nv < best;
         best = pgc_min(best, nv);
@@ -835,7 +840,7 @@ struct SweepG {
   }
   __device__ __forceinline__ int run_layers_cached(double ratio, double rt, double ub) {
     constexpr int M = MC > 0 ? MC : 1;
-    int sX[M][W], sI[M][W];
+    int sX[M][WX], sI[M][WI];
     {
       const PGC_GLOBAL int* __restrict__ srcX = gp(Sx.src);
       const PGC_GLOBAL int* __restrict__ srcI = gp(Sa.src);
@@ -844,10 +849,9 @@ struct SweepG {
         const int c = wave + i * nw;
         const int o = (c < nC ? c : 0) * W * 64 + lane;
 #pragma unroll
-        for (int j = 0; j < W; ++j) {
-          sX[i][j] = srcX[Sx.lstride + o + j * 64];  // interlayer pattern of layer 1
-          sI[i][j] = K > 1 ? srcI[o + j * 64] : 0;   // intralayer pattern of layer 0
-        }
+        for (int j = 0; j < WX; ++j) sX[i][j] = srcX[Sx.lstride + o + j * 64];  // interlayer pattern of layer 1
+#pragma unroll
+        for (int j = 0; j < WI; ++j) sI[i][j] = K > 1 ? srcI[o + j * 64] : 0;  // intralayer pattern of layer 0
       }
     }
     for (int layer = 0; layer <= L; ++layer) {
@@ -916,11 +920,11 @@ struct SweepG {
   }
 };
 
-template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0>
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W>
 __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                        int32_t* predLg, int predLds, int p0, int p1, int odd,
                                                        double ratio, double ub) {
-  typedef SweepG<CT, HAS_TIME, MINCOST, W, LD, MC> SW;
+  typedef SweepG<CT, HAS_TIME, MINCOST, W, LD, MC, WX, WI> SW;
   SW w;
   w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
   w.D = (typename SW::DP)uni(Dg);
