@@ -307,9 +307,18 @@ struct Solver {
   // gets the whole register file; the caller's state is saved once per sweep).
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     SweepState r;
-    if constexpr (DYN)
-      r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, nullptr, 0, bor.prev0,
-                                                        bor.prev1, bor.odd ? 1 : 0, ratio, ub);
+    if constexpr (DYN) {
+      // short slots for the interlayer / intralayer classes: 7 / 6 of 8 (icosphere
+      // product graphs), 12 / 6 of 16 (conjugate product graphs: in-degrees <= 10 / 5)
+      constexpr int SX = W == 8 ? 7 : (W == 16 ? 12 : W), SI = W == 8 || W == 16 ? 6 : W;
+      if (SX < W && J.inter.deg <= SX && J.intra.deg <= SI)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 0, SX, SI>((const SolverJob*)&J, D, Dmin, &sh,
+                                                                            nullptr, 0, bor.prev0, bor.prev1,
+                                                                            bor.odd ? 1 : 0, ratio, ub);
+      else
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W>((const SolverJob*)&J, D, Dmin, &sh, nullptr, 0, bor.prev0,
+                                                          bor.prev1, bor.odd ? 1 : 0, ratio, ub);
+    }
     else if constexpr (!UNROLLED) {
       constexpr int MC = W == 8 && MAXC <= (HAS_TIME ? 2 : 3) ? MAXC : 0;
       // icosphere-like graphs: interlayer in-degree <= 7 and intralayer <= 6

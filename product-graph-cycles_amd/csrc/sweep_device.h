@@ -542,12 +542,12 @@ struct DistPtr<true> {
 // MC > 0: at most MC chunks per wave (rows in LDS); with layer-invariant
 // sources in both classes, the sources of the owned chunks are loaded once
 // per pass and held in registers, and a step loads only costs and times.
-// WX, WI: interlayer and intralayer slots that step reads per node (below W
+// WX, WI: interlayer and intralayer slots a step reads per node (below W
 // when the class's largest in-degree leaves the last slots padding on every
 // node; a padding slot never wins, so skipping it changes no result).
 template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W>
 struct SweepG {
-  static_assert((WX == W && WI == W) || MC > 0, "short slots: cached-source steps only");
+  static_assert(WX <= W && WI <= W, "short slots");
   typedef typename DistPtr<LD>::type DP;
   const PGC_GLOBAL SolverJob* J;
   DP D;
@@ -583,12 +583,13 @@ struct SweepG {
     return __dsub_rn((double)c, HAS_TIME ? __dmul_rn(ratio, (double)t) : rt);
   }
 
-  __device__ __forceinline__ void load(const SlotClass& S, int b, int (&s)[W], CT (&c)[W], float (&t)[W]) const {
+  template <int N>
+  __device__ __forceinline__ void load(const SlotClass& S, int b, int (&s)[N], CT (&c)[N], float (&t)[N]) const {
     const PGC_GLOBAL int* __restrict__ src = gp(S.src);
     const PGC_GLOBAL CT* __restrict__ cost = gp((const CT*)S.cost);
     const PGC_GLOBAL float* __restrict__ tim = gp(S.time);
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < N; ++j) {
       s[j] = src[b + j * 64];
       c[j] = cost[b + j * 64];
       if (HAS_TIME) t[j] = tim[b + j * 64];
@@ -597,15 +598,16 @@ struct SweepG {
 
   // First-minimum scan and write-back of one interlayer vertex v whose slots
   // and gathered source distances are loaded.
-  __device__ __forceinline__ bool inter_relax(int layer, int cur, int prev, int v, const int (&s)[W],
-                                              const CT (&cc)[W], const float (&t)[W], const double (&dv)[W],
+  template <int N>
+  __device__ __forceinline__ bool inter_relax(int layer, int cur, int prev, int v, const int (&s)[N],
+                                              const CT (&cc)[N], const float (&t)[N], const double (&dv)[N],
                                               double ratio, double rt, double ub) {
     double best = PGC_INF;
     int bs = npl;
     CT bc = 0;
     float bt = 0;
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < N; ++j) {
       const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
       const bool better = nv < best;
       best = pgc_min(best, nv);
@@ -640,13 +642,14 @@ struct SweepG {
   }
 
   // The same for an intralayer vertex at depth k (and Dmin at the last depth).
-  __device__ __forceinline__ bool intra_relax(int layer, int cur, int k, int v, const int (&s)[W],
-                                              const CT (&cc)[W], const float (&t)[W], const double (&dv)[W],
+  template <int N>
+  __device__ __forceinline__ bool intra_relax(int layer, int cur, int k, int v, const int (&s)[N],
+                                              const CT (&cc)[N], const float (&t)[N], const double (&dv)[N],
                                               double ratio, double rt, double ub) {
     double best = PGC_INF;
     int bs = npl;
 #pragma unroll
-    for (int j = 0; j < W; ++j) {
+    for (int j = 0; j < N; ++j) {
       const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
       const bool better = nv < best;
       best = pgc_min(best, nv);
@@ -682,26 +685,28 @@ struct SweepG {
   // faster, and one chunk keeps the registers free.
   static constexpr int NG = 1;
 
-  // One interlayer (k == 0) or intralayer (depth k) step over the wave's chunks.
+  // One interlayer (k == 0, N = WX slots) or intralayer (depth k, N = WI)
+  // step over the wave's chunks.
+  template <int N>
   __device__ __forceinline__ bool step_n(int layer, int cur, int prev, int k, double ratio, double rt, double ub) {
     bool imp = false;
     const bool inter = k == 0;
     const SlotClass& S = inter ? Sx : Sa;
     const DP Dg = inter ? ((K == 1) ? D + prev * ds : Dmin) : D + (cur + k - 1) * ds;
     for (int c0 = wave; c0 < nC; c0 += NG * nw) {
-      int s[NG][W];
-      CT cc[NG][W];
-      float t[NG][W];
+      int s[NG][N];
+      CT cc[NG][N];
+      float t[NG][N];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const int c = c0 + g * nw;
         load(S, layer * S.lstride + (c < nC ? c : c0) * W * 64 + lane, s[g], cc[g], t[g]);
       }
-      double dv[NG][W];
+      double dv[NG][N];
 #pragma unroll
       for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int j = 0; j < W; ++j) dv[g][j] = Dg[s[g][j]];
+        for (int j = 0; j < N; ++j) dv[g][j] = Dg[s[g][j]];
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const int c = c0 + g * nw;
@@ -716,10 +721,10 @@ struct SweepG {
     return imp;
   }
   __device__ __forceinline__ bool inter_step(int layer, int cur, int prev, double ratio, double rt, double ub) {
-    return step_n(layer, cur, prev, 0, ratio, rt, ub);
+    return step_n<WX>(layer, cur, prev, 0, ratio, rt, ub);
   }
   __device__ __forceinline__ bool intra_step(int layer, int cur, int k, double ratio, double rt, double ub) {
-    return step_n(layer, cur, K - cur, k, ratio, rt, ub);
+    return step_n<WI>(layer, cur, K - cur, k, ratio, rt, ub);
   }
 
   // --- steps with the sources held in registers (MC > 0)
