@@ -222,11 +222,17 @@ class Graph:
         return Graph(self.ctx, h)
 
     def info(self):
-        e, npl, nl = C.c_int64(), C.c_int64(), C.c_int64()
-        pc, cc = C.c_int32(), C.c_int32()
-        check(load().pgc_graph_info(self._h, C.byref(e), C.byref(pc), C.byref(cc), C.byref(npl), C.byref(nl)))
-        return dict(n_edges=e.value, ps_cols=pc.value, n_cost_cols=cc.value, nodes_per_layer=npl.value,
-                    n_layers=nl.value)
+        # the sizes are fixed when the graph is built: read them once (batched
+        # solves ask for every item's sizes on every call)
+        cached = getattr(self, "_info", None)
+        if cached is None or cached[0] != self._h.value:
+            e, npl, nl = C.c_int64(), C.c_int64(), C.c_int64()
+            pc, cc = C.c_int32(), C.c_int32()
+            check(load().pgc_graph_info(self._h, C.byref(e), C.byref(pc), C.byref(cc), C.byref(npl), C.byref(nl)))
+            cached = (self._h.value, dict(n_edges=e.value, ps_cols=pc.value, n_cost_cols=cc.value,
+                                          nodes_per_layer=npl.value, n_layers=nl.value))
+            self._info = cached
+        return dict(cached[1])
 
     def product_space(self):
         i = self.info()
