@@ -87,7 +87,7 @@ def test_small_cases_with_global_table_forced():
     env = dict(os.environ, PGC_GLOBAL_TABLE="1")
     cmd = [sys.executable, "-m", "pytest", "-x", "-q", "-m", "gpu", "-p", "no:cacheprovider",
            os.path.join(HERE, "test_gpu_parity.py"), os.path.join(HERE, "test_gpu_batch.py"),
-           "-k", "tiny or meshes or modes or pruned or matches_single or requantizes or sweep_cap"]
+           "-k", "tiny or meshes or modes or pruned or matches_single or requantizes or sweep_cap or full_slot_width"]
     # output goes straight to this process's stdout (progress stays visible)
     r = subprocess.run(cmd, env=env, cwd=os.path.dirname(HERE), timeout=900)
     assert r.returncode == 0

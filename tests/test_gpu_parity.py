@@ -211,3 +211,42 @@ def test_ratio_random_configurations(i, sub, ny, K, mode, prune, corrected, seed
 @pytest.mark.parametrize("name", list(MINCOST))
 def test_mincost_random_configurations(i, sub, ny, K, mode, prune, corrected, seed, name):
     _check_mincost(S.make_pair(sub, ny, seed=seed), K, name, prune=prune, corrected=corrected)
+
+
+# Graphs whose in-degrees reach the slot width, where the sweeps read every
+# slot (the icosphere and ring cases above skip the slots that are padding on
+# every node): complete X graphs (8 vertices: interlayer in-degree 8 = the
+# width; 13 vertices: 13 of 16), and an icosphere with one added edge (a
+# vertex of valence 7, interlayer in-degree 8) large enough for the
+# several-chunks-per-wave sweep.
+WIDE = [(s, nx, ny, K) for s, (nx, ny) in enumerate([(8, 5), (13, 4)]) for K in (1, 2, 3)]
+
+
+def _with_extra_edge(p):
+    EX = np.asarray(p["EX"])
+    deg = np.bincount(EX[:, 0], minlength=len(p["VX"]))
+    a = int(np.argmax(deg))
+    nb = set(EX[EX[:, 0] == a, 1].tolist())
+    b = next(v for v in range(len(p["VX"])) if v != a and v not in nb and deg[v] == deg.max())
+    EX2 = np.unique(np.vstack([EX, [[a, b], [b, a]]]).astype(np.int32), axis=0)
+    return dict(p, EX=EX2)
+
+
+@pytest.mark.parametrize("seed,nx,ny,K", WIDE)
+@pytest.mark.parametrize("name", list(LAWLER))
+def test_ratio_full_slot_width(seed, nx, ny, K, name):
+    _check_ratio(S.tiny_pair(nx, ny, seed=seed + 60, ring=False), K, name)
+
+
+@pytest.mark.parametrize("seed,nx,ny,K", WIDE)
+@pytest.mark.parametrize("name", list(MINCOST))
+def test_mincost_full_slot_width(seed, nx, ny, K, name):
+    _check_mincost(S.tiny_pair(nx, ny, seed=seed + 60, ring=False), K, name)
+
+
+@pytest.mark.parametrize("K", [1, 2, 3])
+def test_full_slot_width_many_chunks(K):
+    p = _with_extra_edge(S.make_pair(4, 8, seed=5))
+    assert np.bincount(p["EX"][:, 0]).max() == 7
+    _check_ratio(p, K, "lawlergpu")
+    _check_mincost(p, K, "dijkstragpu")
