@@ -801,6 +801,20 @@ hipError_t launch_energy_batch(const EItem* d_items, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
+// grid.y = item: fills of many device arrays (the predecessor tables of a
+// batch's jobs) in one launch instead of one memset call each
+__global__ void k_bfill_i32(const FillItem* __restrict__ items, int32_t value) {
+  const FillItem it = items[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < it.n; i += (int64_t)gridDim.x * blockDim.x)
+    it.p[i] = value;
+}
+
+hipError_t launch_fill_i32_batch(const FillItem* d_items, int n, int32_t value, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bfill_i32, dim3(64, n), dim3(256), 0, st, d_items, value);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_costs_batch(const FItem* d_items, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bfill_costs, dim3(64, n), dim3(256), 0, st, d_items);

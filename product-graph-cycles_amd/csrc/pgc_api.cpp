@@ -37,6 +37,7 @@ hipError_t launch_energy(const double*, const double*, const int32_t*, int64_t, 
 hipError_t launch_quantize(double*, int64_t, void*, hipStream_t);
 hipError_t launch_quantize_batch(const QItem*, int, void*, void*, double*, hipStream_t);
 hipError_t launch_fill_costs_batch(const FItem*, int, hipStream_t);
+hipError_t launch_fill_i32_batch(const FillItem*, int, int32_t, hipStream_t);
 hipError_t launch_energy_batch(const EItem*, int, hipStream_t);
 size_t select_state_bytes();
 size_t bracket_state_bytes();
@@ -208,6 +209,8 @@ struct pgc_context {
   int64_t qitems_cap = 0;
   pgc::FItem* d_fitems = nullptr;
   int64_t fitems_cap = 0;
+  pgc::FillItem* d_fills = nullptr;  // batched -1 fills of global predecessor tables
+  int64_t fills_cap = 0;
   pgc::EItem* d_eitems = nullptr;  // batched energies
   int64_t eitems_cap = 0;
   // launch groups of a batch run concurrently on these streams (default priority)
@@ -358,7 +361,7 @@ void pgc_context_destroy(pgc_context* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values,
-                c->d_qitems, c->d_fitems, c->d_eitems, c->d_select, c->d_bracket, c->d_cand};
+                c->d_qitems, c->d_fitems, c->d_eitems, c->d_select, c->d_bracket, c->d_cand, c->d_fills};
   for (void* p : ps)
     if (p) cache_free(p);
   for (int k = 0; k < 4; ++k) {
@@ -1454,6 +1457,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   std::vector<SolverJob> jobs(n);
   std::vector<QItem> qitems;
   std::vector<FItem> fitems;
+  std::vector<FillItem> fills;
   for (int64_t i = 0; i < n; ++i) {
     pgc_graph* g = graphs[i];
     const int K = max_depth[i];
@@ -1469,8 +1473,9 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     if ((rc = ensure_bnb(g))) return rc;
     // a batch job is one launch: the kernel starts it with an empty cycle and,
     // for a predecessor table in LDS, with -1 predecessors (no global table to
-    // reset, load or write back)
-    if (!pred_fits_lds(g, K)) HIPCHK(hipMemsetAsync(g->js.pred, 0xff, g->js.predN * 4, st));
+    // reset, load or write back); global tables are reset by one fill launch
+    // for the whole batch
+    if (!pred_fits_lds(g, K)) fills.push_back({g->js.pred, g->js.predN});
     jobs[i] = make_job(g, K, MODE_FULL_RATIO, flags);
     jobs[i].flags |= FLAG_FRESH_JOB;
   }
@@ -1502,6 +1507,13 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     }
     HIPCHK(hipMemcpyAsync(ctx->d_qitems, qitems.data(), qitems.size() * sizeof(QItem), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(ctx->d_fitems, fitems.data(), fitems.size() * sizeof(FItem), hipMemcpyHostToDevice, st));
+    if (!fills.empty()) {
+      if ((rc = grow(&ctx->d_fills, ctx->fills_cap, (int64_t)fills.size()))) return rc;
+      HIPCHK(hipMemcpyAsync(ctx->d_fills, fills.data(), fills.size() * sizeof(FillItem), hipMemcpyHostToDevice, st));
+      for (int64_t a = 0; a < (int64_t)fills.size(); a += chunk)
+        HIPCHK(launch_fill_i32_batch(ctx->d_fills + a, (int)std::min<int64_t>(chunk, (int64_t)fills.size() - a), -1,
+                                     st));
+    }
     for (int64_t a = 0; a < (int64_t)qitems.size(); a += chunk)
       HIPCHK(launch_quantize_batch(ctx->d_qitems + a, (int)std::min<int64_t>(chunk, (int64_t)qitems.size() - a),
                                    ctx->d_select, ctx->d_bracket, ctx->d_cand, st));

@@ -161,6 +161,12 @@ struct CycleItem {
   double gout;          // backward: upstream gradient
 };
 
+// One int32 array of a batched fill: p[0, n) <- the launch's value.
+struct FillItem {
+  int32_t* p;
+  int64_t n;
+};
+
 // One column of a batched quantization. Its median bracket's candidates go
 // to cand[cand_off, cand_off + cand_cap) of the batch's candidate buffer.
 struct QItem {
