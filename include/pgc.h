@@ -56,6 +56,8 @@ typedef struct pgc_result {
   double total_ms;            /* wall time of the call */
   double phase_ms[5];         /* device time inside the solver kernels by phase: layer sweeps,
                                  closing-layer fold, wrap chase, cycle walks, cycle extraction */
+  int32_t status;             /* PGC_OK, or this solve's own error (batched solves: per graph) */
+  int32_t reserved;
 } pgc_result;
 
 const char* pgc_last_error(void);
@@ -127,7 +129,14 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int max_depth, uint32_t flag
  * reference has no batched entry point. max_depth[i] is graph i's depth; graph
  * i's cycle goes to edge_ids[offsets[i] .. offsets[i+1]) and, if rows != NULL,
  * its product-space rows to rows + (offsets[i] - offsets[0]) * ps_cols (all
- * graphs must then have the same ps_cols). A graph may appear once per batch. */
+ * graphs must then have the same ps_cols). A graph may appear once per batch.
+ * Argument errors and graphs that cannot be solved fail the call before any
+ * energy is quantized. A graph whose device branch and bound outgrows its
+ * store of valid-branch cycles (64) is solved again by the host branch and
+ * bound of pgc_solve_ratio on the same quantized costs (same result). Every
+ * res[i].status holds graph i's own outcome; the call returns PGC_OK when all
+ * are PGC_OK, else the first failing graph's status, and the graphs that
+ * solved still have their cycles and results written. */
 int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int32_t* max_depth, int64_t n,
                           uint32_t flags, int32_t* edge_ids, int32_t* rows, const int64_t* offsets,
                           pgc_result* res);

@@ -28,6 +28,8 @@ class BnbHeap {
     return slots_[where_[key]].first;
   }
 
+  bool contains(long key) const { return key >= 0 && key < cap_ && where_[key] != -1; }
+
   // helper/minheap.cpp:137-159
   void push(double value, long key) {
     if (key < 0 || key >= cap_ || where_[key] != -1 || count_ == cap_) return;

@@ -118,6 +118,24 @@ enum BFStatus { NO_NEGATIVE_CYCLE = 2, FOUND_NEGATIVE_CYCLE = 3, ZERO_CYCLE = 4 
 
 struct Stats {
   long bf_calls = 0, sweeps = 0, lawler_iters = 0, branches = 0, relaxations = 0;
+  // Events of the ratio branch and bound at which the reference's algorithm
+  // can end above the once-wrapping optimum (counted, never changed):
+  long q_push_id = 0;       // a push with a branch id outside [0, npl): the heap ignores it
+                            // (helper/minheap.cpp:137-139; branch ids grow past npl, Q9)
+  long q_push_present = 0;  // a push of a branch id already queued: ignored, its key is kept
+  long q_single = 0;        // a multi-wrap branch with one vertex is dropped
+                            // (cost_time_ratio_solver_cuda.cpp:294-297)
+  long q_rise = 0;          // Lawler stopped on a rising ratio (:620-623): its cycle is not the
+                            // branch's best one found
+  long q_stale = 0;         // a non-root branch's first Bellman-Ford call found no negative
+                            // cycle: the branch keeps lambda_UB and the previous branch's cycle
+  long q_cap = 0;           // a Bellman-Ford call stopped at the sweep cap numNodes - 1
+  long q_walk = 0;          // the first closing candidate walk gave a cycle whose reweighted
+                            // cost is not negative, and the call stopped there (:506-518)
+  long q_empty = 0;         // the queue ran empty: the output is the last Lawler cycle
+  long q_maxiter = 0;       // Lawler's iteration cap (:624-627)
+  long q_ub_break = 0;      // the lambda_UB < peak stop (:282-286), lambda_UB never lowered (Q8)
+  long valid_pops = 0;      // the stop on a popped branch whose stored cycle is valid
 };
 
 // ---------------------------------------------------------------------------
@@ -373,13 +391,17 @@ struct CtrSolver {
           negCycle = cyc;
           return FOUND_NEGATIVE_CYCLE;
         }
+        st.q_walk++;
         // a zero-cost cycle leaves the sweep loop through its condition and
         // reaches the final NO_NEGATIVE_CYCLE return (:506-518, :282)
         if (w == 0) return NO_NEGATIVE_CYCLE;
         return zero_or_none();
       }
       iter++;
-      if (iter >= S.numNodes - 1) return NO_NEGATIVE_CYCLE;
+      if (iter >= S.numNodes - 1) {
+        st.q_cap++;
+        return NO_NEGATIVE_CYCLE;
+      }
     }
   }
 
@@ -390,12 +412,21 @@ struct CtrSolver {
     while (true) {
       st.lawler_iters++;
       const BFStatus s = bellman_ford(false, false);
-      if (s == ZERO_CYCLE || s == NO_NEGATIVE_CYCLE) break;
+      if (s == ZERO_CYCLE || s == NO_NEGATIVE_CYCLE) {
+        if (i == 0 && s == NO_NEGATIVE_CYCLE && branchId != 0) st.q_stale++;
+        break;
+      }
       S.ratio = ratio_of(negCycle).ratio;
       if (std::fabs(oldRatio - S.ratio) < tol) break;
-      if (S.ratio - tol > oldRatio) break;
+      if (S.ratio - tol > oldRatio) {
+        st.q_rise++;
+        break;
+      }
       oldRatio = S.ratio;
-      if (i > 1000) break;
+      if (i > 1000) {
+        st.q_maxiter++;
+        break;
+      }
       i++;
     }
     lbCurrentBranch = S.ratio;
@@ -480,30 +511,53 @@ struct CtrSolver {
     std::vector<char> valid(npl, 0);
     std::map<int, std::vector<int>> validCycles;
     std::fill(branch.begin(), branch.end(), 0);
+    // the reference's push, with the cases it ignores counted
+    auto push = [&](double key, long id) {
+      if (id < 0 || id >= npl)
+        st.q_push_id++;
+      else if (heap.contains(id))
+        st.q_push_present++;
+      heap.push(key, id);
+    };
+    bool stopped = false;
     while (!heap.empty()) {
       const auto top = heap.pop();
       branchId = (int)top.second;
       if (valid[branchId]) {
         ratioUpperBound = top.first;
         negCycle = validCycles.at(branchId);
+        st.valid_pops++;
+        stopped = true;
         break;
       }
       st.branches++;
       S.ratio = ratioUpperBound;
       const bool wraps = lawler();
-      if (ratioUpperBound < heap.peak() && !wraps) break;
+      if (ratioUpperBound < heap.peak() && !wraps) {
+        st.q_ub_break++;
+        stopped = true;
+        break;
+      }
       if (!wraps) {
-        heap.push(lbCurrentBranch, branchId);
+        // the reference pushes ratioLowerBoundOfCurrentBranch, a member it
+        // never assigns (cost_time_ratio_solver_cuda.cpp:288,
+        // cost_time_ratio_solver.cpp:835; declared at
+        // cost_time_ratio_solver.hpp:51): the branch's Lawler lambda here
+        push(lbCurrentBranch, branchId);
         valid[branchId] = 1;
         validCycles.emplace(branchId, negCycle);
         continue;
       }
-      if (numInBranch == 1) continue;
+      if (numInBranch == 1) {
+        st.q_single++;
+        continue;
+      }
       branch_split();
       for (int h : hits) {
-        if (h >= 0 && h < npl) heap.push(lbCurrentBranch, branch[h]);
+        if (h >= 0 && h < npl) push(lbCurrentBranch, branch[h]);
       }
     }
+    if (!stopped) st.q_empty++;
     out_edges.clear();
     Ratio r = ratio_of(negCycle, &out_edges);
     out_obj = (float)r.ratio;
@@ -835,8 +889,10 @@ void orc_quantize(double* cost, long E, int ncols) {
 // Lawler ratio solve. cost is column-major E x ncols, quantized in place.
 // flags bit0: candidate pick rule of the GPU reference (last reporting
 // vertex) instead of the CPU order; bit1: corrected odd-L fold (Q1). Outputs
-// the cycle's edge ids in walk order. stats: [bf_calls, sweeps, lawler_iters,
-// branches, relaxations].
+// the cycle's edge ids in walk order. stats[16]: [bf_calls, sweeps, lawler_iters,
+// branches, relaxations], then the branch-and-bound events of Stats (q_push_id,
+// q_push_present, q_single, q_rise, q_stale, q_cap, q_walk, q_empty, q_maxiter,
+// q_ub_break, valid_pops).
 int orc_ctr_solve(const int32_t* SRC, const int32_t* TRGT, long E, double* cost, int ncols, long npl, int K,
                   int flags, int32_t* out_edges, long cap, long* out_len, float* out_obj, int* passes,
                   long* stats) {
@@ -877,6 +933,10 @@ int orc_ctr_solve(const int32_t* SRC, const int32_t* TRGT, long E, double* cost,
       stats[2] = s.st.lawler_iters;
       stats[3] = s.st.branches;
       stats[4] = s.st.relaxations;
+      const long q[] = {s.st.q_push_id, s.st.q_push_present, s.st.q_single, s.st.q_rise,  s.st.q_stale,
+                        s.st.q_cap,     s.st.q_walk,         s.st.q_empty,  s.st.q_maxiter, s.st.q_ub_break,
+                        s.st.valid_pops};
+      for (int i = 0; i < 11; ++i) stats[5 + i] = q[i];
     }
     return 0;
   } catch (const std::exception& ex) {

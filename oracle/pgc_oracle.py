@@ -104,9 +104,22 @@ def quantize(cost):
     return cm.T.copy()
 
 
+# Branch-and-bound events counted by orc_ctr_solve (stats[5:16]); see Stats in
+# pgc_oracle.cpp. The first ten are the ways the reference's algorithm can end
+# above the once-wrapping optimum.
+CTR_EVENTS = ("push_id", "push_present", "single", "rise", "stale", "cap", "walk", "empty", "maxiter",
+              "ub_break", "valid_pop")
+
+
+def ctr_events(stats):
+    return {k: int(stats[5 + i]) for i, k in enumerate(CTR_EVENTS)}
+
+
 def ctr_solve(src, trgt, cost, npl, K, gpu_pick=False, corrected=False):
     """Lawler ratio solve. Returns (edge_ids, objective float32, passes, stats, quantized cost).
-    corrected: fold the closing layer's depth-0 row for odd L (Q1 fixed)."""
+    corrected: fold the closing layer's depth-0 row for odd L (Q1 fixed).
+    stats: [bf_calls, sweeps, lawler_iters, branches, relaxations] and the
+    branch-and-bound events of CTR_EVENTS at indices 5..15."""
     src = np.ascontiguousarray(src, np.int32)
     trgt = np.ascontiguousarray(trgt, np.int32)
     cost = np.asarray(cost, np.float64)
@@ -120,7 +133,7 @@ def ctr_solve(src, trgt, cost, npl, K, gpu_pick=False, corrected=False):
     n = C.c_long(0)
     obj = C.c_float(0)
     passes = C.c_int(0)
-    stats = np.zeros(5, np.int64)
+    stats = np.zeros(16, np.int64)
     flags = int(gpu_pick) | (2 if corrected else 0)
     rc = lib().orc_ctr_solve(_p(src), _p(trgt), E, _p(cm), cm.shape[0], npl, K, flags, _p(out), cap,
                              C.byref(n), C.byref(obj), C.byref(passes), _p(stats))

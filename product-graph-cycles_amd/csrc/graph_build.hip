@@ -831,8 +831,20 @@ hipError_t launch_layout_count(const int32_t* src, const int32_t* trgt, int64_t 
   return hipGetLastError();
 }
 
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t value) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+// p[0, n) <- value (a kernel, not a memset node: callers may capture it)
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<int64_t>(nb(n, 256), 4096)), dim3(256), 0, st, p, n, value);
+  return hipGetLastError();
+}
+
 hipError_t launch_layout_width(const int32_t* cnt, int64_t nTot, int32_t* width, hipStream_t st) {
-  hipError_t e = hipMemsetAsync(width, 0, 4 * 4, st);
+  hipError_t e = launch_fill_i32(width, 4, 0, st);
   if (e != hipSuccess) return e;
   // few blocks: every wave ends with four atomics on the same four words
   const unsigned g = (unsigned)std::min<int64_t>(32, std::max<int64_t>(1, (nTot + 255) / 256));

@@ -20,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <unordered_map>
+#include <unordered_set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -52,6 +53,7 @@ hipError_t launch_gather_rows(const int32_t*, int, const int32_t*, int64_t, int3
 hipError_t launch_gather_strided(const int32_t*, int64_t, int64_t, int32_t*, hipStream_t);
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
+hipError_t launch_fill_i32(int32_t*, int64_t, int32_t, hipStream_t);
 hipError_t launch_cycle_value(const CycleItem*, int, double*, hipStream_t);
 // conj_build.hip
 hipError_t launch_conj_product_space(const int32_t*, int64_t, const int32_t*, int64_t, int, const int32_t*,
@@ -180,6 +182,33 @@ static hipError_t dalloc(T** p, size_t n) {
   if (n == 0) n = 1;
   return cache_malloc((void**)p, n * sizeof(T));
 }
+
+// Where the device buffers of a solve come from: the memory cache above (the
+// default), or, for the allocation-free entry point, consecutive 256-byte
+// aligned pieces of the library's static scratch (an arena: nothing is
+// returned, the whole arena is reused by the next such call).
+struct DevAlloc {
+  uint8_t* base = nullptr;  // arena start (device address), nullptr: the memory cache
+  size_t cap = 0, used = 0;
+  bool overflow = false;  // an arena request did not fit
+  template <typename T>
+  hipError_t get(T** p, size_t n) {
+    if (n == 0) n = 1;
+    if (!base) return dalloc(p, n);
+    const size_t b = (n * sizeof(T) + 255) & ~(size_t)255;
+    if (used + b > cap) {
+      *p = nullptr;
+      overflow = true;
+      return hipErrorOutOfMemory;
+    }
+    *p = (T*)(base + used);
+    used += b;
+    return hipSuccess;
+  }
+  void put(void* p) {
+    if (!base && p) cache_free(p);
+  }
+};
 
 struct pgc_context {
   int device = 0;
@@ -329,7 +358,7 @@ struct pgc_graph {
 extern "C" {
 
 const char* pgc_last_error(void) { return g_last_error.c_str(); }
-int pgc_abi_version(void) { return 1; }
+int pgc_abi_version(void) { return 2; }
 
 int pgc_context_create(int device, pgc_context** out) {
   if (!out) return fail(PGC_ERR_ARG, "out is null");
@@ -401,7 +430,7 @@ int finish_graph_meta(pgc_graph* g) {
 
 constexpr int64_t kSlotPad = 32 * 64;
 
-int build_layout(pgc_graph* g) {
+int build_layout(pgc_graph* g, DevAlloc& A) {
   if (g->layout) return PGC_OK;
   hipStream_t st = g->ctx->stream;
   const int64_t npl = g->npl;
@@ -410,13 +439,13 @@ int build_layout(pgc_graph* g) {
   const int nC = g->nChunks;
   const int64_t nTot = g->nLayers * npl;
   int32_t *d_dropped = nullptr, *d_fill = nullptr;
-  HIPCHK(dalloc(&g->d_cnt, 2 * nTot));
-  HIPCHK(dalloc(&d_fill, 2 * nTot));
-  HIPCHK(dalloc(&d_dropped, 2));
-  HIPCHK(dalloc(&g->d_width, 4));
-  HIPCHK(hipMemsetAsync(g->d_cnt, 0, 2 * nTot * 4, st));
-  HIPCHK(hipMemsetAsync(d_fill, 0, 2 * nTot * 4, st));
-  HIPCHK(hipMemsetAsync(d_dropped, 0, 4, st));
+  HIPCHK(A.get(&g->d_cnt, 2 * nTot));
+  HIPCHK(A.get(&d_fill, 2 * nTot));
+  HIPCHK(A.get(&d_dropped, 2));
+  HIPCHK(A.get(&g->d_width, 4));
+  HIPCHK(launch_fill_i32(g->d_cnt, 2 * nTot, 0, st));
+  HIPCHK(launch_fill_i32(d_fill, 2 * nTot, 0, st));
+  HIPCHK(launch_fill_i32(d_dropped, 1, 0, st));
   HIPCHK(launch_layout_count(g->d_src, g->d_trgt, g->E, npl, g->numNodes, g->d_cnt, d_dropped, st));
   HIPCHK(launch_layout_width(g->d_cnt, nTot, g->d_width, st));
   int32_t dropped = 0;
@@ -448,10 +477,10 @@ int build_layout(pgc_graph* g) {
     // the solver may read up to 32 slots per lane past a chunk start; the
     // padding keeps those reads inside the allocation
     const int64_t alloc = total + kSlotPad;
-    HIPCHK(dalloc(&g->d_s[c], alloc));
-    HIPCHK(dalloc(&g->d_e[c], alloc));
-    HIPCHK(hipMemsetAsync(g->d_s[c], 0xff, alloc * 4, st));
-    HIPCHK(hipMemsetAsync(g->d_e[c], 0xff, alloc * 4, st));
+    HIPCHK(A.get(&g->d_s[c], alloc));
+    HIPCHK(A.get(&g->d_e[c], alloc));
+    HIPCHK(launch_fill_i32(g->d_s[c], alloc, -1, st));
+    HIPCHK(launch_fill_i32(g->d_e[c], alloc, -1, st));
   }
   HIPCHK(launch_layout_fill(g->d_src, g->d_trgt, g->E, npl, g->numNodes, nC, d_fill, g->d_cnt, g->d_width,
                             g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
@@ -459,7 +488,7 @@ int build_layout(pgc_graph* g) {
   // layer-invariant sources: every layer repeats the X adjacency (the
   // interlayer class from layer 1 on, as layer 0 has no interlayer in-edges;
   // the intralayer class up to layer L-1, as the closing layer L has none)
-  HIPCHK(hipMemsetAsync(d_dropped, 0xff, 8, st));
+  HIPCHK(launch_fill_i32(d_dropped, 2, -1, st));
   HIPCHK(launch_src_invariant(g->d_s[0], g->lstride[0], 0, g->nLayers - 1, d_dropped, st));
   HIPCHK(launch_src_invariant(g->d_s[1], g->lstride[1], 1, g->nLayers, d_dropped + 1, st));
   int32_t inv[2] = {0, 0};
@@ -467,16 +496,21 @@ int build_layout(pgc_graph* g) {
   HIPCHK(hipStreamSynchronize(st));
   g->src_inv[0] = inv[0] != 0;
   g->src_inv[1] = inv[1] != 0;
-  cache_free(d_dropped);
-  cache_free(d_fill);
+  A.put(d_dropped);
+  A.put(d_fill);
   g->layout = true;
   return PGC_OK;
+}
+
+int build_layout(pgc_graph* g) {
+  DevAlloc A;
+  return build_layout(g, A);
 }
 
 // kind 1: float q-unit cost (+ float time when a time column exists); kind 2: double raw cost
 // Slot cost buffers of the given kind (sized, reused while the size fits);
 // fill_costs then writes every slot.
-int cost_buffers(pgc_graph* g, int kind) {
+int cost_buffers(pgc_graph* g, int kind, DevAlloc& A) {
   hipStream_t st = g->ctx->stream;
   for (int c = 0; c < 2; ++c) {
     const int64_t n = g->nslots[c];
@@ -487,30 +521,37 @@ int cost_buffers(pgc_graph* g, int kind) {
     if (g->cost_bytes[c] != cb) {
       if (g->d_cost[c]) {
         HIPCHK(hipStreamSynchronize(st));
-        cache_free(g->d_cost[c]);
+        A.put(g->d_cost[c]);
       }
       g->d_cost[c] = nullptr;
       g->cost_bytes[c] = 0;
-      HIPCHK(cache_malloc(&g->d_cost[c], cb));
-      HIPCHK(hipMemsetAsync(g->d_cost[c], 0, cb, st));
+      int32_t* p = nullptr;
+      HIPCHK(A.get(&p, cb / 4));
+      HIPCHK(launch_fill_i32(p, (int64_t)(cb / 4), 0, st));
+      g->d_cost[c] = p;
       g->cost_bytes[c] = cb;
     }
     if (g->time_bytes[c] != tb) {
       if (g->d_time[c]) {
         HIPCHK(hipStreamSynchronize(st));
-        cache_free(g->d_time[c]);
+        A.put(g->d_time[c]);
       }
       g->d_time[c] = nullptr;
       g->time_bytes[c] = 0;
       if (tb) {
-        HIPCHK(dalloc(&g->d_time[c], n + kSlotPad));
-        HIPCHK(hipMemsetAsync(g->d_time[c], 0, tb, st));
+        HIPCHK(A.get(&g->d_time[c], n + kSlotPad));
+        HIPCHK(launch_fill_i32((int32_t*)g->d_time[c], n + kSlotPad, 0, st));
         g->time_bytes[c] = tb;
       }
     }
   }
   g->cost_kind = kind;
   return PGC_OK;
+}
+
+int cost_buffers(pgc_graph* g, int kind) {
+  DevAlloc A;
+  return cost_buffers(g, kind, A);
 }
 
 int fill_costs(pgc_graph* g, int kind) {
@@ -574,10 +615,12 @@ size_t job_lds_bytes(const pgc_graph* g, int K) {
   return b;
 }
 
-int ensure_scratch(pgc_graph* g, int K) {
+// Solver scratch for depth K. min_cost: with the buffers only the min-cost
+// solve reads (chains, lower bounds, start vertices, node and edge lists).
+int ensure_scratch(pgc_graph* g, int K, DevAlloc& A, bool min_cost = true) {
   JobScratch& s = g->js;
   if (s.K == K && s.pred) return PGC_OK;
-  s.release();
+  if (!A.base) s.release();
   const int64_t npl = g->npl, L = g->L;
   s.K = K;
   s.predN = (L + 1) * K * npl;
@@ -587,50 +630,62 @@ int ensure_scratch(pgc_graph* g, int K) {
   if (cap >= ((int64_t)1 << 31) || s.predN >= ((int64_t)1 << 31))
     return fail(PGC_ERR_UNSUPPORTED, "expanded graph exceeds int32 node ids");
   s.cyc_cap = (int)cap;
-  HIPCHK(dalloc(&s.pred, s.predN));
-  HIPCHK(dalloc(&s.wnext, npl));
-  HIPCHK(dalloc(&s.wlen, npl));
-  HIPCHK(dalloc(&s.chain, s.chainN));
-  HIPCHK(dalloc(&s.tmp_nodes, cap));
-  HIPCHK(dalloc(&s.tmp_eid, cap));
-  HIPCHK(dalloc(&s.neg_nodes, cap));
-  HIPCHK(dalloc(&s.neg_eid, cap));
-  HIPCHK(dalloc(&s.neg_len, 1));
-  HIPCHK(dalloc(&s.tmp_c, cap));
-  HIPCHK(dalloc(&s.tmp_t, cap));
-  HIPCHK(dalloc(&s.neg_c, cap));
-  HIPCHK(dalloc(&s.neg_t, cap));
-  HIPCHK(dalloc(&s.hits, npl));
-  HIPCHK(dalloc(&s.back, npl));
-  HIPCHK(dalloc(&s.branch, npl));
-  HIPCHK(dalloc(&s.nodes_io, cap));
-  HIPCHK(dalloc(&s.eids_io, cap));
-  HIPCHK(dalloc(&s.lb_key, npl));
-  HIPCHK(dalloc(&s.backup, npl));
-  HIPCHK(dalloc(&s.res, 1));
-  HIPCHK(dalloc(&s.djob, 1));
-  if (kernel_maxc(g, K) == 0) HIPCHK(dalloc(&s.dtab, (int64_t)(2 * K + 1) * (npl + 1)));
+  HIPCHK(A.get(&s.pred, s.predN));
+  HIPCHK(A.get(&s.wnext, npl));
+  HIPCHK(A.get(&s.wlen, npl));
+  HIPCHK(A.get(&s.tmp_nodes, cap));
+  HIPCHK(A.get(&s.tmp_eid, cap));
+  HIPCHK(A.get(&s.neg_nodes, cap));
+  HIPCHK(A.get(&s.neg_eid, cap));
+  HIPCHK(A.get(&s.neg_len, 1));
+  HIPCHK(A.get(&s.tmp_c, cap));
+  HIPCHK(A.get(&s.tmp_t, cap));
+  HIPCHK(A.get(&s.neg_c, cap));
+  HIPCHK(A.get(&s.neg_t, cap));
+  HIPCHK(A.get(&s.hits, npl));
+  HIPCHK(A.get(&s.branch, npl));
+  HIPCHK(A.get(&s.backup, npl));
+  HIPCHK(A.get(&s.res, 1));
+  HIPCHK(A.get(&s.djob, 1));
+  if (min_cost) {
+    HIPCHK(A.get(&s.chain, s.chainN));
+    HIPCHK(A.get(&s.back, npl));
+    HIPCHK(A.get(&s.nodes_io, cap));
+    HIPCHK(A.get(&s.eids_io, cap));
+    HIPCHK(A.get(&s.lb_key, npl));
+  }
+  if (kernel_maxc(g, K) == 0) HIPCHK(A.get(&s.dtab, (int64_t)(2 * K + 1) * (npl + 1)));
   return PGC_OK;
+}
+
+int ensure_scratch(pgc_graph* g, int K) {
+  DevAlloc A;
+  return ensure_scratch(g, K, A);
 }
 
 // Queue and valid-branch cycle store of the device-resident branch and bound.
 constexpr int kValidCycles = 64;
 
-int ensure_bnb(pgc_graph* g) {
+int ensure_bnb(pgc_graph* g, DevAlloc& A, int vc_cap) {
   JobScratch& s = g->js;
   if (s.hkey) return PGC_OK;
   const int64_t npl = g->npl;
-  s.vc_cap = kValidCycles;
+  s.vc_cap = vc_cap;
   s.vc_stride = (int)(g->L * s.K + 2);  // a once-wrapping cycle has at most L*K+1 edges
-  HIPCHK(dalloc(&s.hkey, npl));
-  HIPCHK(dalloc(&s.hval, npl));
-  HIPCHK(dalloc(&s.hwhere, npl));
-  HIPCHK(dalloc(&s.vslot, npl));
-  HIPCHK(dalloc(&s.vc_eid, (int64_t)s.vc_cap * s.vc_stride));
-  HIPCHK(dalloc(&s.vc_c, (int64_t)s.vc_cap * s.vc_stride));
-  HIPCHK(dalloc(&s.vc_t, (int64_t)s.vc_cap * s.vc_stride));
-  HIPCHK(dalloc(&s.vc_len, s.vc_cap));
+  HIPCHK(A.get(&s.hkey, npl));
+  HIPCHK(A.get(&s.hval, npl));
+  HIPCHK(A.get(&s.hwhere, npl));
+  HIPCHK(A.get(&s.vslot, npl));
+  HIPCHK(A.get(&s.vc_eid, (int64_t)s.vc_cap * s.vc_stride));
+  HIPCHK(A.get(&s.vc_c, (int64_t)s.vc_cap * s.vc_stride));
+  HIPCHK(A.get(&s.vc_t, (int64_t)s.vc_cap * s.vc_stride));
+  HIPCHK(A.get(&s.vc_len, s.vc_cap));
   return PGC_OK;
+}
+
+int ensure_bnb(pgc_graph* g) {
+  DevAlloc A;
+  return ensure_bnb(g, A, kValidCycles);
 }
 
 SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
@@ -696,6 +751,9 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.vc_t = s.vc_t;
   j.vc_len = s.vc_len;
   j.vc_cap = s.vc_cap;
+  // PGC_VALID_STORE=n caps the device store of valid-branch cycles at n (a
+  // validation switch for the host fallback of batched solves)
+  if (const char* vs = std::getenv("PGC_VALID_STORE")) j.vc_cap = std::min(j.vc_cap, std::max(0, std::atoi(vs)));
   j.vc_stride = s.vc_stride;
   return j;
 }
@@ -1229,24 +1287,20 @@ int pgc_graph_derive(pgc_graph* base, const double* feat_diff, int on_device, pg
   return PGC_OK;
 }
 
-// ------------------------------------------------------------ Lawler + BnB
-int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
-                    int64_t capacity, pgc_result* res) {
-  Timer wall;
+}  // extern "C"
+
+namespace {
+
+// The host branch and bound of solveWithBnbGPULawler
+// (cost_time_ratio_solver_cuda.cpp:175-331, the CPU twin
+// cost_time_ratio_solver.cpp:794-862) around the device Lawler launches, on a
+// graph whose costs are quantized and filled and whose scratch is sized for K.
+int ratio_bnb_host(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
+                   int64_t capacity, pgc_result* res, const Timer& wall) {
+  int rc;
   for (double& x : g_phase_ms) x = 0;
-  if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
-  if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
-  int rc = check_solvable(g, K);
-  if (rc) return rc;
-  HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const int64_t npl = g->npl;
-  // robustFloatToIntConversion of every column, in place (:875-882)
-  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
-  if ((rc = build_layout(g))) return rc;
-  if ((rc = check_width(g))) return rc;
-  if ((rc = fill_costs(g, 1))) return rc;
-  if ((rc = ensure_scratch(g, K))) return rc;
   JobScratch& s = g->js;
   HIPCHK(hipMemsetAsync(s.pred, 0xff, s.predN * 4, st));
   HIPCHK(hipMemsetAsync(s.neg_len, 0, 4, st));
@@ -1377,7 +1431,40 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   res->kernel_ms = kms;
   res->total_ms = wall.ms();
   for (int i = 0; i < 5; ++i) res->phase_ms[i] = g_phase_ms[i];
+  res->status = PGC_OK;
   return PGC_OK;
+}
+
+// Everything a ratio solve of g at depth K needs besides its energy: the
+// in-edge layout, the slot cost buffers and the solver scratch. A graph that
+// cannot be solved is refused here, before its energy is quantized in place.
+int prepare_ratio(pgc_graph* g, int K) {
+  int rc;
+  if ((rc = check_solvable(g, K))) return rc;
+  if ((rc = build_layout(g))) return rc;
+  if ((rc = check_width(g))) return rc;
+  if ((rc = cost_buffers(g, 1))) return rc;
+  return ensure_scratch(g, K);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ Lawler + BnB
+int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
+                    int64_t capacity, pgc_result* res) {
+  Timer wall;
+  if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
+  if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int rc = prepare_ratio(g, K);
+  if (rc) return rc;
+  // robustFloatToIntConversion of every column, in place (:875-882)
+  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
+  if ((rc = fill_costs(g, 1))) return rc;
+  return ratio_bnb_host(ctx, g, K, flags, edge_ids, rows, capacity, res, wall);
 }
 
 // ------------------------------------------------- batched complete solves
@@ -1441,19 +1528,30 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   int rc;
-  for (int64_t i = 0; i < n; ++i) {
-    pgc_graph* g = graphs[i];
-    if (!g) return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " is null");
-    if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " belongs to another context");
-    if (offsets[i + 1] < offsets[i]) return fail(PGC_ERR_ARG, "offsets must be non-decreasing");
-    if (rows && g->ps_cols != graphs[0]->ps_cols)
-      return fail(PGC_ERR_ARG, "rows output needs one product-space width for the whole batch");
-    for (int64_t k = 0; k < i && k < 64; ++k)
-      if (graphs[k] == g) return fail(PGC_ERR_ARG, "a graph appears twice in one batch");
+  {
+    // every graph once: two items would quantize one energy in place twice
+    // and share one solver scratch
+    std::unordered_set<const pgc_graph*> seen;
+    seen.reserve((size_t)n * 2);
+    for (int64_t i = 0; i < n; ++i) {
+      pgc_graph* g = graphs[i];
+      if (!g) return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " is null");
+      if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " belongs to another context");
+      if (offsets[i + 1] < offsets[i]) return fail(PGC_ERR_ARG, "offsets must be non-decreasing");
+      if (rows && g->ps_cols != graphs[0]->ps_cols)
+        return fail(PGC_ERR_ARG, "rows output needs one product-space width for the whole batch");
+      if (!seen.insert(g).second)
+        return fail(PGC_ERR_ARG, "graph " + std::to_string(i) + " appears twice in one batch");
+    }
   }
-  // per graph: layout, cost buffers, scratch; then the in-place
+  // per graph: layout, cost buffers, scratch (a graph that cannot be solved
+  // fails the call here, before any energy is quantized); then the in-place
   // quantization of every energy column and the fp32 slot costs of every
   // graph in a few batched launches (each column as pgc_solve_ratio does it)
+  for (int64_t i = 0; i < n; ++i) {
+    if ((rc = prepare_ratio(graphs[i], max_depth[i])) || (rc = ensure_bnb(graphs[i])))
+      return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
+  }
   std::vector<SolverJob> jobs(n);
   std::vector<QItem> qitems;
   std::vector<FItem> fitems;
@@ -1461,16 +1559,10 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   for (int64_t i = 0; i < n; ++i) {
     pgc_graph* g = graphs[i];
     const int K = max_depth[i];
-    if ((rc = check_solvable(g, K))) return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
-    if ((rc = build_layout(g))) return rc;
-    if ((rc = check_width(g))) return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
-    if ((rc = cost_buffers(g, 1))) return rc;
     for (int c = 0; c < g->ncols; ++c) qitems.push_back({g->d_energy + (int64_t)c * g->E, g->E, 0, 0});
     for (int c = 0; c < 2; ++c)
       fitems.push_back({g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, (float*)g->d_cost[c],
                         g->ncols > 1 ? g->d_time[c] : nullptr});
-    if ((rc = ensure_scratch(g, K))) return rc;
-    if ((rc = ensure_bnb(g))) return rc;
     // a batch job is one launch: the kernel starts it with an empty cycle and,
     // for a predecessor table in LDS, with -1 predecessors (no global table to
     // reset, load or write back); global tables are reset by one fill launch
@@ -1606,22 +1698,36 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   HIPCHK(hipStreamSynchronize(st));
   float kms = 0;
   HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+  // per-item outcome: solved on the device; solved again by the host branch
+  // and bound when the device's valid-branch store or branch cap ran out
+  // (same quantized costs, same cycle as pgc_solve_ratio); or failed with its
+  // own status
+  std::vector<int> status(n, PGC_OK);
+  std::vector<char> redo(n, 0);
+  std::vector<std::string> why(n);
   for (int64_t i = 0; i < n; ++i) {
     const JobResult& r = out[i];
-    if (r.error)
-      return fail(r.error == ERR_CYCLE_TOO_LONG && r.out_len > jobs[i].out_cap ? PGC_ERR_CAPACITY : PGC_ERR_SOLVER,
-                  "graph " + std::to_string(i) + ": solver error " + std::to_string(r.error) +
-                      (r.error == ERR_CYCLE_TOO_LONG ? " (cycle of " + std::to_string(r.out_len) +
-                                                           " edges, capacity " + std::to_string(jobs[i].out_cap) + ")"
-                                                     : ""));
+    if (!r.error) continue;
+    if (r.error == ERR_VALID_STORE || r.error == ERR_BRANCH_CAP) {
+      redo[i] = 1;
+      continue;
+    }
+    status[i] = r.error == ERR_CYCLE_TOO_LONG && r.out_len > jobs[i].out_cap ? PGC_ERR_CAPACITY : PGC_ERR_SOLVER;
+    why[i] = "graph " + std::to_string(i) + ": solver error " + std::to_string(r.error) +
+             (r.error == ERR_CYCLE_TOO_LONG ? " (cycle of " + std::to_string(r.out_len) + " edges, capacity " +
+                                                  std::to_string(jobs[i].out_cap) + ")"
+                                            : "");
   }
+  const int64_t pc = graphs[0]->ps_cols;
   if (rows) {
     int64_t tot = 0;
-    for (int64_t i = 0; i < n; ++i) tot += out[i].out_len * (int64_t)graphs[i]->ps_cols;
+    for (int64_t i = 0; i < n; ++i)
+      if (status[i] == PGC_OK && !redo[i]) tot += out[i].out_len * (int64_t)graphs[i]->ps_cols;
     if ((rc = grow(&ctx->d_rows, ctx->rows_cap, std::max<int64_t>(tot, 1)))) return rc;
     int64_t at = 0;
-    std::vector<int64_t> rowAt(n);
+    std::vector<int64_t> rowAt(n, 0);
     for (int64_t i = 0; i < n; ++i) {
+      if (status[i] != PGC_OK || redo[i]) continue;
       rowAt[i] = at;
       if (out[i].out_len)
         HIPCHK(launch_gather_rows(graphs[i]->d_rows, graphs[i]->ps_cols, jobs[i].out_eid, out[i].out_len,
@@ -1632,16 +1738,44 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     HIPCHK(hipMemcpyAsync(h.data(), ctx->d_rows, tot * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     // rows of graph i at rows + (offsets[i] - offsets[0]) * ps_cols
-    const int64_t pc = graphs[0]->ps_cols;
     for (int64_t i = 0; i < n; ++i)
-      std::memcpy(rows + (offsets[i] - offsets[0]) * pc, h.data() + rowAt[i],
-                  out[i].out_len * (int64_t)graphs[i]->ps_cols * 4);
+      if (status[i] == PGC_OK && !redo[i])
+        std::memcpy(rows + (offsets[i] - offsets[0]) * pc, h.data() + rowAt[i],
+                    out[i].out_len * (int64_t)graphs[i]->ps_cols * 4);
+  }
+  std::map<int64_t, pgc_result> host_res;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!redo[i]) continue;
+    pgc_result one;
+    std::memset(&one, 0, sizeof(one));
+    Timer t1;
+    rc = ratio_bnb_host(ctx, graphs[i], max_depth[i], flags, edge_ids ? edge_ids + offsets[i] : nullptr,
+                        rows ? rows + (offsets[i] - offsets[0]) * pc : nullptr, offsets[i + 1] - offsets[i], &one,
+                        t1);
+    if (rc) {
+      status[i] = rc;
+      why[i] = "graph " + std::to_string(i) + ": " + g_last_error;
+      continue;
+    }
+    host_res[i] = one;
   }
   const double total = wall.ms();
+  int64_t first_bad = -1;
   for (int64_t i = 0; i < n; ++i) {
     const JobResult& r = out[i];
     pgc_result& o = res[i];
     std::memset(&o, 0, sizeof(o));
+    o.status = status[i];
+    if (status[i] != PGC_OK) {
+      if (first_bad < 0) first_bad = i;
+      continue;
+    }
+    if (redo[i]) {
+      o = host_res[i];
+      o.kernel_ms += kms;
+      o.total_ms = total;
+      continue;
+    }
     o.passes_layers_once = 1;
     o.objective = (float)(r.out_cs / r.out_ts);
     o.n_edges = r.out_len;
@@ -1655,6 +1789,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     const int64_t ticks[5] = {r.t_sweep, r.t_fold, r.t_chase, r.t_walk, r.t_extract};
     for (int k = 0; k < 5; ++k) o.phase_ms[k] = (double)ticks[k] * 1e-5;
   }
+  if (first_bad >= 0) return fail(status[first_bad], why[first_bad]);
   return PGC_OK;
 }
 
@@ -2019,6 +2154,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   res->kernel_ms = kms;
   res->total_ms = wall.ms();
   for (int i = 0; i < 5; ++i) res->phase_ms[i] = g_phase_ms[i];
+  res->status = PGC_OK;
   return PGC_OK;
 }
 

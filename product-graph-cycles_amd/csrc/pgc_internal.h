@@ -16,6 +16,7 @@
 //     iteration order; padding slots carry src = npl (a sentinel whose
 //     distance is +inf in the solver) and eid = -1.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #define PGC_WAVE 64

@@ -35,12 +35,14 @@ class pgc_result(C.Structure):
         ("kernel_ms", C.c_double),
         ("total_ms", C.c_double),
         ("phase_ms", C.c_double * 5),
+        ("status", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     PHASES = ("sweep", "fold", "chase", "walk", "extract")
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "phase_ms"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("phase_ms", "reserved")}
         d["phase_ms"] = dict(zip(self.PHASES, list(self.phase_ms)))
         return d
 
@@ -290,11 +292,15 @@ class Graph:
             pass
 
 
-def solve_ratio_batch(ctx, graphs, max_depth, flags=PGC_LAWLER_PICK_LAST, with_rows=True):
+def solve_ratio_batch(ctx, graphs, max_depth, flags=PGC_LAWLER_PICK_LAST, with_rows=True, raise_on_error=True,
+                      capacity=None):
     """Complete lawlergpu solves of a list of Graphs in persistent launches.
 
     max_depth: one int for all graphs or one per graph. Returns a list of
-    (edge_ids, rows or None, pgc_result) in the order of `graphs`.
+    (edge_ids, rows or None, pgc_result) in the order of `graphs`. With
+    raise_on_error=False a failing graph does not raise: its result's status
+    is its error code (the graphs that solved are returned as usual).
+    capacity: output edges per graph (default: the reference's cycle buffer).
     """
     L = load()
     n = len(graphs)
@@ -302,7 +308,10 @@ def solve_ratio_batch(ctx, graphs, max_depth, flags=PGC_LAWLER_PICK_LAST, with_r
         return []
     K = np.broadcast_to(np.asarray(max_depth, np.int32), (n,)).copy()
     infos = [g.info() for g in graphs]
-    caps = np.array([max(64, 101 * i["n_layers"] * int(k) + 8) for i, k in zip(infos, K)], np.int64)
+    if capacity is None:
+        caps = np.array([max(64, 101 * i["n_layers"] * int(k) + 8) for i, k in zip(infos, K)], np.int64)
+    else:
+        caps = np.broadcast_to(np.asarray(capacity, np.int64), (n,)).copy()
     off = np.zeros(n + 1, np.int64)
     np.cumsum(caps, out=off[1:])
     eids = np.empty(int(off[-1]), np.int32)
@@ -310,7 +319,9 @@ def solve_ratio_batch(ctx, graphs, max_depth, flags=PGC_LAWLER_PICK_LAST, with_r
     rows = np.empty((int(off[-1]), cols), np.int32) if with_rows else None
     handles = (C.c_void_p * n)(*[g._h.value for g in graphs])
     res = (pgc_result * n)()
-    check(L.pgc_solve_ratio_batch(ctx.handle, handles, ptr(K), n, int(flags), ptr(eids), ptr(rows), ptr(off), res))
+    rc = L.pgc_solve_ratio_batch(ctx.handle, handles, ptr(K), n, int(flags), ptr(eids), ptr(rows), ptr(off), res)
+    if rc != 0 and (raise_on_error or all(res[i].status == 0 for i in range(n))):
+        check(rc)
     out = []
     for i in range(n):
         m = res[i].n_edges

@@ -229,3 +229,77 @@ def test_corrected_odd_L_mincost_matches_bruteforce():
             e2, o2, _, _ = O.dijkstra_solve(src, trgt, en[:, 0], nx, K)
             wrong_compat += o2 != np.float32(best[0])
     assert wrong_compat > 0
+
+
+# ---- the ratio branch and bound on instances that branch
+#
+# Seeded tiny rings on which the Lawler solve splits branches (found by a seed
+# scan; every instance explores at least three branches under one pick rule).
+# The reference's branch and bound is a heuristic in the ways the oracle
+# counts (ctr_events): pushes the heap ignores (branch ids >= npl, ids already
+# queued), single-vertex multi-wrap branches dropped, Lawler stops on a rising
+# ratio or at its iteration cap, branches whose first Bellman-Ford call finds
+# nothing (they keep lambda_UB and the previous cycle), Bellman-Ford calls at
+# the sweep cap, candidate walks that close on a non-negative cycle, and a
+# queue that runs empty (the output is then the last Lawler cycle, which may
+# wrap more than once). The pins: the cycle is a closed walk of the product
+# graph; without an empty queue it wraps exactly once and is never below the
+# brute-force optimum; it is the optimum (within the reference's 0.5 q-unit
+# zero tolerance) unless one of those events fired.
+from branching_cases import BRANCHING  # noqa: E402
+
+QUIRKS = ("push_id", "push_present", "single", "rise", "stale", "cap", "walk", "empty", "maxiter")
+
+
+def _branching_solve(seed, nx, ny, K, gpu_pick):
+    p = S.tiny_pair(nx, ny, seed=seed, ring=True)
+    rows, src, trgt = O.product_space(p["EX"], p["EY"])
+    en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"])
+    edges, obj, passes, stats, q = O.ctr_solve(src, trgt, en, nx, K, gpu_pick=gpu_pick)
+    return src, trgt, edges, obj, stats, q
+
+
+@pytest.mark.parametrize("seed,nx,ny,K", BRANCHING)
+@pytest.mark.parametrize("gpu_pick", [False, True])
+def test_ratio_bnb_branching_pins(seed, nx, ny, K, gpu_pick):
+    src, trgt, edges, obj, stats, q = _branching_solve(seed, nx, ny, K, gpu_pick)
+    ev = O.ctr_events(stats)
+    L = (int(src.max()) + 1) // nx
+    numNodes = L * nx
+    cq, tq = q[:, 0], np.full(len(q), 5e4)
+    best = B.best_ratio(src, trgt, nx, K, cq, tq)
+    # a closed walk: consecutive edges meet, the last one returns to the first
+    # one's layer-0 vertex (through the closing copy of layer 0)
+    assert len(edges) > 0
+    for a, b in zip(edges[:-1], edges[1:]):
+        assert trgt[a] == src[b]
+    assert int(trgt[edges[-1]]) % numNodes == int(src[edges[0]]) % numNodes
+    inter = int(np.sum(src[edges] // nx != trgt[edges] // nx))
+    assert inter % L == 0 and inter >= L
+    c, t = int(cq[edges].sum()), int(tq[edges].sum())
+    assert obj == np.float32(c / t)
+    gap = (c / t - float(best[0])) * best[2]  # q-units of reweighted cost at the optimum's time
+    if ev["empty"] == 0:
+        assert inter == L, "a stop other than an empty queue returns a once-wrapping cycle"
+        assert gap > -0.5, "below the exact once-wrapping optimum"
+    optimal = inter == L and gap < 0.5 + 1e-9
+    assert optimal or any(ev[k] for k in QUIRKS), "not optimal and no branch-and-bound event: %s" % ev
+
+
+def test_ratio_bnb_branching_coverage():
+    # the set reaches every part of the branch and bound: splits, the heap's
+    # ignored pushes, stored valid branches popped as the answer, the
+    # lambda_UB stop, an empty queue, single-vertex drops, stale branches,
+    # sweep-cap calls and walks that close on non-negative cycles
+    tot = {k: 0 for k in O.CTR_EVENTS}
+    branches = 0
+    for seed, nx, ny, K in BRANCHING:
+        for gpu_pick in (False, True):
+            st = _branching_solve(seed, nx, ny, K, gpu_pick)[4]
+            branches += int(st[3] > 1)
+            for k, v in O.ctr_events(st).items():
+                tot[k] += int(v > 0)
+    assert branches >= 50
+    for k in ("push_id", "push_present", "single", "stale", "cap", "walk", "empty", "ub_break", "valid_pop"):
+        assert tot[k] >= 1, k
+    assert tot["valid_pop"] >= 20 and tot["empty"] >= 8
