@@ -141,6 +141,22 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
                           uint32_t flags, int32_t* edge_ids, int32_t* rows, const int64_t* offsets,
                           pgc_result* res);
 
+/* pgc_solve_ratio without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same quantization in
+ * place, cycle, objective and counters as pgc_solve_ratio; the whole solve
+ * runs in one launch (the device-resident branch and bound), and every buffer
+ * beyond the graph's own arrays and the caller's outputs lives in the
+ * library's static device scratch: 128 MiB per device, held from the first
+ * context on that device until the process ends (graphs up to about the size
+ * of an icosphere of subdivision 4 against a 128-point curve at max_depth 3).
+ * A solve that needs more returns PGC_ERR_CAPACITY. The scratch is shared by
+ * all calls on a device: calls on one device are serialized by the library
+ * (without that they would race), and the scratch keeps the last call's
+ * contents; every call rebuilds what it reads. */
+int pgc_solve_ratio_noalloc(pgc_context* ctx, pgc_graph* g, int max_depth, uint32_t flags, int32_t* edge_ids,
+                            int32_t* rows, int64_t capacity, pgc_result* res);
+
 /* Recomputes the graph's per-edge energy from another feat_diff ([nVX x nVY]
  * row-major float64, device memory when on_device != 0, else host) with the
  * graph's construction modes; the product space and layout are unchanged.

@@ -54,6 +54,10 @@ hipError_t launch_gather_strided(const int32_t*, int64_t, int64_t, int32_t*, hip
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
 hipError_t launch_fill_i32(int32_t*, int64_t, int32_t, hipStream_t);
+// scratch.hip
+hipError_t scratch_setup(int device);
+void* scratch_ptr(int device);
+std::mutex& scratch_call_lock(int device);
 hipError_t launch_cycle_value(const CycleItem*, int, double*, hipStream_t);
 // conj_build.hip
 hipError_t launch_conj_product_space(const int32_t*, int64_t, const int32_t*, int64_t, int, const int32_t*,
@@ -378,6 +382,9 @@ int pgc_context_create(int device, pgc_context** out) {
     delete c;
     return fail(PGC_ERR_HIP, std::string("context setup: ") + hipGetErrorString(e));
   }
+  // the device address of the static scratch, resolved once per device; a
+  // failure only disables pgc_solve_ratio_noalloc (which reports it)
+  (void)scratch_setup(device);
   *out = c;
   return PGC_OK;
 }
@@ -1465,6 +1472,183 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
   if ((rc = fill_costs(g, 1))) return rc;
   return ratio_bnb_host(ctx, g, K, flags, edge_ids, rows, capacity, res, wall);
+}
+
+// ------------------------------------------- allocation-free ratio solve
+}  // extern "C"
+
+namespace {
+
+// Lends a graph layout, slot-cost and solver-scratch state that lives in the
+// static scratch for one call, and gives the graph its own state back when it
+// ends (nothing of the scratch is ever freed or kept by the graph).
+struct ScratchView {
+  pgc_graph* g;
+  bool layout;
+  int nChunks;
+  int64_t nLayers, nslots[2], dropped, isv, xsv;
+  int32_t* d_width;
+  int32_t *d_s[2], *d_e[2], *d_cnt;
+  int32_t width[2], lstride[2], src_inv[2], cls_deg[2], maxw, mid;
+  void* d_cost[2];
+  float* d_time[2];
+  int cost_kind;
+  size_t cost_bytes[2], time_bytes[2];
+  JobScratch js;
+  explicit ScratchView(pgc_graph* gr) : g(gr) {
+    layout = g->layout;
+    nChunks = g->nChunks;
+    nLayers = g->nLayers;
+    dropped = g->dropped;
+    isv = g->intra_slots_valid;
+    xsv = g->inter_slots_valid;
+    d_width = g->d_width;
+    d_cnt = g->d_cnt;
+    maxw = g->maxw;
+    mid = g->max_in_degree;
+    cost_kind = g->cost_kind;
+    js = g->js;
+    for (int c = 0; c < 2; ++c) {
+      nslots[c] = g->nslots[c];
+      d_s[c] = g->d_s[c];
+      d_e[c] = g->d_e[c];
+      width[c] = g->width[c];
+      lstride[c] = g->lstride[c];
+      src_inv[c] = g->src_inv[c];
+      cls_deg[c] = g->cls_deg[c];
+      d_cost[c] = g->d_cost[c];
+      d_time[c] = g->d_time[c];
+      cost_bytes[c] = g->cost_bytes[c];
+      time_bytes[c] = g->time_bytes[c];
+      g->d_cost[c] = nullptr;
+      g->d_time[c] = nullptr;
+      g->cost_bytes[c] = g->time_bytes[c] = 0;
+    }
+    g->layout = false;
+    g->js = JobScratch();
+  }
+  ~ScratchView() {
+    g->layout = layout;
+    g->nChunks = nChunks;
+    g->nLayers = nLayers;
+    g->dropped = dropped;
+    g->intra_slots_valid = isv;
+    g->inter_slots_valid = xsv;
+    g->d_width = d_width;
+    g->d_cnt = d_cnt;
+    g->maxw = maxw;
+    g->max_in_degree = mid;
+    g->cost_kind = cost_kind;
+    g->js = js;
+    for (int c = 0; c < 2; ++c) {
+      g->nslots[c] = nslots[c];
+      g->d_s[c] = d_s[c];
+      g->d_e[c] = d_e[c];
+      g->width[c] = width[c];
+      g->lstride[c] = lstride[c];
+      g->src_inv[c] = src_inv[c];
+      g->cls_deg[c] = cls_deg[c];
+      g->d_cost[c] = d_cost[c];
+      g->d_time[c] = d_time[c];
+      g->cost_bytes[c] = cost_bytes[c];
+      g->time_bytes[c] = time_bytes[c];
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int pgc_solve_ratio_noalloc(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
+                            int64_t capacity, pgc_result* res) {
+  Timer wall;
+  if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
+  if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
+  HIPCHK(hipSetDevice(ctx->device));
+  uint8_t* base = (uint8_t*)scratch_ptr(ctx->device);
+  if (!base) return fail(PGC_ERR_HIP, "the static scratch of this device has no device address");
+  // the scratch is shared by every call on this device: calls are serialized
+  std::lock_guard<std::mutex> lock(scratch_call_lock(ctx->device));
+  int rc = check_solvable(g, K);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  ScratchView view(g);
+  DevAlloc A;
+  A.base = base;
+  A.cap = kScratchBytes;
+  auto fit = [&](int code) {
+    if (A.overflow)
+      return fail(PGC_ERR_CAPACITY, "this solve needs more device memory than the " +
+                                        std::to_string(kScratchBytes >> 20) + " MiB static scratch holds");
+    return code;
+  };
+  const int64_t npl = g->npl;
+  if ((rc = build_layout(g, A))) return fit(rc);
+  if ((rc = check_width(g))) return rc;
+  if ((rc = cost_buffers(g, 1, A))) return fit(rc);
+  if ((rc = ensure_scratch(g, K, A, false))) return fit(rc);
+  // one store slot per branch id below npl: every valid branch fits
+  if ((rc = ensure_bnb(g, A, (int)npl))) return fit(rc);
+  JobScratch& s = g->js;
+  const int64_t outCap = std::max<int64_t>(0, std::min<int64_t>(capacity, s.cyc_cap));
+  int32_t *d_out = nullptr, *d_rows = nullptr;
+  void* sel = nullptr;
+  uint8_t* selb = nullptr;
+  if (A.get(&d_out, outCap) != hipSuccess) return fit(PGC_ERR_HIP);
+  if (rows && A.get(&d_rows, outCap * g->ps_cols) != hipSuccess) return fit(PGC_ERR_HIP);
+  if (A.get(&selb, select_state_bytes()) != hipSuccess) return fit(PGC_ERR_HIP);
+  sel = selb;
+  // robustFloatToIntConversion of every column, in place (:875-882), then
+  // the slot costs
+  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, sel, st));
+  for (int c = 0; c < 2; ++c)
+    HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, g->ncols, false, g->d_cost[c],
+                             g->ncols > 1 ? g->d_time[c] : nullptr, st));
+  // the complete solve in one launch: upper bound, branch and bound, output
+  SolverJob job = make_job(g, K, MODE_FULL_RATIO, flags | FLAG_FRESH_JOB);
+  job.vc_cap = s.vc_cap;
+  job.res = s.res;
+  job.out_eid = d_out;
+  job.out_cap = (int32_t)outCap;
+  if (!pred_fits_lds(g, K)) HIPCHK(launch_fill_i32(s.pred, s.predN, -1, st));
+  HIPCHK(hipMemcpyAsync(s.djob, &job, sizeof(job), hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  HIPCHK(launch_solver(s.djob, 1, 1, nullptr, kernel_maxc(g, K), g->maxw, false, g->d_time[0] != nullptr, false,
+                       job_lds_bytes(g, K), st));
+  HIPCHK(hipEventRecord(ctx->ev1, st));
+  JobResult r;
+  HIPCHK(hipMemcpyAsync(&r, s.res, sizeof(r), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float kms = 0;
+  HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+  if (r.error) {
+    if (r.error == ERR_CYCLE_TOO_LONG && r.out_len > outCap)
+      return fail(PGC_ERR_CAPACITY, "output capacity too small: need " + std::to_string(r.out_len));
+    return fail(PGC_ERR_SOLVER, "solver error " + std::to_string(r.error));
+  }
+  const int64_t m = r.out_len;
+  if (edge_ids && m) HIPCHK(hipMemcpyAsync(edge_ids, d_out, m * 4, hipMemcpyDeviceToHost, st));
+  if (rows && m) {
+    HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, d_out, m, d_rows, st));
+    HIPCHK(hipMemcpyAsync(rows, d_rows, m * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  std::memset(res, 0, sizeof(*res));
+  res->passes_layers_once = 1;
+  res->objective = (float)(r.out_cs / r.out_ts);
+  res->n_edges = m;
+  res->sweeps = r.sweeps;
+  res->bf_calls = r.bf_calls;
+  res->lawler_iters = r.lawler_iters;
+  res->branches = r.branches;
+  res->relaxations = relax_per_sweep(g, K) * (double)r.sweeps;
+  res->kernel_ms = kms;
+  res->total_ms = wall.ms();
+  const int64_t ticks[5] = {r.t_sweep, r.t_fold, r.t_chase, r.t_walk, r.t_extract};
+  for (int k = 0; k < 5; ++k) res->phase_ms[k] = (double)ticks[k] * 1e-5;
+  res->status = PGC_OK;
+  return PGC_OK;
 }
 
 // ------------------------------------------------- batched complete solves

@@ -24,6 +24,9 @@
 
 namespace pgc {
 
+// Bytes of the static device scratch of the allocation-free solve (per device).
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+
 struct SlotClass {
   const int32_t* src;    // source vertex index inside its layer; padding = npl (sentinel)
   const void* cost;      // float (q-units, exact) or double (raw)

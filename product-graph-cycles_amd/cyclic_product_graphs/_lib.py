@@ -53,7 +53,7 @@ EXPORTED = [
     "pgc_graph_from_arrays", "pgc_graph_set_branch_graph", "pgc_graph_info", "pgc_graph_get_product_space",
     "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
     "pgc_solve_ratio_batch", "pgc_graph_set_feat_diff", "pgc_graphs_set_feat_diff", "pgc_graph_derive",
-    "pgc_cycle_ratio_values", "pgc_cycle_ratio_grad",
+    "pgc_cycle_ratio_values", "pgc_cycle_ratio_grad", "pgc_solve_ratio_noalloc",
 ]
 
 _lib = None
@@ -87,6 +87,7 @@ def load():
     L.pgc_graph_destroy.restype = None
     L.pgc_solve_ratio.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
     L.pgc_solve_min_cost.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
+    L.pgc_solve_ratio_noalloc.argtypes = [P, P, C.c_int, C.c_uint32, P, P, I64, C.POINTER(pgc_result)]
     L.pgc_solve_ratio_batch.argtypes = [P, P, P, I64, C.c_uint32, P, P, P, P]
     L.pgc_graph_set_feat_diff.argtypes = [P, P, C.c_int]
     L.pgc_graphs_set_feat_diff.argtypes = [P, P, I64, C.c_int]
@@ -276,6 +277,10 @@ class Graph:
 
     def solve_ratio(self, max_depth, flags=PGC_LAWLER_PICK_LAST):
         return self._solve(load().pgc_solve_ratio, max_depth, flags)
+
+    def solve_ratio_noalloc(self, max_depth, flags=PGC_LAWLER_PICK_LAST):
+        """solve_ratio with no device allocation (the library's static scratch)."""
+        return self._solve(load().pgc_solve_ratio_noalloc, max_depth, flags)
 
     def solve_min_cost(self, max_depth, flags=0):
         return self._solve(load().pgc_solve_min_cost, max_depth, flags)
