@@ -13,7 +13,8 @@ the same pair, Bellman-Ford GTEPS (edge relaxations / solver-kernel time), the
 roofline of the dominant kernel (the persistent solver kernel, HBM-bound:
 algorithmic bytes per sweep = E*(4 src + 4 cost [+ 4 time]) + V_x*4 parent,
 summed over the sweeps of each launch, divided by the launch's HIP-event
-duration), and the CPU oracle (restated reference CPU path) timed on the host.
+duration), and both CPU solvers of the oracle (the restated lawlercpu and
+dijkstracpu paths) timed on the host beside their GPU counterparts.
 
 The "batched" object is the serving-scale workload: 256 shape pairs of uneven
 size (icosphere subdivision in {2,3,4}, |VY| in [32,160], max_depth in
@@ -63,6 +64,37 @@ def cpu_baseline(p, K, seconds):
     return {"value": n / el, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": "%d complete lawler solves (product space + energy + quantization + BnB) of the same pair, "
                       "%.1f s, oracle/pgc_oracle.cpp with OpenMP" % (n, el)}
+
+
+def cpu_baseline_mincost(p, K, seconds):
+    """The oracle's dijkstracpu restatement (CPU rules) on the same pair."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pgc_oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    O.set_threads(threads)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        rows, src, trgt = O.product_space(p["EX"], p["EY"])
+        en = O.energy(p["VX"], p["VY"], rows, p["feat_diff"].astype(np.float64))
+        O.dijkstra_solve(src, trgt, en[:, 0], len(p["VX"]), K, cpu_rules=True)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and n >= 2:
+            break
+    return {"value": n / el, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": "%d complete dijkstracpu solves (product space + energy + BnB sweep) of the same pair, "
+                      "%.1f s, oracle/pgc_oracle.cpp with OpenMP" % (n, el)}
+
+
+def _profile_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def copy_calibration(mib=256, reps=4):
@@ -135,6 +167,15 @@ def batched(args):
     corrected = None
     if not args.no_corrected:
         corrected = batched_corrected(args, prob, step, cycles, lam.detach().cpu().numpy(), pairs, owner)
+    tr = _profile_traffic("batch_step_c4_traffic.json")
+    b_roof = {"bound": "hbm", "achieved": b_achieved, "peak": 8000.0, "unit": "GB/s",
+              "frac": b_achieved / 8000.0, "algorithmic_bytes": alg_bytes,
+              "traffic": tr["traffic_bytes_per_step"] if tr else None}
+    if args.no_plain:
+        prob.close()
+        return {"workload": "%d uneven pairs x (1 + %d perturbations) = %d lawlergpu solves, forward+backward" % (
+            len(pairs), S_, n), "batched_ms_per_step": b_el / args.batch_steps * 1e3,
+            "batched_solver_kernel_ms": kms, "batched_roofline": b_roof}
 
     # plain loop over the same items: energy from the item's feat_diff, then
     # one pgc_solve_ratio call per item, on one graph per pair
@@ -162,8 +203,7 @@ def batched(args):
         "batched_sweeps": sweeps,
         # the solver launches of one step against the HBM roof; their time is
         # the makespan of the longest item, while most CUs idle near the end
-        "batched_roofline": {"bound": "hbm", "achieved": b_achieved, "peak": 8000.0, "unit": "GB/s",
-                             "frac": b_achieved / 8000.0, "algorithmic_bytes": alg_bytes},
+        "batched_roofline": b_roof,
         "plain_loop_solves_per_s": n / p_el,
         "plain_loop_s": p_el,
         "batched_vs_plain": (n * args.batch_steps / b_el) / (n / p_el),
@@ -238,11 +278,23 @@ def conjugate(steps=2):
     g = runs[-1][0]
     info = g._graph.info()
     r = g.last_result
+    # algorithmic bytes per sweep as for the single pair: 4 B source + 4 B
+    # q-cost per product edge, 4 B parent per expanded node
+    vx = info["nodes_per_layer"] * (info["n_layers"] + 1) * 2
+    bps = info["n_edges"] * 8 + vx * 4
+    achieved = bps * r["sweeps"] / (r["kernel_ms"] * 1e-3) / 1e9
+    tr = _profile_traffic("solve_kernel_c2_traffic.json")
+    launches = r["branches"] + 1
     return {"workload": "conjugate, icosphere s=4 x 100-point curve, max_depth 2, regularised, vanilla/nomode",
             "n_edges": info["n_edges"], "nodes_per_layer": info["nodes_per_layer"],
             "generate_s": sum(x[1] for x in runs) / steps, "lawler_solve_s": sum(x[2] for x in runs) / steps,
             "sweeps": r["sweeps"], "solver_kernel_ms": r["kernel_ms"],
             "gteps": r["relaxations"] / (r["kernel_ms"] * 1e-3) / 1e9,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_sweep": bps,
+                         "algorithmic_bytes_per_launch": bps * r["sweeps"] / launches,
+                         "avg_launch_ms": r["kernel_ms"] / launches,
+                         "traffic": tr["traffic_bytes_per_launch"] if tr else None},
             "distance_table": "global memory (46080 nodes per layer exceed the LDS)"}
 
 
@@ -289,6 +341,9 @@ def main():
                     help="run only the batch item that runs to the sweep cap (for profiling its solver launch)")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched workload (its steps and the plain loop), print its JSON object")
+    ap.add_argument("--no-plain", action="store_true", help="batched workload without the plain per-item loop")
+    ap.add_argument("--conj-only", action="store_true", help="run only the conjugate-graph measurement")
+    ap.add_argument("--conj-steps", type=int, default=2)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -298,6 +353,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(batched(args)))
+        return
+    if args.conj_only:
+        print(json.dumps(conjugate(args.conj_steps)))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path in this build "
@@ -385,11 +443,12 @@ def main():
         print("parity check skipped: %s" % ex, file=sys.stderr)
 
     batch = None if args.no_batch else batched(args)
-    conj = None if args.no_conj else conjugate()
+    conj = None if args.no_conj else conjugate(args.conj_steps)
 
-    cpu = None
+    cpu = cpu_mc = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, K, args.cpu_seconds)
+        cpu_mc = cpu_baseline_mincost(p, K, args.cpu_seconds / 2)
 
     line = {
         "metric": "solves/sec & Bellman-Ford GTEPS; batched fwd+bwd vs plain loop, same cycle",
@@ -414,6 +473,7 @@ def main():
         "solver_phase_ms_per_solve": phases,
         "mincost_solves_per_s": nd / d_el,
         "mincost_ms_per_solve": d_el / nd * 1e3,
+        "mincost_cpu_baseline": cpu_mc,
         "parity_with_oracle": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
