@@ -73,6 +73,9 @@ hipError_t launch_cycle_grad(const CycleItem*, int, hipStream_t);
 hipError_t launch_src_invariant(const int32_t*, int64_t, int64_t, int64_t, int32_t*, hipStream_t);
 // solver.hip
 size_t solver_lds_bytes(int npl, int K);
+size_t grid_state_bytes();
+hipError_t launch_solver_grid(const SolverJob*, int, int, bool, bool, hipStream_t);
+int solver_grid_capacity(int, bool, bool, int);
 hipError_t launch_solver(const SolverJob*, int, int, int*, int, int, bool, bool, bool, size_t, hipStream_t);
 hipError_t launch_lookup(const SolverJob*, const int32_t*, int, int32_t*, bool, bool, hipStream_t);
 }  // namespace pgc
@@ -272,6 +275,7 @@ struct JobScratch {
   double* lb_key = nullptr;
   double* backup = nullptr;
   double* dtab = nullptr;  // distance table in global memory (graphs beyond the LDS)
+  int32_t* grid = nullptr;  // GridState of the device-wide solve (grid_state_bytes())
   JobResult* res = nullptr;
   SolverJob* djob = nullptr;
   // device-resident branch and bound (batched solves)
@@ -282,7 +286,7 @@ struct JobScratch {
   void release() {
     if (pred) (void)hipDeviceSynchronize();  // queued solver work may still use the scratch
     void* ps[] = {pred, wnext, wlen, chain, tmp_nodes, tmp_eid, neg_nodes, neg_eid, neg_len, tmp_c, tmp_t,
-                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, dtab, res, djob,
+                  neg_c, neg_t, hits, back, branch, nodes_io, eids_io, lb_key, backup, dtab, grid, res, djob,
                   hkey, hval, hwhere, vslot, vc_eid, vc_len, vc_c, vc_t};
     for (void* p : ps)
       if (p) cache_free(p);
@@ -661,7 +665,10 @@ int ensure_scratch(pgc_graph* g, int K, DevAlloc& A, bool min_cost = true) {
     HIPCHK(A.get(&s.eids_io, cap));
     HIPCHK(A.get(&s.lb_key, npl));
   }
-  if (kernel_maxc(g, K) == 0) HIPCHK(A.get(&s.dtab, (int64_t)(2 * K + 1) * (npl + 1)));
+  if (kernel_maxc(g, K) == 0) {
+    HIPCHK(A.get(&s.dtab, (int64_t)(2 * K + 1) * (npl + 1)));
+    HIPCHK(A.get(&s.grid, (grid_state_bytes() + 3) / 4));
+  }
   return PGC_OK;
 }
 
@@ -750,6 +757,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.hwhere = s.hwhere;
   j.vslot = s.vslot;
   j.dtab = s.dtab;
+  j.grid = (GridState*)s.grid;
   j.pred_lds = pred_fits_lds(g, K) ? 1 : 0;
   j.wrap_lds = wrap_fits_lds(g, K) ? 1 : 0;
   j.org_lds = org_fits_lds(g, K) ? 1 : 0;
@@ -775,15 +783,41 @@ struct Timer {
 // one solver launch: upload job, run, read result (kernel time accumulated)
 static thread_local double g_phase_ms[5];
 
+// Workgroups of the device-wide solve of a job (0: one workgroup). Graphs
+// whose table lives in global memory and that have at least 128 chunks of 64
+// nodes per layer are solved by ceil(chunks / 16) workgroups, one chunk per
+// wave (C2's 720 chunks: 45 workgroups); smaller ones keep one workgroup, as
+// a device-wide barrier per step would cost more than their steps take.
+// PGC_GRID=0 turns it off; PGC_GRID=n forces n workgroups for every such job
+// (a validation switch for small graphs).
+int grid_blocks(const pgc_graph* g, int K, bool mincost) {
+  if (kernel_maxc(g, K) != 0 || g->maxw > 32) return 0;
+  static const int env = [] {
+    const char* e = std::getenv("PGC_GRID");
+    return e ? std::atoi(e) : -1;
+  }();
+  if (env == 0) return 0;
+  int n = env > 0 ? env : (g->nChunks >= 128 ? (g->nChunks + 15) / 16 : 0);
+  if (n <= 0) return 0;
+  const int cap = solver_grid_capacity(g->maxw, mincost, g->d_time[0] != nullptr, std::max(1, g->ctx->num_cu));
+  n = std::min(n, std::min(cap, 256));
+  return n >= 1 ? n : 0;
+}
+
 int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, double& kms) {
   pgc_context* c = g->ctx;
   hipStream_t st = c->stream;
   HIPCHK(hipMemcpyAsync(g->js.djob, &j, sizeof(j), hipMemcpyHostToDevice, st));
   const int maxc = kernel_maxc(g, j.K);
   const size_t lds = job_lds_bytes(g, j.K);
+  const int nblk = j.grid ? grid_blocks(g, j.K, mincost) : 0;
+  if (nblk > 0) HIPCHK(launch_fill_i32(g->js.grid, (int64_t)((grid_state_bytes() + 3) / 4), 0, st));
   HIPCHK(hipEventRecord(c->ev0, st));
-  HIPCHK(launch_solver(g->js.djob, 1, 1, nullptr, maxc, g->maxw, mincost, g->d_time[0] != nullptr,
-                       g->cost_kind == 2, lds, st));
+  if (nblk > 0)
+    HIPCHK(launch_solver_grid(g->js.djob, nblk, g->maxw, mincost, g->d_time[0] != nullptr, st));
+  else
+    HIPCHK(launch_solver(g->js.djob, 1, 1, nullptr, maxc, g->maxw, mincost, g->d_time[0] != nullptr,
+                         g->cost_kind == 2, lds, st));
   HIPCHK(hipEventRecord(c->ev1, st));
   HIPCHK(hipMemcpyAsync(&out, g->js.res, sizeof(out), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

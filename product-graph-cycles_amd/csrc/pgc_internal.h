@@ -63,7 +63,10 @@ enum JobError : int32_t {
   ERR_SWEEP_CAP = 5,
   ERR_VALID_STORE = 6,  // more valid branches than the job's cycle store holds
   ERR_BRANCH_CAP = 7,   // safety cap on branches of one solve (4*npl + 64)
+  ERR_GRID_SYNC = 8,    // a device-wide barrier waited past its time limit
 };
+
+struct GridState;
 
 // flags
 enum : uint32_t {
@@ -150,6 +153,7 @@ struct SolverJob {
   int32_t org_lds;        // 1 (K == 1, pred and wrap map in LDS): two rows of chain origins in LDS
                           // after the wrap map, kept by the sweep, replace the wrap chase
   double* dtab;           // [(2K+1)*(npl+1)] distance table in global memory (graphs beyond the LDS)
+  GridState* grid;        // barrier and broadcast state of a device-wide solve (cooperative launch)
 };
 
 // One cycle of the raw-unit ratio value / gradient kernels.

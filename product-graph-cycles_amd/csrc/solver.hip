@@ -9,7 +9,33 @@ hipError_t launch_solver_ratio(int, int, const SolverJob*, int, int, int*, size_
 hipError_t launch_solver_ratio_time(int, int, const SolverJob*, int, int, int*, size_t, hipStream_t);
 hipError_t launch_solver_mincost(int, int, const SolverJob*, int, int, int*, size_t, hipStream_t);
 
+hipError_t launch_solver_grid_ratio(int, const SolverJob*, int, hipStream_t);
+hipError_t launch_solver_grid_ratio_time(int, const SolverJob*, int, hipStream_t);
+hipError_t launch_solver_grid_mincost(int, const SolverJob*, int, hipStream_t);
+int solver_grid_capacity_ratio(int, int);
+int solver_grid_capacity_ratio_time(int, int);
+int solver_grid_capacity_mincost(int, int);
+
 size_t solver_lds_bytes(int npl, int K) { return (size_t)(2 * K + 1) * (npl + 1) * sizeof(double); }
+
+size_t grid_state_bytes() { return sizeof(GridState); }
+
+// Device-wide solve of one job with its table in global memory (J.grid zeroed
+// by the caller): nblk workgroups of one cooperative launch.
+hipError_t launch_solver_grid(const SolverJob* d_job, int nblk, int maxw, bool mincost, bool hasTime,
+                              hipStream_t st) {
+  if (nblk < 1 || nblk > kGridMaxBlocks || maxw > 32) return hipErrorInvalidValue;
+  if (mincost) return launch_solver_grid_mincost(maxw, d_job, nblk, st);
+  if (hasTime) return launch_solver_grid_ratio_time(maxw, d_job, nblk, st);
+  return launch_solver_grid_ratio(maxw, d_job, nblk, st);
+}
+
+// Workgroups of that launch that the device holds at once.
+int solver_grid_capacity(int maxw, bool mincost, bool hasTime, int numCU) {
+  if (mincost) return solver_grid_capacity_mincost(maxw, numCU);
+  if (hasTime) return solver_grid_capacity_ratio_time(maxw, numCU);
+  return solver_grid_capacity_ratio(maxw, numCU);
+}
 
 // Solver launch: grid workgroups over njobs jobs (one job per workgroup when
 // queue is null, else a persistent launch on the queue counter). maxw = the

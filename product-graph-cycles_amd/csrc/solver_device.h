@@ -62,6 +62,7 @@ __device__ __forceinline__ int round_to_int(double w) {
 
 struct Shared {
   int orCount[2];  // barrier_or counters, alternating by parity
+  int gridFlag;    // device-wide solve: this workgroup has a true predicate (GridOr)
   int i0, i1, i2, i3;
   double d0, d1, d2;
   int status;
@@ -101,6 +102,84 @@ struct BarrierOr {
 };
 
 __device__ __forceinline__ uint64_t wclock() { return wall_clock64(); }
+
+// ------------------------------------------------------------ device-wide solve
+// Graphs whose distance rows exceed the LDS can be solved by all workgroups of
+// a cooperative launch together: every (layer, depth) step of a sweep is
+// spread over the waves of the whole grid (one 64-vertex chunk per wave), and
+// the steps are separated by device-wide barriers. The control flow (Lawler
+// iterations, bounds, cycle walks) runs in every workgroup alike from values
+// that one thread writes to GridState::sh and all threads read after a
+// barrier.
+constexpr int kGridMaxBlocks = 256;
+
+struct GridState {
+  unsigned long long arrive;  // barrier arrivals (monotonic, zero at launch)
+  unsigned int orc[2];        // device-wide OR counters, alternating by parity
+  unsigned int abort;         // a barrier waited past its time limit: every workgroup stops
+  unsigned int pad;
+  Shared sh;                  // broadcast values (written by the grid's first thread)
+  int redI[2][kGridMaxBlocks];     // per-workgroup partials of a reduction, two banks
+  double redD[2][kGridMaxBlocks];
+};
+
+// Device-wide barrier: thread 0 of every workgroup adds one arrival to a
+// monotonic counter and waits until all workgroups have arrived at this
+// barrier (its ordinal times the grid size). Device-scope fences release the
+// writes before it and acquire the other workgroups' writes after it (the
+// per-XCD L2 caches are not coherent with each other otherwise). A wait of
+// more than two seconds (the wall clock runs at 100 MHz) sets the abort flag
+// instead of hanging; every later barrier then returns at once.
+__device__ __forceinline__ bool grid_sync(GridState* gs, uint64_t& bgen, uint32_t nblk) {
+  __syncthreads();
+  bgen += 1;
+  if (threadIdx.x == 0) {
+    const unsigned long long target = (unsigned long long)bgen * nblk;
+    __threadfence();
+    __hip_atomic_fetch_add(&gs->arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t0 = wall_clock64();
+    while (__hip_atomic_load(&gs->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      if (wall_clock64() - t0 > 200000000ull) {
+        __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+  return __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+}
+
+// Device-wide OR of a predicate: a workgroup with a true predicate in any wave
+// adds one to the counter of the current parity before the barrier; after it
+// every thread compares the counter with its previous reading. Counters
+// alternate by parity, as in BarrierOr.
+struct GridOr {
+  unsigned int prev0 = 0, prev1 = 0;
+  bool odd = false;
+  __device__ __forceinline__ bool operator()(GridState* gs, uint64_t& bgen, uint32_t nblk, int* ldsFlag, bool pred) {
+    if (__ballot(pred) && (threadIdx.x & 63) == 0) *(volatile int*)ldsFlag = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && *(volatile int*)ldsFlag) {
+      __hip_atomic_fetch_add(&gs->orc[odd ? 1 : 0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *(volatile int*)ldsFlag = 0;
+    }
+    grid_sync(gs, bgen, nblk);
+    const unsigned int c = __hip_atomic_load(&gs->orc[odd ? 1 : 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool any;
+    if (odd) {
+      any = c != prev1;
+      prev1 = c;
+    } else {
+      any = c != prev0;
+      prev0 = c;
+    }
+    odd = !odd;
+    return any;
+  }
+};
 
 }  // namespace pgc
 
@@ -224,9 +303,10 @@ __device__ __forceinline__ bool lookup_edge(const JT& J, int a, int b, int& eid,
 // MAXC = chunks (64-vertex slices) per wave, unrolled; MAXC == 0: any number
 // of chunks per wave in runtime loops, with the distance table in global
 // memory (graphs whose table exceeds the LDS).
-template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W>
+template <typename CT, bool HAS_TIME, bool MINCOST, int MAXC, int W, bool GRID = false>
 struct Solver {
   static constexpr bool DYN = MAXC == 0;
+  static_assert(!GRID || DYN, "the device-wide solve keeps its table in global memory");
   // The unrolled sweep (slots of the next block prefetched into registers)
   // pays off with one chunk per wave and 8 slots per vertex; with more, its
   // register sets spill, and the sweep that takes one chunk at a time (rows
@@ -248,6 +328,95 @@ struct Solver {
   int npl, ds, K, L, nC, wave, lane, nw, npass;
   BarrierOr bor;
   uint64_t tSweep = 0, tFold = 0, tChase = 0, tWalk = 0, tExtract = 0;
+  // device-wide solve (GRID): sh is GridState::sh in global memory, lsh the
+  // workgroup's own LDS block
+  GridState* gs = nullptr;
+  PGC_LDS Shared* lsh = nullptr;
+  uint64_t bgen = 0;
+  uint32_t nblk = 1;
+  GridOr gor;
+  int rbank = 0;
+  bool alive = true;
+
+  // workgroup barrier, or the device-wide one
+  __device__ __forceinline__ void sync() {
+    if constexpr (GRID)
+      alive = grid_sync(gs, bgen, nblk) && alive;
+    else
+      __syncthreads();
+  }
+  // the thread that runs the sequential parts
+  __device__ __forceinline__ bool leader() const {
+    if constexpr (GRID) return blockIdx.x == 0 && threadIdx.x == 0;
+    return threadIdx.x == 0;
+  }
+  // strided loops over all threads of the solve
+  __device__ __forceinline__ int tfirst() const {
+    if constexpr (GRID) return (int)(blockIdx.x * PGC_BLOCK + threadIdx.x);
+    return (int)threadIdx.x;
+  }
+  __device__ __forceinline__ int tstride() const {
+    if constexpr (GRID) return (int)(gridDim.x * PGC_BLOCK);
+    return PGC_BLOCK;
+  }
+  // OR of a predicate over all threads of the solve
+  __device__ __forceinline__ bool any(bool pred) {
+    if constexpr (GRID) {
+      const bool a = gor(gs, bgen, nblk, (int*)&lsh->gridFlag, pred);
+      alive = alive && __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+      return a;
+    }
+    return bor(sh, pred);
+  }
+  // reductions over all threads of the solve: the workgroup's partial, then
+  // (GRID) the partials of all workgroups in workgroup order, two banks in turn
+  __device__ __forceinline__ int rsum(int x) {
+    if constexpr (GRID) {
+      const int part = block_sum(*(Shared*)lsh, x);
+      if (threadIdx.x == 0) gs->redI[rbank][blockIdx.x] = part;
+      sync();
+      int t = 0;
+      for (uint32_t b = 0; b < nblk; ++b) t += gs->redI[rbank][b];
+      rbank ^= 1;
+      return t;
+    }
+    return block_sum(sh, x);
+  }
+  __device__ __forceinline__ int rmax(int x) {
+    if constexpr (GRID) {
+      const int part = block_max(*(Shared*)lsh, x);
+      if (threadIdx.x == 0) gs->redI[rbank][blockIdx.x] = part;
+      sync();
+      int t = gs->redI[rbank][0];
+      for (uint32_t b = 1; b < nblk; ++b) t = max(t, gs->redI[rbank][b]);
+      rbank ^= 1;
+      return t;
+    }
+    return block_max(sh, x);
+  }
+  __device__ __forceinline__ void rmin_key(double& key, int& v) {
+    if constexpr (GRID) {
+      block_min_key(*(Shared*)lsh, key, v);
+      if (threadIdx.x == 0) {
+        gs->redD[rbank][blockIdx.x] = key;
+        gs->redI[rbank][blockIdx.x] = v;
+      }
+      sync();
+      key = gs->redD[rbank][0];
+      v = gs->redI[rbank][0];
+      for (uint32_t b = 1; b < nblk; ++b) {
+        const double k2 = gs->redD[rbank][b];
+        const int v2 = gs->redI[rbank][b];
+        if (k2 < key || (k2 == key && v2 < v)) {
+          key = k2;
+          v = v2;
+        }
+      }
+      rbank ^= 1;
+      return;
+    }
+    block_min_key(sh, key, v);
+  }
 
   __device__ __forceinline__ Solver(const PGC_CONST SolverJob& j, double* d, double* dm, Shared& s, int32_t* pl)
       : J(j), D(d), Dmin(dm), sh(s) {
@@ -263,9 +432,14 @@ struct Solver {
     K = J.K;
     L = J.L;
     nC = J.nChunks;
-    wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     lane = threadIdx.x & 63;
-    nw = PGC_BLOCK / PGC_WAVE;
+    if constexpr (GRID) {
+      wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (PGC_BLOCK / PGC_WAVE) + (threadIdx.x >> 6)));
+      nw = (int)gridDim.x * (PGC_BLOCK / PGC_WAVE);
+    } else {
+      wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+      nw = PGC_BLOCK / PGC_WAVE;
+    }
     npass = DYN ? (nC + nw - 1) / nw : MAXC;
     fdNpl.init((uint32_t)npl);
     fdPer.init((uint32_t)(npl * K));
@@ -307,7 +481,31 @@ struct Solver {
   // gets the whole register file; the caller's state is saved once per sweep).
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     SweepState r;
-    if constexpr (DYN) {
+    if constexpr (GRID) {
+      constexpr int SX = W == 8 ? 7 : (W == 16 ? 12 : W), SI = W == 8 || W == 16 ? 6 : W;
+      // with at most one chunk per wave the sources of a wave's chunk stay in
+      // registers for the whole sweep (layer-invariant sources)
+      const bool one = nC <= nw;
+      const bool shortSlots = SX < W && J.inter.deg <= SX && J.intra.deg <= SI;
+      int p0 = (int)gor.prev0, p1 = (int)gor.prev1, od = gor.odd ? 1 : 0;
+      if (one && shortSlots)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 1, SX, SI, true>(
+            (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
+      else if (one)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 1, W, W, true>(
+            (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
+      else if (shortSlots)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 0, SX, SI, true>(
+            (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
+      else
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 0, W, W, true>(
+            (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
+      gor.prev0 = (unsigned)r.p0;
+      gor.prev1 = (unsigned)r.p1;
+      gor.odd = r.odd != 0;
+      alive = alive && __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+      return r.complete;
+    } else if constexpr (DYN) {
       // short slots for the interlayer / intralayer classes: 7 / 6 of 8 (icosphere
       // product graphs), 12 / 6 of 16 (conjugate product graphs: in-degrees <= 10 / 5)
       constexpr int SX = W == 8 ? 7 : (W == 16 ? 12 : W), SI = W == 8 || W == 16 ? 6 : W;
@@ -353,9 +551,9 @@ struct Solver {
 
   // D <- inf, branch sources <- 0, pred of sources <- -1; returns #sources.
   __device__ __forceinline__ int setup(int bid, int onlyV) {
-    for (int i = threadIdx.x; i < 2 * K * ds; i += PGC_BLOCK) D[i] = PGC_INF;
-    for (int i = threadIdx.x; i < ds; i += PGC_BLOCK) Dmin[i] = PGC_INF;
-    __syncthreads();
+    for (int i = tfirst(); i < 2 * K * ds; i += tstride()) D[i] = PGC_INF;
+    for (int i = tfirst(); i < ds; i += tstride()) Dmin[i] = PGC_INF;
+    sync();
     int cnt = 0, last = -1;
     inb = 0;
 #pragma unroll
@@ -371,13 +569,13 @@ struct Solver {
         last = max(last, v);
       }
     }
-    const int n = block_sum(sh, cnt);
-    const int lv = block_max(sh, last);
-    if (threadIdx.x == 0) {
+    const int n = rsum(cnt);
+    const int lv = rmax(last);
+    if (leader()) {
       sh.numInBranch = n;
       sh.indexNode = lv;
     }
-    __syncthreads();
+    sync();
     return n;
   }
 
@@ -419,7 +617,7 @@ struct Solver {
         D[v] = bk;
       }
     }
-    return bor(sh, imp);
+    return any(imp);
   }
 
   // Chase every layer-0 vertex's predecessor chain back to the previous
@@ -614,7 +812,7 @@ struct Solver {
   // edge lookups for tmp_nodes[0..n) -> tmp_eid/c/t[0..n-1); returns ok (block-uniform)
   __device__ __forceinline__ bool lookup_pairs(int n) {
     bool bad = false;
-    for (int i = threadIdx.x; i + 1 < n; i += PGC_BLOCK) {
+    for (int i = tfirst(); i + 1 < n; i += tstride()) {
       int eid = -1;
       double c = 0, t = 0;
       if (!lookup_edge<CT, HAS_TIME>(J, gp(J.tmp_nodes)[i], gp(J.tmp_nodes)[i + 1], eid, c, t)) bad = true;
@@ -622,11 +820,11 @@ struct Solver {
       gp(J.tmp_c)[i] = c;
       gp(J.tmp_t)[i] = t;
     }
-    return !bor(sh, bad);
+    return !any(bad);
   }
 
   __device__ __forceinline__ void copy_tmp_to_neg(int n) {
-    for (int i = threadIdx.x; i < n; i += PGC_BLOCK) {
+    for (int i = tfirst(); i < n; i += tstride()) {
       gp(J.neg_nodes)[i] = gp(J.tmp_nodes)[i];
       if (i + 1 < n) {
         gp(J.neg_eid)[i] = gp(J.tmp_eid)[i];
@@ -634,19 +832,19 @@ struct Solver {
         gp(J.neg_t)[i] = gp(J.tmp_t)[i];
       }
     }
-    if (threadIdx.x == 0) *gp(J.neg_len) = n;
-    __syncthreads();
+    if (leader()) *gp(J.neg_len) = n;
+    sync();
   }
 
   // ZERO_CYCLE if the current cycle has (rounded) zero reweighted cost, else NO_NEGATIVE_CYCLE
   __device__ __forceinline__ int zero_or_none(double ratio) {
-    if (threadIdx.x == 0) {
+    if (leader()) {
       const int n = *gp(J.neg_len);
       double cs, ts, ws;
       pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
       sh.status = round_to_int(ws) == 0 ? ST_ZERO_CYCLE : ST_NO_NEGATIVE_CYCLE;
     }
-    __syncthreads();
+    sync();
     return sh.status;
   }
 
@@ -672,11 +870,15 @@ struct Solver {
       const int complete = run_layers(ratio, ub);
       uint64_t t1 = wclock();
       tSweep += t1 - t0;
+      if (GRID && !alive) {
+        err = ERR_GRID_SYNC;
+        return ST_NO_NEGATIVE_CYCLE;
+      }
       if (!complete) return zero_or_none(ratio);
 
       if (ubSearch) {
         // path from the closing copy of the start vertex back to it (:370-397)
-        if (threadIdx.x == 0) {
+        if (leader()) {
           const int start = sh.indexNode;
           const int64_t maxLength = (int64_t)L * K;
           int cnt = 0;
@@ -692,24 +894,24 @@ struct Solver {
           }
           sh.i0 = cnt;
         }
-        __syncthreads();
+        sync();
         const int cnt = sh.i0;
         // reverse the path in place: each pair (i, cnt-1-i) belongs to one thread
-        for (int i = threadIdx.x; i < cnt / 2; i += PGC_BLOCK) {
+        for (int i = tfirst(); i < cnt / 2; i += tstride()) {
           const int a = gp(J.tmp_nodes)[i], b = gp(J.tmp_nodes)[cnt - 1 - i];
           gp(J.tmp_nodes)[i] = b;
           gp(J.tmp_nodes)[cnt - 1 - i] = a;
         }
-        __syncthreads();
+        sync();
         if (!lookup_pairs(cnt)) err = ERR_EDGE_LOOKUP;
         copy_tmp_to_neg(cnt);
-        if (threadIdx.x == 0) {
+        if (leader()) {
           double cs, ts, ws;
           pair_sums(J.neg_c, J.neg_t, cnt - 1, ratio, cs, ts, ws);
           const int w = round_to_int(ws);
           sh.status = w == 0 ? ST_ZERO_CYCLE : (w < 0 ? ST_FOUND_NEGATIVE_CYCLE : ST_NO_NEGATIVE_CYCLE);
         }
-        __syncthreads();
+        sync();
         return sh.status;
       }
 
@@ -720,7 +922,7 @@ struct Solver {
       if (!folded) return zero_or_none(ratio);
 
       chase_wraps();
-      __syncthreads();
+      sync();
       t0 = wclock();
       tChase += t0 - t1;
 
@@ -740,12 +942,12 @@ struct Solver {
       // minima) only when some thread has a candidate: in long Bellman-Ford
       // calls no walk closes in most sweeps, and then every thread holds
       // (+inf, none) already, which is the minimum
-      if (bor(sh, bestV != 0x7fffffff)) block_min_key(sh, bestKey, bestV);
+      if (any(bestV != 0x7fffffff)) rmin_key(bestKey, bestV);
       t1 = wclock();
       tWalk += t1 - t0;
       if (bestV != 0x7fffffff) {
         // extract the cycle through v* (first repeated vertex of bestV's walk)
-        if (threadIdx.x == 0) {
+        if (leader()) {
           int vs = -1;
           walk_from(bestV, capSteps, vs);
           int64_t m = 0;
@@ -757,14 +959,14 @@ struct Solver {
           sh.i0 = vs;
           sh.i1 = (m + 1 <= J.cyc_cap) ? (int)m : -1;
         }
-        __syncthreads();
+        sync();
         const int vs = sh.i0;
         const int m = sh.i1;
         if (m < 0) {
           err = ERR_CYCLE_TOO_LONG;
           return ST_NO_NEGATIVE_CYCLE;
         }
-        if (threadIdx.x == 0) {
+        if (leader()) {
           // re-walk each wrap's predecessor chain (unchanged since the
           // chase), writing original node ids back to front
           int x = vs, offp = 0, guard = 0;
@@ -780,19 +982,19 @@ struct Solver {
           } while (x != vs && x >= 0 && offp < m && ++guard < 4 * npl + 8);
           gp(J.tmp_nodes)[m] = vs + (int)J.numNodes;
         }
-        __syncthreads();
+        sync();
         const int nn = m + 1;
         if (!lookup_pairs(nn)) {
           err = ERR_EDGE_LOOKUP;
           return ST_NO_NEGATIVE_CYCLE;
         }
-        if (threadIdx.x == 0) {
+        if (leader()) {
           double cs, ts, ws;
           pair_sums(J.tmp_c, J.tmp_t, nn - 1, ratio, cs, ts, ws);
           sh.i2 = round_to_int(ws);
           sh.d0 = cs / ts;
         }
-        __syncthreads();
+        sync();
         tExtract += wclock() - t1;
         const int w = sh.i2;
         if (w < 0) {
@@ -805,6 +1007,10 @@ struct Solver {
         return zero_or_none(ratio);
       }
       iter++;
+      if (GRID && !alive) {
+        err = ERR_GRID_SYNC;
+        return ST_NO_NEGATIVE_CYCLE;
+      }
       if (iter >= J.numNodes - 1) return ST_NO_NEGATIVE_CYCLE;
       if (sweeps >= J.max_sweeps) {
         err = ERR_SWEEP_CAP;
@@ -863,6 +1069,7 @@ struct Solver {
       bfc++;
       const int st = bellman_ford(ratio, false, false, bid, -1, sweeps, err);
       lastStatus = st;
+      if (GRID && !alive) err = ERR_GRID_SYNC;
       if (err != ERR_NONE) break;
       if (st == ST_ZERO_CYCLE || st == ST_NO_NEGATIVE_CYCLE) break;
       ratio = sh.d0;
@@ -903,7 +1110,7 @@ struct Solver {
     int err = ERR_NONE;
     double ratio;
     const int st = lawler_core(J.branchId, J.ratio_in, ratio, sweeps, bfc, its, err);
-    if (threadIdx.x == 0) {
+    if (leader()) {
       int nw_ = 0, nh = 0;
       wraps_and_hits(nw_, nh);
       write_result(st, err, ratio, 0, nw_, nh, -1, sweeps, bfc, its);
@@ -919,7 +1126,7 @@ struct Solver {
     // copy (accumulated in ascending edge id), among vertices with > 3 in-edges
     double bestKey = PGC_INF;
     int bestV = 0x7fffffff;
-    for (int v = threadIdx.x; v < npl; v += PGC_BLOCK) {
+    for (int v = tfirst(); v < npl; v += tstride()) {
       const int node_layer = 1;
       float mean = 0.0f;
       int n = 0;
@@ -988,7 +1195,7 @@ struct Solver {
         bestV = v;
       }
     }
-    block_min_key(sh, bestKey, bestV);
+    rmin_key(bestKey, bestV);
     const int start = (bestV == 0x7fffffff) ? 0 : bestV;
     startV = start;
     double ratio = 10.0;
@@ -998,13 +1205,13 @@ struct Solver {
     // computeRatioForCurrentCycle: cs / ts of the same cycle, which does not
     // depend on the doubled ratio. The loop only decides when to stop, so
     // lambda_UB is that quotient, computed once.
-    if (threadIdx.x == 0) {
+    if (leader()) {
       const int n = *gp(J.neg_len);
       double cs, ts, ws;
       pair_sums(J.neg_c, J.neg_t, n > 0 ? n - 1 : 0, ratio, cs, ts, ws);
       sh.d1 = (double)(float)(cs / ts);
     }
-    __syncthreads();
+    sync();
     return sh.d1;
   }
 
@@ -1012,7 +1219,7 @@ struct Solver {
     int64_t sweeps = 0;
     int err = ERR_NONE, start = 0;
     const double r_ub = upper_bound_core(sweeps, err, start);
-    if (threadIdx.x == 0) {
+    if (leader()) {
       write_result(ST_NO_NEGATIVE_CYCLE, err, r_ub, r_ub, 0, 0, start, sweeps, 1, 0);
       PGC_GLOBAL JobResult& r = *gp(J.res);
       r.num_in_branch = 1;
@@ -1227,9 +1434,10 @@ struct Solver {
         gp(J.wlen)[v] = cnt < stride ? cnt : stride;
       }
     }
-    __syncthreads();
+    sync();
     tChase += wclock() - tc0;
-    if (threadIdx.x == 0) {
+    if (GRID && !alive) err = ERR_GRID_SYNC;
+    if (leader()) {
       PGC_GLOBAL JobResult& r = *gp(J.res);
       r.status = status;
       r.error = err;
@@ -1309,6 +1517,34 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel(const SolverJob* __res
     if (!queue) break;
     next += gridDim.x;
   }
+}
+
+// The device-wide solve of one job (upper bound, Lawler subroutine or
+// min-cost sweep) by every workgroup of a cooperative launch; J.grid is
+// zeroed before the launch.
+template <typename CT, bool HAS_TIME, bool MINCOST, int W>
+__global__ void __launch_bounds__(PGC_BLOCK) solve_kernel_grid(const SolverJob* __restrict__ jobs) {
+  __shared__ Shared lsh;
+  if (threadIdx.x == 0) {
+    lsh.orCount[0] = 0;
+    lsh.orCount[1] = 0;
+    lsh.gridFlag = 0;
+  }
+  __syncthreads();
+  const PGC_CONST SolverJob& J = ((const PGC_CONST SolverJob*)jobs)[0];
+  GridState* gs = J.grid;
+  double* D = J.dtab;
+  double* Dmin = D + 2 * J.K * (J.npl + 1);
+  Solver<CT, HAS_TIME, MINCOST, 0, W, true> s(J, D, Dmin, gs->sh, nullptr);
+  s.gs = gs;
+  s.lsh = (PGC_LDS Shared*)&lsh;
+  s.nblk = gridDim.x;
+  if (MINCOST)
+    s.min_cost();
+  else if (J.mode == MODE_UPPER_BOUND)
+    s.upper_bound();
+  else
+    s.lawler();
 }
 
 // Edge lookups for an arbitrary node list (min-cost path output).

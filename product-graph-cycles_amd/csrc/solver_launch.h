@@ -41,4 +41,41 @@ static hipError_t launch_width(int maxw, int maxc, const SolverJob* jobs, int nj
   return launch_maxc<CT, HT, MC, 32>(maxc, jobs, njobs, grid, queue, lds, st);
 }
 
+// The device-wide solve of one job: a cooperative launch of nblk workgroups
+// (all resident at once, which the runtime checks).
+template <typename CT, bool HT, bool MC, int W>
+static hipError_t launch_grid_one(const SolverJob* jobs, int nblk, hipStream_t st) {
+  void* args[] = {(void*)&jobs};
+  return hipLaunchCooperativeKernel((const void*)solve_kernel_grid<CT, HT, MC, W>, dim3(nblk), dim3(PGC_BLOCK), args,
+                                    0, st);
+}
+
+// Workgroups of the device-wide solve that fit the device at once.
+template <typename CT, bool HT, bool MC, int W>
+static int grid_capacity_one(int numCU) {
+  static int per_cu = -1;
+  if (per_cu < 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void*)solve_kernel_grid<CT, HT, MC, W>, PGC_BLOCK,
+                                                     0) != hipSuccess)
+      n = 0;
+    per_cu = n;
+  }
+  return per_cu * numCU;
+}
+
+template <typename CT, bool HT, bool MC>
+static hipError_t launch_grid(int maxw, const SolverJob* jobs, int nblk, hipStream_t st) {
+  if (maxw <= 8) return launch_grid_one<CT, HT, MC, 8>(jobs, nblk, st);
+  if (maxw <= 16) return launch_grid_one<CT, HT, MC, 16>(jobs, nblk, st);
+  return launch_grid_one<CT, HT, MC, 32>(jobs, nblk, st);
+}
+
+template <typename CT, bool HT, bool MC>
+static int grid_capacity(int maxw, int numCU) {
+  if (maxw <= 8) return grid_capacity_one<CT, HT, MC, 8>(numCU);
+  if (maxw <= 16) return grid_capacity_one<CT, HT, MC, 16>(numCU);
+  return grid_capacity_one<CT, HT, MC, 32>(numCU);
+}
+
 }  // namespace pgc

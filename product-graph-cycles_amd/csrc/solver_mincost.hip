@@ -9,4 +9,10 @@ hipError_t launch_solver_mincost(int maxw, int maxc, const SolverJob* jobs, int 
   return launch_width<double,false,true>(maxw, maxc, jobs, njobs, grid, queue, lds, st);
 }
 
+hipError_t launch_solver_grid_mincost(int maxw, const SolverJob* jobs, int nblk, hipStream_t st) {
+  return launch_grid<double,false,true>(maxw, jobs, nblk, st);
+}
+
+int solver_grid_capacity_mincost(int maxw, int numCU) { return grid_capacity<double,false,true>(maxw, numCU); }
+
 }  // namespace pgc

@@ -9,4 +9,10 @@ hipError_t launch_solver_ratio(int maxw, int maxc, const SolverJob* jobs, int nj
   return launch_width<float,false,false>(maxw, maxc, jobs, njobs, grid, queue, lds, st);
 }
 
+hipError_t launch_solver_grid_ratio(int maxw, const SolverJob* jobs, int nblk, hipStream_t st) {
+  return launch_grid<float,false,false>(maxw, jobs, nblk, st);
+}
+
+int solver_grid_capacity_ratio(int maxw, int numCU) { return grid_capacity<float,false,false>(maxw, numCU); }
+
 }  // namespace pgc

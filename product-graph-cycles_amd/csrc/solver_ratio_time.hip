@@ -9,4 +9,10 @@ hipError_t launch_solver_ratio_time(int maxw, int maxc, const SolverJob* jobs, i
   return launch_width<float,true,false>(maxw, maxc, jobs, njobs, grid, queue, lds, st);
 }
 
+hipError_t launch_solver_grid_ratio_time(int maxw, const SolverJob* jobs, int nblk, hipStream_t st) {
+  return launch_grid<float,true,false>(maxw, jobs, nblk, st);
+}
+
+int solver_grid_capacity_ratio_time(int maxw, int numCU) { return grid_capacity<float,true,false>(maxw, numCU); }
+
 }  // namespace pgc

@@ -545,9 +545,14 @@ struct DistPtr<true> {
 // WX, WI: interlayer and intralayer slots a step reads per node (below W
 // when the class's largest in-degree leaves the last slots padding on every
 // node; a padding slot never wins, so skipping it changes no result).
-template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W>
+// GRID: the device-wide sweep (cooperative launch): wave and nw number the
+// waves of the whole grid, the steps end in device-wide barriers and the
+// reachability test in a device-wide OR.
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W,
+          bool GRID = false>
 struct SweepG {
   static_assert(WX <= W && WI <= W, "short slots");
+  static_assert(!(GRID && LD), "the device-wide sweep keeps its rows in global memory");
   typedef typename DistPtr<LD>::type DP;
   const PGC_GLOBAL SolverJob* J;
   DP D;
@@ -559,8 +564,31 @@ struct SweepG {
   SlotClass Sa, Sx;
   int npl, ds, K, L, nC, wave, lane, nw;
   int p0, p1, odd;
+  GridState* gs;  // GRID only
+  uint64_t bgen;
+  uint32_t nblk;
+  bool alive = true;  // GRID: no barrier has timed out
+
+  __device__ __forceinline__ void sync() {
+    if constexpr (GRID)
+      alive = grid_sync(gs, bgen, nblk) && alive;
+    else
+      __syncthreads();
+  }
 
   __device__ __forceinline__ bool bor(bool pred_) {
+    if constexpr (GRID) {
+      GridOr g;
+      g.prev0 = (unsigned)p0;
+      g.prev1 = (unsigned)p1;
+      g.odd = odd != 0;
+      const bool any = g(gs, bgen, nblk, (int*)&sh->gridFlag, pred_);
+      p0 = (int)g.prev0;
+      p1 = (int)g.prev1;
+      odd = g.odd ? 1 : 0;
+      alive = alive && __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+      return any && alive;
+    }
     PGC_LDS int* cnt = odd ? &sh->orCount[1] : &sh->orCount[0];
     const unsigned long long b = __ballot(pred_);
     if (b && lane == 0) __atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED);
@@ -870,11 +898,11 @@ struct SweepG {
           if (check) {
             if (!bor(imp)) return 0;
           } else {
-            __syncthreads();
+            sync();
           }
           continue;
         }
-        __syncthreads();
+        sync();
       }
       for (int k = 1; k < K; ++k) {
         // the closing layer L has its own (empty) intralayer slots
@@ -882,7 +910,7 @@ struct SweepG {
         if (k == K - 1 && check) {
           if (!bor(imp)) return 0;
         } else {
-          __syncthreads();
+          sync();
         }
       }
     }
@@ -906,18 +934,18 @@ struct SweepG {
           if (check) {
             if (!bor(imp)) return 0;
           } else {
-            __syncthreads();
+            sync();
           }
           continue;
         }
-        __syncthreads();
+        sync();
       }
       for (int k = 1; k < K; ++k) {
         imp |= intra_step(layer, cur, k, ratio, rt, ub);
         if (k == K - 1 && check) {
           if (!bor(imp)) return 0;
         } else {
-          __syncthreads();
+          sync();
         }
       }
     }
@@ -925,11 +953,13 @@ struct SweepG {
   }
 };
 
-template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W>
+template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W,
+          bool GRID = false>
 __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                        int32_t* predLg, int predLds, int p0, int p1, int odd,
-                                                       double ratio, double ub) {
-  typedef SweepG<CT, HAS_TIME, MINCOST, W, LD, MC, WX, WI> SW;
+                                                       double ratio, double ub, GridState* gs = nullptr,
+                                                       uint64_t* bgen = nullptr) {
+  typedef SweepG<CT, HAS_TIME, MINCOST, W, LD, MC, WX, WI, GRID> SW;
   SW w;
   w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
   w.D = (typename SW::DP)uni(Dg);
@@ -945,9 +975,17 @@ __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, doub
   w.K = uni(w.J->K);
   w.L = uni(w.J->L);
   w.nC = uni(w.J->nChunks);
-  w.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   w.lane = threadIdx.x & 63;
-  w.nw = PGC_BLOCK / PGC_WAVE;
+  if constexpr (GRID) {
+    w.wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (PGC_BLOCK / PGC_WAVE) + (threadIdx.x >> 6)));
+    w.nw = (int)gridDim.x * (PGC_BLOCK / PGC_WAVE);
+    w.gs = uni(gs);
+    w.bgen = uni((int64_t)*bgen);
+    w.nblk = gridDim.x;
+  } else {
+    w.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    w.nw = PGC_BLOCK / PGC_WAVE;
+  }
   w.p0 = uni(p0);
   w.p1 = uni(p1);
   w.odd = uni(odd);
@@ -956,6 +994,7 @@ __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, doub
   r.p0 = w.p0;
   r.p1 = w.p1;
   r.odd = w.odd;
+  if constexpr (GRID) *bgen = w.bgen;
   return r;
 }
 
