@@ -288,7 +288,7 @@ def conjugate(steps=2):
     return {"workload": "conjugate, icosphere s=4 x 100-point curve, max_depth 2, regularised, vanilla/nomode",
             "n_edges": info["n_edges"], "nodes_per_layer": info["nodes_per_layer"],
             "generate_s": sum(x[1] for x in runs) / steps, "lawler_solve_s": sum(x[2] for x in runs) / steps,
-            "sweeps": r["sweeps"], "solver_kernel_ms": r["kernel_ms"],
+            "sweeps": r["sweeps"], "solver_kernel_ms": r["kernel_ms"], "solver_phase_ms": r["phase_ms"],
             "gteps": r["relaxations"] / (r["kernel_ms"] * 1e-3) / 1e9,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "bytes_per_sweep": bps,

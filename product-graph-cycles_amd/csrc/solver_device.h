@@ -63,6 +63,7 @@ __device__ __forceinline__ int round_to_int(double w) {
 struct Shared {
   int orCount[2];  // barrier_or counters, alternating by parity
   int gridFlag;    // device-wide solve: this workgroup has a true predicate (GridOr)
+  int gridOk;      // device-wide solve: no barrier has timed out (set by thread 0)
   int i0, i1, i2, i3;
   double d0, d1, d2;
   int status;
@@ -125,31 +126,41 @@ struct GridState {
 
 // Device-wide barrier: thread 0 of every workgroup adds one arrival to a
 // monotonic counter and waits until all workgroups have arrived at this
-// barrier (its ordinal times the grid size). Device-scope fences release the
-// writes before it and acquire the other workgroups' writes after it (the
-// per-XCD L2 caches are not coherent with each other otherwise). A wait of
-// more than two seconds (the wall clock runs at 100 MHz) sets the abort flag
-// instead of hanging; every later barrier then returns at once.
-__device__ __forceinline__ bool grid_sync(GridState* gs, uint64_t& bgen, uint32_t nblk) {
+// barrier (its ordinal times the grid size). FULL: a device-scope release
+// before the arrival (it writes back the XCD's L2, whose lines are not
+// coherent with the other XCDs' otherwise) and an acquire after the wait (it
+// invalidates the L2). Without FULL only the acquire: the barrier then orders
+// the data stored with device-scope (write-through) stores before it, which
+// every wave has completed before its workgroup arrives (the distance rows
+// between the steps of a sweep). A wait of more than two seconds (the wall
+// clock runs at 100 MHz) sets the abort flag instead of hanging; every later
+// barrier then returns at once. Returns false once the solve is aborted
+// (thread 0 reads the flag, the workgroup gets it through okFlag in LDS).
+template <bool FULL = true>
+__device__ __forceinline__ bool grid_sync(GridState* gs, uint64_t& bgen, uint32_t nblk, int* okFlag) {
+  if (!FULL) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   bgen += 1;
   if (threadIdx.x == 0) {
     const unsigned long long target = (unsigned long long)bgen * nblk;
-    __threadfence();
+    if (FULL) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(&gs->arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t t0 = wall_clock64();
-    while (__hip_atomic_load(&gs->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    bool ok = __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    while (ok && __hip_atomic_load(&gs->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (wall_clock64() - t0 > 200000000ull) {
         __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = false;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
+      ok = __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     }
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *(volatile int*)okFlag = ok ? 1 : 0;
   }
   __syncthreads();
-  return __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  return *(volatile int*)okFlag != 0;
 }
 
 // Device-wide OR of a predicate: a workgroup with a true predicate in any wave
@@ -159,14 +170,16 @@ __device__ __forceinline__ bool grid_sync(GridState* gs, uint64_t& bgen, uint32_
 struct GridOr {
   unsigned int prev0 = 0, prev1 = 0;
   bool odd = false;
-  __device__ __forceinline__ bool operator()(GridState* gs, uint64_t& bgen, uint32_t nblk, int* ldsFlag, bool pred) {
+  // ldsFlag, okFlag: two ints of the workgroup's LDS; ok: false once the solve is aborted
+  __device__ __forceinline__ bool operator()(GridState* gs, uint64_t& bgen, uint32_t nblk, int* ldsFlag, int* okFlag,
+                                             bool pred, bool& ok) {
     if (__ballot(pred) && (threadIdx.x & 63) == 0) *(volatile int*)ldsFlag = 1;
     __syncthreads();
     if (threadIdx.x == 0 && *(volatile int*)ldsFlag) {
       __hip_atomic_fetch_add(&gs->orc[odd ? 1 : 0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       *(volatile int*)ldsFlag = 0;
     }
-    grid_sync(gs, bgen, nblk);
+    ok = grid_sync(gs, bgen, nblk, okFlag) && ok;
     const unsigned int c = __hip_atomic_load(&gs->orc[odd ? 1 : 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     bool any;
     if (odd) {
@@ -341,10 +354,14 @@ struct Solver {
   // workgroup barrier, or the device-wide one
   __device__ __forceinline__ void sync() {
     if constexpr (GRID)
-      alive = grid_sync(gs, bgen, nblk) && alive;
+      alive = grid_sync(gs, bgen, nblk, (int*)&lsh->gridOk) && alive;
     else
       __syncthreads();
   }
+  // distance rows outside the sweep (the solver's barriers write them back
+  // and invalidate the caches)
+  __device__ __forceinline__ double dld(const double* p) const { return *p; }
+  __device__ __forceinline__ void dst(double* p, double v) const { *p = v; }
   // the thread that runs the sequential parts
   __device__ __forceinline__ bool leader() const {
     if constexpr (GRID) return blockIdx.x == 0 && threadIdx.x == 0;
@@ -361,11 +378,7 @@ struct Solver {
   }
   // OR of a predicate over all threads of the solve
   __device__ __forceinline__ bool any(bool pred) {
-    if constexpr (GRID) {
-      const bool a = gor(gs, bgen, nblk, (int*)&lsh->gridFlag, pred);
-      alive = alive && __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
-      return a;
-    }
+    if constexpr (GRID) return gor(gs, bgen, nblk, (int*)&lsh->gridFlag, (int*)&lsh->gridOk, pred, alive);
     return bor(sh, pred);
   }
   // reductions over all threads of the solve: the workgroup's partial, then
@@ -503,7 +516,7 @@ struct Solver {
       gor.prev0 = (unsigned)r.p0;
       gor.prev1 = (unsigned)r.p1;
       gor.odd = r.odd != 0;
-      alive = alive && __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+      alive = alive && lsh->gridOk != 0;
       return r.complete;
     } else if constexpr (DYN) {
       // short slots for the interlayer / intralayer classes: 7 / 6 of 8 (icosphere
@@ -551,8 +564,8 @@ struct Solver {
 
   // D <- inf, branch sources <- 0, pred of sources <- -1; returns #sources.
   __device__ __forceinline__ int setup(int bid, int onlyV) {
-    for (int i = tfirst(); i < 2 * K * ds; i += tstride()) D[i] = PGC_INF;
-    for (int i = tfirst(); i < ds; i += tstride()) Dmin[i] = PGC_INF;
+    for (int i = tfirst(); i < 2 * K * ds; i += tstride()) dst(D + i, PGC_INF);
+    for (int i = tfirst(); i < ds; i += tstride()) dst(Dmin + i, PGC_INF);
     sync();
     int cnt = 0, last = -1;
     inb = 0;
@@ -563,7 +576,7 @@ struct Solver {
       const bool in = onlyV >= 0 ? (v == onlyV) : (gp(J.branch)[v] == bid);
       if (!DYN && in) inb |= 1u << i;
       if (in) {
-        D[v] = 0;
+        dst(D + v, 0.0);
         pset(v, -1);
         cnt++;
         last = max(last, v);
@@ -599,7 +612,7 @@ struct Solver {
       for (int i = 0; i < np(); ++i) {
         const int v = vert(i);
         if (v < 0) continue;
-        for (int k = 1; k < K; ++k) D[k * ds + v] = D[(K + k) * ds + v];
+        for (int k = 1; k < K; ++k) dst(D + k * ds + v, dld(D + (K + k) * ds + v));
       }
     }
     bool imp = false;
@@ -607,14 +620,14 @@ struct Solver {
     for (int i = 0; i < np(); ++i) {
       const int v = vert(i);
       if (v < 0 || !(DYN ? in_branch(v, bid, onlyV) : ((inb >> i) & 1u) != 0)) continue;
-      const double dl = D[lastRow * ds + v];
+      const double dl = dld(D + lastRow * ds + v);
       const double bk = gp(J.backup)[v];
       if (dl < bk) {
         imp = true;
-        D[v] = dl;
+        dst(D + v, dl);
         pset(v, pget((int64_t)npl * K * L + v));
       } else {
-        D[v] = bk;
+        dst(D + v, bk);
       }
     }
     return any(imp);
@@ -777,7 +790,7 @@ struct Solver {
     for (int i = 0; i < np(); ++i) {
       const int v = vert(i);
       if (v < 0 || !in_branch(v, bid, onlyV)) continue;
-      const double lb = D[v];
+      const double lb = dld(D + v);
       if (!zeroSearch && lb >= 0) continue;
       if (zeroSearch && (lb <= -J.tol || lb >= J.tol)) continue;
       int vs;
@@ -863,7 +876,7 @@ struct Solver {
 #pragma unroll
       for (int i = 0; i < np(); ++i) {
         const int v = vert(i);
-        if (v >= 0) gp(J.backup)[v] = D[v];
+        if (v >= 0) gp(J.backup)[v] = dld(D + v);
       }
       sweeps++;
       uint64_t t0 = wclock();
@@ -1412,7 +1425,7 @@ struct Solver {
           gp(J.wlen)[v] = 0;
           continue;
         }
-        double key = D[lastRow * ds + v];
+        double key = dld(D + lastRow * ds + v);
         if (!cpuRules) key = (double)(float)key;
         gp(J.lb_key)[v] = key;
         int back = -1, cnt = 0;
@@ -1529,6 +1542,7 @@ __global__ void __launch_bounds__(PGC_BLOCK) solve_kernel_grid(const SolverJob* 
     lsh.orCount[0] = 0;
     lsh.orCount[1] = 0;
     lsh.gridFlag = 0;
+    lsh.gridOk = 1;
   }
   __syncthreads();
   const PGC_CONST SolverJob& J = ((const PGC_CONST SolverJob*)jobs)[0];

@@ -569,11 +569,24 @@ struct SweepG {
   uint32_t nblk;
   bool alive = true;  // GRID: no barrier has timed out
 
+  // the barrier between two steps (GRID: device-wide, without the L2
+  // write-back: the rows that cross workgroups are stored through dput)
   __device__ __forceinline__ void sync() {
     if constexpr (GRID)
-      alive = grid_sync(gs, bgen, nblk) && alive;
+      alive = grid_sync<false>(gs, bgen, nblk, (int*)&sh->gridOk) && alive;
     else
       __syncthreads();
+  }
+
+  // distance rows and per-node minima. GRID: stored with device-scope
+  // (write-through) stores, read with ordinary loads after the step
+  // barrier's acquire
+  __device__ __forceinline__ double dget(const DP p) const { return *p; }
+  __device__ __forceinline__ void dput(DP p, double v) const {
+    if constexpr (GRID)
+      __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      *p = v;
   }
 
   __device__ __forceinline__ bool bor(bool pred_) {
@@ -582,11 +595,10 @@ struct SweepG {
       g.prev0 = (unsigned)p0;
       g.prev1 = (unsigned)p1;
       g.odd = odd != 0;
-      const bool any = g(gs, bgen, nblk, (int*)&sh->gridFlag, pred_);
+      const bool any = g(gs, bgen, nblk, (int*)&sh->gridFlag, (int*)&sh->gridOk, pred_, alive);
       p0 = (int)g.prev0;
       p1 = (int)g.prev1;
       odd = g.odd ? 1 : 0;
-      alive = alive && __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
       return any && alive;
     }
     PGC_LDS int* cnt = odd ? &sh->orCount[1] : &sh->orCount[0];
@@ -650,7 +662,7 @@ struct SweepG {
       if (K > 1) {
         const double bw = weight(bc, bt, ratio, rt);
         for (int kk = 0; kk < K; ++kk) {
-          if (D[(prev + kk) * ds + bs] + bw == best) {
+          if (dget(D + (prev + kk) * ds + bs) + bw == best) {
             ks = kk;
             break;
           }
@@ -665,7 +677,7 @@ struct SweepG {
       res = best;
       imp = true;
     }
-    D[cur * ds + v] = res;
+    dput(D + cur * ds + v, res);
     return imp;
   }
 
@@ -683,7 +695,7 @@ struct SweepG {
       best = pgc_min(best, nv);
       bs = better ? s[j] : bs;
     }
-    const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
+    const double curv = (layer == 0) ? dget(D + (cur + k) * ds + v) : PGC_INF;
     double res = curv;
     bool imp = false;
     if (best < curv && best < ub) {
@@ -696,14 +708,14 @@ struct SweepG {
       res = best;
       imp = true;
     }
-    D[(cur + k) * ds + v] = res;
+    dput(D + (cur + k) * ds + v, res);
     if (k == K - 1) {
-      double m = D[cur * ds + v];
+      double m = dget(D + cur * ds + v);
       for (int kk = 1; kk < K; ++kk) {
-        const double x = D[(cur + kk) * ds + v];
+        const double x = dget(D + (cur + kk) * ds + v);
         m = x < m ? x : m;
       }
-      Dmin[v] = m;
+      dput(Dmin + v, m);
     }
     return imp;
   }
@@ -734,7 +746,7 @@ struct SweepG {
 #pragma unroll
       for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int j = 0; j < N; ++j) dv[g][j] = Dg[s[g][j]];
+        for (int j = 0; j < N; ++j) dv[g][j] = dget(Dg + s[g][j]);
 #pragma unroll
       for (int g = 0; g < NG; ++g) {
         const int c = c0 + g * nw;
@@ -782,7 +794,7 @@ struct SweepG {
       const int(&s)[WX] = sX[i];
       double dv[WX];
 #pragma unroll
-      for (int j = 0; j < WX; ++j) dv[j] = Dp[s[j]];
+      for (int j = 0; j < WX; ++j) dv[j] = dget(Dp + s[j]);
       double best = PGC_INF;
       int bs = npl;
       CT bc = 0;
@@ -802,7 +814,7 @@ struct SweepG {
         if (K > 1) {
           const double bw = weight(bc, bt, ratio, rt);
           for (int kk = 0; kk < K; ++kk) {
-            if (D[(prev + kk) * ds + bs] + bw == best) {
+            if (dget(D + (prev + kk) * ds + bs) + bw == best) {
               ks = kk;
               break;
             }
@@ -817,7 +829,7 @@ struct SweepG {
         res = best;
         imp = true;
       }
-      D[cur * ds + v] = res;
+      dput(D + cur * ds + v, res);
     }
     return imp;
   }
@@ -837,7 +849,7 @@ struct SweepG {
       const int(&s)[WI] = sI[i];
       double dv[WI];
 #pragma unroll
-      for (int j = 0; j < WI; ++j) dv[j] = Dk[s[j]];
+      for (int j = 0; j < WI; ++j) dv[j] = dget(Dk + s[j]);
       double best = PGC_INF;
       int bs = npl;
 #pragma unroll
@@ -847,7 +859,7 @@ struct SweepG {
         best = pgc_min(best, nv);
         bs = better ? s[j] : bs;
       }
-      const double curv = (layer == 0) ? D[(cur + k) * ds + v] : PGC_INF;
+      const double curv = (layer == 0) ? dget(D + (cur + k) * ds + v) : PGC_INF;
       double res = curv;
       if (best < curv && best < ub) {
         const int64_t pi = (int64_t)layer * npl * K + (int64_t)k * npl + v;
@@ -859,14 +871,14 @@ struct SweepG {
         res = best;
         imp = true;
       }
-      D[(cur + k) * ds + v] = res;
+      dput(D + (cur + k) * ds + v, res);
       if (k == K - 1) {
-        double m = D[cur * ds + v];
+        double m = dget(D + cur * ds + v);
         for (int kk = 1; kk < K; ++kk) {
-          const double x = D[(cur + kk) * ds + v];
+          const double x = dget(D + (cur + kk) * ds + v);
           m = x < m ? x : m;
         }
-        Dmin[v] = m;
+        dput(Dmin + v, m);
       }
     }
     return imp;
