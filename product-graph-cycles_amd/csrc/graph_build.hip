@@ -674,6 +674,21 @@ __global__ void k_sort_slots(const int32_t* __restrict__ cnt, uint32_t npl, int6
   }
 }
 
+// groups of eight slots per chunk (per class): the largest in-degree of the
+// chunk's nodes over all layers, rounded up to eight, divided by eight
+__global__ void k_chunk_groups(const int32_t* __restrict__ cnt, uint32_t npl, int64_t nTot, FastDiv fd,
+                               int nChunks, int32_t* __restrict__ grp) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= 2 * nTot) return;
+  const int n = cnt[g];
+  if (n <= 0) return;
+  const int cls = g >= nTot ? 1 : 0;
+  const uint32_t node = (uint32_t)(g - cls * nTot);
+  const uint32_t layer = fd.div(node);
+  const uint32_t v = node - layer * npl;
+  atomicMax(&grp[cls * nChunks + (int)(v >> 6)], (n + 7) >> 3);
+}
+
 // padding slots (src < 0) point at the sentinel vertex npl of the solver's
 // distance rows (always +inf), so the solver needs no per-slot validity test
 __global__ void k_pad_src(int32_t* __restrict__ s, int64_t n, int32_t npl) {
@@ -863,6 +878,15 @@ hipError_t launch_layout_fill(const int32_t* src, const int32_t* trgt, int64_t E
   const int64_t nTot = (numNodes / npl + 1) * npl;
   hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nTot, 256)), dim3(256), 0, st, cnt, (uint32_t)npl, nTot, fd, nChunks,
                      width, s0, s1, e0, e1);
+  return hipGetLastError();
+}
+
+hipError_t launch_chunk_groups(const int32_t* cnt, int64_t npl, int64_t nTot, int nChunks, int32_t* grp,
+                               hipStream_t st) {
+  FastDiv fd;
+  fd.init((uint32_t)npl);
+  hipLaunchKernelGGL(k_chunk_groups, dim3(nb(2 * nTot, 256)), dim3(256), 0, st, cnt, (uint32_t)npl, nTot, fd, nChunks,
+                     grp);
   return hipGetLastError();
 }
 

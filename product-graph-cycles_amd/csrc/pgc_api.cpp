@@ -54,6 +54,7 @@ hipError_t launch_gather_strided(const int32_t*, int64_t, int64_t, int32_t*, hip
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
 hipError_t launch_fill_i32(int32_t*, int64_t, int32_t, hipStream_t);
+hipError_t launch_chunk_groups(const int32_t*, int64_t, int64_t, int, int32_t*, hipStream_t);
 // scratch.hip
 hipError_t scratch_setup(int device);
 void* scratch_ptr(int device);
@@ -308,6 +309,7 @@ struct pgc_graph {
   int64_t nLayers = 0;
   int64_t nslots[2] = {0, 0};
   int32_t* d_width = nullptr;  // per class: slots per lane, then edge count (4 values)
+  int32_t* d_grp = nullptr;    // slot widths above 32: per class and chunk, groups of 8 slots ([2][nChunks])
   int32_t *d_s[2] = {nullptr, nullptr}, *d_e[2] = {nullptr, nullptr};
   int32_t width[2] = {0, 0};
   int32_t lstride[2] = {0, 0};
@@ -347,7 +349,7 @@ struct pgc_graph {
     js.release();
     if (base) {
       d_rows = d_src = d_trgt = nullptr;
-      d_width = d_cnt = nullptr;
+      d_width = d_cnt = d_grp = nullptr;
       d_s[0] = d_s[1] = d_e[0] = d_e[1] = nullptr;
       d_vx = d_vy = nullptr;
       d_ex = d_ey = nullptr;
@@ -356,7 +358,7 @@ struct pgc_graph {
     if (d_vy) cache_free(d_vy);
     if (d_ex) cache_free(d_ex);
     if (d_ey) cache_free(d_ey);
-    void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_width, d_s[0], d_s[1], d_e[0], d_e[1],
+    void* ps[] = {d_rows, d_src, d_trgt, d_energy, d_width, d_grp, d_s[0], d_s[1], d_e[0], d_e[1],
                   d_cnt, d_cost[0], d_cost[1], d_time[0], d_time[1]};
     for (void* p : ps)
       if (p) cache_free(p);
@@ -465,9 +467,11 @@ int build_layout(pgc_graph* g, DevAlloc& A) {
   HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   // both classes get the solver's slot width: the smallest of 8, 16, 32 that
-  // holds the largest in-degree (wider graphs are rejected by check_width)
+  // holds the largest in-degree, above 32 the largest in-degree rounded up to
+  // a multiple of 8 (the runtime-width variant; beyond kMaxSlotWidth the
+  // graph is rejected by check_width)
   const int32_t mx = std::max(wn[0], wn[1]);
-  const int32_t W = mx <= 8 ? 8 : (mx <= 16 ? 16 : (mx <= 32 ? 32 : mx));
+  const int32_t W = mx <= 8 ? 8 : (mx <= 16 ? 16 : (mx <= 32 ? 32 : (mx + 7) / 8 * 8));
   g->width[0] = W;
   g->width[1] = W;
   g->max_in_degree = mx;
@@ -496,6 +500,11 @@ int build_layout(pgc_graph* g, DevAlloc& A) {
   HIPCHK(launch_layout_fill(g->d_src, g->d_trgt, g->E, npl, g->numNodes, nC, d_fill, g->d_cnt, g->d_width,
                             g->d_s[0], g->d_s[1], g->d_e[0], g->d_e[1], st));
   for (int c = 0; c < 2; ++c) HIPCHK(launch_pad_src(g->d_s[c], g->nslots[c] + kSlotPad, (int32_t)npl, st));
+  if (W > 32) {
+    HIPCHK(A.get(&g->d_grp, 2 * (int64_t)nC));
+    HIPCHK(launch_fill_i32(g->d_grp, 2 * (int64_t)nC, 0, st));
+    HIPCHK(launch_chunk_groups(g->d_cnt, npl, nTot, nC, g->d_grp, st));
+  }
   // layer-invariant sources: every layer repeats the X adjacency (the
   // interlayer class from layer 1 on, as layer 0 has no interlayer in-edges;
   // the intralayer class up to layer L-1, as the closing layer L has none)
@@ -590,6 +599,7 @@ int kernel_maxc(const pgc_graph* g, int K) {
   if (force_global) return 0;
   const int per = (int)((g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE));
   if (per > 8 || solver_lds_bytes((int)g->npl, K) > 160 * 1024 - 1024) return 0;
+  if (g->maxw > 32) return 8;  // the runtime-width variant has the one LDS instance
   return per <= 4 ? per : 8;
 }
 
@@ -728,6 +738,7 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
     S.lstride = g->lstride[c];
     S.src_inv = g->src_inv[c];
     S.deg = g->cls_deg[c];
+    S.grp = g->d_grp ? g->d_grp + (int64_t)c * g->nChunks : nullptr;
     S.src = g->d_s[c];
     S.cost = g->d_cost[c];
     S.time = g->d_time[c];
@@ -840,7 +851,9 @@ int check_solvable(pgc_graph* g, int K) {
 }
 
 int check_width(pgc_graph* g) {
-  if (g->maxw > 32) return fail(PGC_ERR_UNSUPPORTED, "a node has more than 32 in-edges of one class");
+  if (g->maxw > kMaxSlotWidth)
+    return fail(PGC_ERR_UNSUPPORTED, "a node has more than " + std::to_string(kMaxSlotWidth) +
+                                         " in-edges of one class");
   return PGC_OK;
 }
 
@@ -1302,6 +1315,7 @@ int pgc_graph_derive(pgc_graph* base, const double* feat_diff, int on_device, pg
     g->cls_deg[c] = b->cls_deg[c];
   }
   g->d_width = b->d_width;
+  g->d_grp = b->d_grp;
   g->maxw = b->maxw;
   g->d_cnt = b->d_cnt;
   g->dropped = b->dropped;
@@ -1521,7 +1535,7 @@ struct ScratchView {
   bool layout;
   int nChunks;
   int64_t nLayers, nslots[2], dropped, isv, xsv;
-  int32_t* d_width;
+  int32_t *d_width, *d_grp;
   int32_t *d_s[2], *d_e[2], *d_cnt;
   int32_t width[2], lstride[2], src_inv[2], cls_deg[2], maxw, mid;
   void* d_cost[2];
@@ -1537,6 +1551,7 @@ struct ScratchView {
     isv = g->intra_slots_valid;
     xsv = g->inter_slots_valid;
     d_width = g->d_width;
+    d_grp = g->d_grp;
     d_cnt = g->d_cnt;
     maxw = g->maxw;
     mid = g->max_in_degree;
@@ -1559,6 +1574,7 @@ struct ScratchView {
       g->cost_bytes[c] = g->time_bytes[c] = 0;
     }
     g->layout = false;
+    g->d_grp = nullptr;
     g->js = JobScratch();
   }
   ~ScratchView() {
@@ -1569,6 +1585,7 @@ struct ScratchView {
     g->intra_slots_valid = isv;
     g->inter_slots_valid = xsv;
     g->d_width = d_width;
+    g->d_grp = d_grp;
     g->d_cnt = d_cnt;
     g->maxw = maxw;
     g->max_in_degree = mid;
@@ -1854,7 +1871,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   for (int64_t i = 0; i < n; ++i) {
     pgc_graph* g = graphs[i];
     const int mc = kernel_maxc(g, max_depth[i]);
-    const int mw = g->maxw <= 8 ? 8 : (g->maxw <= 16 ? 16 : 32);
+    const int mw = g->maxw <= 8 ? 8 : (g->maxw <= 16 ? 16 : (g->maxw <= 32 ? 32 : kMaxSlotWidth));
     groups[{mc, mw, g->d_time[0] != nullptr}].push_back(i);
   }
   if (groups.size() > 64) return fail(PGC_ERR_UNSUPPORTED, "too many launch groups");

@@ -24,6 +24,10 @@
 
 namespace pgc {
 
+// Largest slot width of the in-edge layout: nodes with up to this many
+// in-edges of one class (the runtime-width solver variant above 32).
+constexpr int kMaxSlotWidth = 1024;
+
 // Bytes of the static device scratch of the allocation-free solve (per device).
 constexpr size_t kScratchBytes = (size_t)128 << 20;
 
@@ -39,6 +43,8 @@ struct SlotClass {
                          // class on layers 1..L (the same X adjacency on every layer)
   int32_t deg;           // the largest in-degree of the class: slots deg..w-1 are padding on
                          // every node
+  const int32_t* grp;    // slot widths above 32 (runtime width): per chunk, the groups of eight
+                         // slots that hold its largest in-degree; nullptr otherwise
 };
 
 // Conjugate energy (cost, time) kinds of the device kernel.

@@ -39,13 +39,16 @@ int solver_grid_capacity(int maxw, bool mincost, bool hasTime, int numCU) {
 
 // Solver launch: grid workgroups over njobs jobs (one job per workgroup when
 // queue is null, else a persistent launch on the queue counter). maxw = the
-// layout's slot width (<= 32), maxChunksPerWave = ceil(ceil(npl/64) / 16) (<= 8)
-// for the largest job, or 0 for the variant with the distance table in global
-// memory (any number of chunks per wave; lds is then 0).
+// layout's slot width (8, 16, 32, or a multiple of 8 up to kMaxSlotWidth for
+// the runtime-width variant), maxChunksPerWave = ceil(ceil(npl/64) / 16) (<= 8)
+// for the largest job (8 for any count with the runtime width), or 0 for the
+// variant with the distance table in global memory (any number of chunks per
+// wave; lds is then 0).
 hipError_t launch_solver(const SolverJob* d_jobs, int njobs, int grid, int* queue, int maxChunksPerWave, int maxw,
                          bool mincost, bool hasTime, bool doubleCost, size_t lds, hipStream_t st) {
   if (lds > 160 * 1024 - 1024) return hipErrorInvalidValue;
-  if (maxChunksPerWave < 0 || maxChunksPerWave > 8 || maxw > 32) return hipErrorInvalidValue;
+  if (maxChunksPerWave < 0 || maxChunksPerWave > 8 || maxw > kMaxSlotWidth) return hipErrorInvalidValue;
+  if (maxw > 32 && maxChunksPerWave != 0 && maxChunksPerWave != 8) return hipErrorInvalidValue;
   if (mincost) {
     if (!doubleCost) return hipErrorInvalidValue;
     return launch_solver_mincost(maxw, maxChunksPerWave, d_jobs, njobs, grid, queue, lds, st);

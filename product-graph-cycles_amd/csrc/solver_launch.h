@@ -38,7 +38,11 @@ static hipError_t launch_width(int maxw, int maxc, const SolverJob* jobs, int nj
                                size_t lds, hipStream_t st) {
   if (maxw <= 8) return launch_maxc<CT, HT, MC, 8>(maxc, jobs, njobs, grid, queue, lds, st);
   if (maxw <= 16) return launch_maxc<CT, HT, MC, 16>(maxc, jobs, njobs, grid, queue, lds, st);
-  return launch_maxc<CT, HT, MC, 32>(maxc, jobs, njobs, grid, queue, lds, st);
+  if (maxw <= 32) return launch_maxc<CT, HT, MC, 32>(maxc, jobs, njobs, grid, queue, lds, st);
+  // runtime slot width (in-degrees above 32): the table in global memory, or
+  // in LDS with up to eight chunks per wave
+  if (maxc == 0) return launch_one<CT, HT, MC, 0, 0>(jobs, njobs, grid, queue, lds, st);
+  return launch_one<CT, HT, MC, 8, 0>(jobs, njobs, grid, queue, lds, st);
 }
 
 // The device-wide solve of one job: a cooperative launch of nblk workgroups

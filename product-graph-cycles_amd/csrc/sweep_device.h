@@ -46,6 +46,8 @@ __device__ __forceinline__ SlotClass uni_class(const PGC_GLOBAL SlotClass& c) {
   r.w = uni(c.w);
   r.lstride = uni(c.lstride);
   r.src_inv = uni(c.src_inv);
+  r.deg = uni(c.deg);
+  r.grp = uni(c.grp);
   return r;
 }
 
@@ -655,6 +657,13 @@ struct SweepG {
       bc = better ? cc[j] : bc;
       if (HAS_TIME) bt = better ? t[j] : bt;
     }
+    return inter_finish(layer, cur, prev, v, best, bs, bc, bt, ratio, rt, ub);
+  }
+
+  // The write-back of an interlayer vertex after its scan: depth of the
+  // winning source, predecessor, distance.
+  __device__ __forceinline__ bool inter_finish(int layer, int cur, int prev, int v, double best, int bs, CT bc,
+                                               float bt, double ratio, double rt, double ub) {
     double res = PGC_INF;
     bool imp = false;
     if (best < PGC_INF && best < ub) {
@@ -695,6 +704,10 @@ struct SweepG {
       best = pgc_min(best, nv);
       bs = better ? s[j] : bs;
     }
+    return intra_finish(layer, cur, k, v, best, bs, ub);
+  }
+
+  __device__ __forceinline__ bool intra_finish(int layer, int cur, int k, int v, double best, int bs, double ub) {
     const double curv = (layer == 0) ? dget(D + (cur + k) * ds + v) : PGC_INF;
     double res = curv;
     bool imp = false;
@@ -760,11 +773,56 @@ struct SweepG {
     }
     return imp;
   }
+  // Runtime slot width (W == 0: a node of the graph has more than 32 in-edges
+  // of one class). A chunk's slots are scanned in groups of eight, as many as
+  // its largest in-degree needs (SlotClass::grp), the running first minimum
+  // carried across the groups in slot order.
+  __device__ __forceinline__ bool step_rt(int layer, int cur, int prev, int k, double ratio, double rt, double ub) {
+    bool imp = false;
+    const bool inter = k == 0;
+    const SlotClass& S = inter ? Sx : Sa;
+    const DP Dg = inter ? ((K == 1) ? D + prev * ds : Dmin) : D + (cur + k - 1) * ds;
+    const int w = S.w;
+    for (int c = wave; c < nC; c += nw) {
+      const int ng = uni(gp(S.grp)[c]);
+      const int base = layer * S.lstride + c * w * 64 + lane;
+      double best = PGC_INF;
+      int bs = npl;
+      CT bc = 0;
+      float bt = 0;
+      for (int g = 0; g < ng; ++g) {
+        int sg[8];
+        CT cg[8];
+        float tg[8];
+        load<8>(S, base + g * 8 * 64, sg, cg, tg);
+        double dv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dv[j] = dget(Dg + sg[j]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const double nv = dv[j] + weight(cg[j], tg[j], ratio, rt);
+          const bool better = nv < best;
+          best = pgc_min(best, nv);
+          bs = better ? sg[j] : bs;
+          bc = better ? cg[j] : bc;
+          if (HAS_TIME) bt = better ? tg[j] : bt;
+        }
+      }
+      const int v = c * 64 + lane;
+      if (v >= npl) continue;
+      imp |= inter ? inter_finish(layer, cur, prev, v, best, bs, bc, bt, ratio, rt, ub)
+                   : intra_finish(layer, cur, k, v, best, bs, ub);
+    }
+    return imp;
+  }
+
   __device__ __forceinline__ bool inter_step(int layer, int cur, int prev, double ratio, double rt, double ub) {
-    return step_n<WX>(layer, cur, prev, 0, ratio, rt, ub);
+    if constexpr (W == 0) return step_rt(layer, cur, prev, 0, ratio, rt, ub);
+    else return step_n<WX>(layer, cur, prev, 0, ratio, rt, ub);
   }
   __device__ __forceinline__ bool intra_step(int layer, int cur, int k, double ratio, double rt, double ub) {
-    return step_n<WI>(layer, cur, K - cur, k, ratio, rt, ub);
+    if constexpr (W == 0) return step_rt(layer, cur, K - cur, k, ratio, rt, ub);
+    else return step_n<WI>(layer, cur, K - cur, k, ratio, rt, ub);
   }
 
   // --- steps with the sources held in registers (MC > 0)

@@ -76,11 +76,26 @@ def curve_normals_2d(VY: np.ndarray) -> np.ndarray:
     return n
 
 
+def bipyramid(n_ring: int):
+    """Unit bipyramid over an n_ring-gon: two apexes (valence n_ring) and an
+    equator of valence-4 vertices; vertices (float64 [n+2,3]), triangles."""
+    ang = 2.0 * np.pi * np.arange(n_ring) / n_ring
+    ring = np.stack([np.cos(ang), np.sin(ang), np.zeros(n_ring)], axis=1)
+    V = np.concatenate([[[0.0, 0.0, 1.0]], ring, [[0.0, 0.0, -1.0]]], axis=0)
+    top, bot = 0, n_ring + 1
+    F = []
+    for i in range(n_ring):
+        a, b = 1 + i, 1 + (i + 1) % n_ring
+        F += [(top, a, b), (bot, b, a)]
+    return V, np.array(F, dtype=np.int32)
+
+
 def make_pair(subdiv: int = 3, n_y: int = 64, seed: int = 0, dim: int = 32,
               sigma_radial: float = 0.05, sigma_curve: float = 0.01, sigma_desc: float = 0.1,
-              with_normals: bool = False):
+              with_normals: bool = False, mesh=None):
+    """mesh: (vertices, triangles) of X instead of the icosphere of `subdiv`."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    V0, F = icosphere(subdiv)
+    V0, F = icosphere(subdiv) if mesh is None else mesh
     VX = V0 * (1.0 + sigma_radial * rng.standard_normal(len(V0)))[:, None]
     EX = directed_edges(F)
     ang = 2.0 * np.pi * np.arange(n_y) / n_y
