@@ -111,12 +111,24 @@ def copy_calibration(mib=256, reps=4):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
 
 
+def _tuning(prefix):
+    """The autotuned launch choices of a workload (cyclic_product_graphs/
+    tuning_mi355x.json, written by python -m cyclic_product_graphs.autotune),
+    applied to the default context; None: the built-in rules."""
+    from cyclic_product_graphs import _lib, autotune
+    choices = autotune.choices_for(prefix)
+    _lib.default_context(0).set_tuning(choices)
+    return choices
+
+
 def batched(args):
     """Batched forward+backward over uneven pairs vs the plain per-item loop."""
     import numpy as np
     import torch
     from cyclic_product_graphs import _lib, synthetic as S
     from cyclic_product_graphs import autograd as A
+
+    tuned = _tuning("C4")
 
     pairs = S.make_uneven_pairs(args.batch_pairs, seed=0)
     dev = torch.device("cuda:0")
@@ -196,6 +208,7 @@ def batched(args):
     return {
         "workload": "%d uneven pairs x (1 + %d perturbations) = %d lawlergpu solves, forward+backward" % (
             len(pairs), S_, n),
+        "tuning": tuned,
         "batched_solves_per_s": n * args.batch_steps / b_el,
         "batched_ms_per_step": b_el / args.batch_steps * 1e3,
         "batched_solver_kernel_ms": kms,
@@ -259,6 +272,7 @@ def conjugate(steps=2):
     import numpy as np
     import cyclic_product_graphs as cpg
     from cyclic_product_graphs import synthetic as S
+    tuned = _tuning("C2")
     p = S.make_pair(4, 100, seed=0, with_normals=True)
     NX, NY = p["NX"].astype(np.float64), p["NY"].astype(np.float64)
 
@@ -286,7 +300,7 @@ def conjugate(steps=2):
     tr = _profile_traffic("solve_kernel_c2_traffic.json")
     launches = r["branches"] + 1
     return {"workload": "conjugate, icosphere s=4 x 100-point curve, max_depth 2, regularised, vanilla/nomode",
-            "n_edges": info["n_edges"], "nodes_per_layer": info["nodes_per_layer"],
+            "tuning": tuned, "n_edges": info["n_edges"], "nodes_per_layer": info["nodes_per_layer"],
             "generate_s": sum(x[1] for x in runs) / steps, "lawler_solve_s": sum(x[2] for x in runs) / steps,
             "sweeps": r["sweeps"], "solver_kernel_ms": r["kernel_ms"], "solver_phase_ms": r["phase_ms"],
             "gteps": r["relaxations"] / (r["kernel_ms"] * 1e-3) / 1e9,
@@ -376,6 +390,7 @@ def main():
     FD = torch.tensor(p["feat_diff"].astype(np.float64), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     ctx = _lib.default_context(0)
+    tuned_c1 = _tuning("C1")
 
     def step(mincost=False):
         g = _lib.Graph.generate_device(ctx, VX, EX, VY, EY, FD)
@@ -442,6 +457,7 @@ def main():
     except Exception as ex:  # the checker is optional at run time
         print("parity check skipped: %s" % ex, file=sys.stderr)
 
+    _lib.default_context(0).set_tuning(None)
     batch = None if args.no_batch else batched(args)
     conj = None if args.no_conj else conjugate(args.conj_steps)
 
@@ -467,6 +483,7 @@ def main():
                                "max_depth=3, vanilla/nomode, inputs resident in HBM",
                    "n_edges": E, "nodes_per_layer": npl, "layers": L, "expanded_nodes": Vx},
         "gteps": gteps,
+        "tuning": tuned_c1,
         "sweeps_per_solve": sweeps / args.steps,
         "solver_launches_per_solve": launches / args.steps,
         "solver_kernel_ms_per_solve": kms / args.steps,

@@ -60,12 +60,35 @@ typedef struct pgc_result {
   int32_t reserved;
 } pgc_result;
 
+/* Launch choices of a context's solves. Every field defaults to -1, the
+ * built-in rule; the autotuner (cyclic_product_graphs/autotune.py) times the
+ * alternatives on a workload and keeps the fastest. None changes a result. */
+typedef struct pgc_tuning {
+  int32_t grid_chunks_per_wave; /* graphs whose distance table exceeds the LDS: 0 solves them in one
+                                   workgroup, n >= 1 in one cooperative launch with n 64-node chunks
+                                   per wave; -1: one chunk per wave from 128 chunks per layer, else
+                                   one workgroup */
+  int32_t pred_in_lds;          /* predecessor table in LDS when it fits: -1 yes, 0 no */
+  int32_t wrap_in_lds;          /* wrap map in LDS when it fits: -1 yes, 0 no */
+  int32_t origins;              /* max_depth 1: rows of chain origins in LDS instead of the wrap
+                                   chase, when they fit: -1 yes, 0 no */
+  int32_t cached_sources;       /* layer-invariant source slots held in registers: -1 yes, 0 no */
+  int32_t short_slots;          /* skip slots that are padding on every node: -1 yes, 0 no */
+  int32_t reserved[2];
+} pgc_tuning;
+
 const char* pgc_last_error(void);
 int pgc_abi_version(void);
 
 /* Context: device ordinal + one stream. */
 int pgc_context_create(int device, pgc_context** out);
 void pgc_context_destroy(pgc_context* ctx);
+
+/* The context's launch choices (NULL: the built-in rules) and a copy of them.
+ * No counterpart in the reference (its launch geometry is fixed,
+ * cost_time_ratio_solver_cuda.cpp:219-231). */
+int pgc_context_set_tuning(pgc_context* ctx, const pgc_tuning* tuning);
+int pgc_context_get_tuning(const pgc_context* ctx, pgc_tuning* out);
 
 /* Builds the layered product graph and its per-edge cost/time on the device.
  * Replaces ProductGraphGenerators::generate() / generateConjugate()

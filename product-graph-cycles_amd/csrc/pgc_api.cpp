@@ -221,6 +221,7 @@ struct DevAlloc {
 struct pgc_context {
   int device = 0;
   int num_cu = 0;
+  pgc_tuning tune = {-1, -1, -1, -1, -1, -1, {0, 0}};  // launch choices (-1: the built-in rule)
   hipStream_t stream = nullptr;
   void* select_state = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -368,7 +369,7 @@ struct pgc_graph {
 extern "C" {
 
 const char* pgc_last_error(void) { return g_last_error.c_str(); }
-int pgc_abi_version(void) { return 2; }
+int pgc_abi_version(void) { return 3; }
 
 int pgc_context_create(int device, pgc_context** out) {
   if (!out) return fail(PGC_ERR_ARG, "out is null");
@@ -392,6 +393,26 @@ int pgc_context_create(int device, pgc_context** out) {
   // failure only disables pgc_solve_ratio_noalloc (which reports it)
   (void)scratch_setup(device);
   *out = c;
+  return PGC_OK;
+}
+
+int pgc_context_set_tuning(pgc_context* c, const pgc_tuning* t) {
+  if (!c) return fail(PGC_ERR_ARG, "context is null");
+  if (!t) {
+    c->tune = pgc_tuning{-1, -1, -1, -1, -1, -1, {0, 0}};
+    return PGC_OK;
+  }
+  if (t->grid_chunks_per_wave < -1 || t->grid_chunks_per_wave > 64)
+    return fail(PGC_ERR_ARG, "grid_chunks_per_wave must be -1, 0 or 1..64");
+  for (int32_t v : {t->pred_in_lds, t->wrap_in_lds, t->origins, t->cached_sources, t->short_slots})
+    if (v != -1 && v != 0) return fail(PGC_ERR_ARG, "placement choices must be -1 (rule) or 0 (off)");
+  c->tune = *t;
+  return PGC_OK;
+}
+
+int pgc_context_get_tuning(const pgc_context* c, pgc_tuning* out) {
+  if (!c || !out) return fail(PGC_ERR_ARG, "invalid arguments");
+  *out = c->tune;
   return PGC_OK;
 }
 
@@ -605,14 +626,14 @@ int kernel_maxc(const pgc_graph* g, int K) {
 
 // The predecessor table of small graphs lives in LDS after the distance table.
 bool pred_fits_lds(const pgc_graph* g, int K) {
-  if (kernel_maxc(g, K) == 0) return false;
+  if (kernel_maxc(g, K) == 0 || g->ctx->tune.pred_in_lds == 0) return false;
   const size_t pred = (size_t)(g->L + 1) * K * g->npl * 4;
   return solver_lds_bytes((int)g->npl, K) + pred <= 160 * 1024 - 1024;
 }
 
 // The wrap map (two ints per vertex) follows in LDS when it also fits.
 bool wrap_fits_lds(const pgc_graph* g, int K) {
-  if (kernel_maxc(g, K) == 0) return false;
+  if (kernel_maxc(g, K) == 0 || g->ctx->tune.wrap_in_lds == 0) return false;
   size_t b = solver_lds_bytes((int)g->npl, K);
   if (pred_fits_lds(g, K)) b += (size_t)(g->L + 1) * K * g->npl * 4;
   return b + (size_t)g->npl * 8 <= 160 * 1024 - 1024;
@@ -621,7 +642,9 @@ bool wrap_fits_lds(const pgc_graph* g, int K) {
 // K == 1 graphs of the unrolled sweep (one chunk per wave, 8 slots) with pred
 // and the wrap map in LDS: two rows of chain origins follow when they fit.
 bool org_fits_lds(const pgc_graph* g, int K) {
-  if (K != 1 || kernel_maxc(g, K) != 1 || g->maxw > 8 || !pred_fits_lds(g, K) || !wrap_fits_lds(g, K)) return false;
+  if (K != 1 || kernel_maxc(g, K) != 1 || g->maxw > 8 || !pred_fits_lds(g, K) || !wrap_fits_lds(g, K) ||
+      g->ctx->tune.origins == 0)
+    return false;
   const size_t b = solver_lds_bytes((int)g->npl, K) + (size_t)(g->L + 1) * K * g->npl * 4 + (size_t)g->npl * 8;
   return b + (size_t)g->npl * 8 <= 160 * 1024 - 1024;
 }
@@ -723,6 +746,8 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   j.numNodes = g->numNodes;
   j.mode = mode;
   j.flags = flags;
+  if (g->ctx->tune.cached_sources == 0) j.flags |= FLAG_NO_CACHED_SOURCES;
+  if (g->ctx->tune.short_slots == 0) j.flags |= FLAG_FULL_SLOTS;
   j.branchId = 0;
   j.chain_stride = (int32_t)(g->L * K + 2);
   j.cyc_cap = s.cyc_cap;
@@ -808,7 +833,16 @@ int grid_blocks(const pgc_graph* g, int K, bool mincost) {
     return e ? std::atoi(e) : -1;
   }();
   if (env == 0) return 0;
-  int n = env > 0 ? env : (g->nChunks >= 128 ? (g->nChunks + 15) / 16 : 0);
+  const int cpw = g->ctx->tune.grid_chunks_per_wave;
+  int n;
+  if (env > 0)
+    n = env;
+  else if (cpw == 0)
+    n = 0;
+  else if (cpw > 0)
+    n = (g->nChunks + 16 * cpw - 1) / (16 * cpw);
+  else
+    n = g->nChunks >= 128 ? (g->nChunks + 15) / 16 : 0;
   if (n <= 0) return 0;
   const int cap = solver_grid_capacity(g->maxw, mincost, g->d_time[0] != nullptr, std::max(1, g->ctx->num_cu));
   n = std::min(n, std::min(cap, 256));

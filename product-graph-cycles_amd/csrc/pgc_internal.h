@@ -80,6 +80,8 @@ enum : uint32_t {
   FLAG_MINCOST_CPU_RULES = 2u, // min-cost: double lower-bound keys
   FLAG_CORRECTED_ODD_L = 4u,   // odd L, K >= 2: the closing layer's depth-0 row (not row L % 2)
   FLAG_FRESH_JOB = 1u << 16,   // the solve's only launch: the kernel clears neg_len and an LDS pred table
+  FLAG_NO_CACHED_SOURCES = 1u << 17,  // tuning: load the source slots every step (no register copy)
+  FLAG_FULL_SLOTS = 1u << 18,         // tuning: read every slot of the layout's width (no short instances)
 };
 
 struct JobResult {

@@ -498,8 +498,8 @@ struct Solver {
       constexpr int SX = W == 8 ? 7 : (W == 16 ? 12 : W), SI = W == 8 || W == 16 ? 6 : W;
       // with at most one chunk per wave the sources of a wave's chunk stay in
       // registers for the whole sweep (layer-invariant sources)
-      const bool one = nC <= nw;
-      const bool shortSlots = SX < W && J.inter.deg <= SX && J.intra.deg <= SI;
+      const bool one = nC <= nw && !(J.flags & FLAG_NO_CACHED_SOURCES);
+      const bool shortSlots = SX < W && J.inter.deg <= SX && J.intra.deg <= SI && !(J.flags & FLAG_FULL_SLOTS);
       int p0 = (int)gor.prev0, p1 = (int)gor.prev1, od = gor.odd ? 1 : 0;
       if (one && shortSlots)
         r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 1, SX, SI, true>(
@@ -522,7 +522,7 @@ struct Solver {
       // short slots for the interlayer / intralayer classes: 7 / 6 of 8 (icosphere
       // product graphs), 12 / 6 of 16 (conjugate product graphs: in-degrees <= 10 / 5)
       constexpr int SX = W == 8 ? 7 : (W == 16 ? 12 : W), SI = W == 8 || W == 16 ? 6 : W;
-      if (SX < W && J.inter.deg <= SX && J.intra.deg <= SI)
+      if (SX < W && J.inter.deg <= SX && J.intra.deg <= SI && !(J.flags & FLAG_FULL_SLOTS))
         r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 0, SX, SI>((const SolverJob*)&J, D, Dmin, &sh,
                                                                             nullptr, 0, bor.prev0, bor.prev1,
                                                                             bor.odd ? 1 : 0, ratio, ub);
@@ -534,7 +534,7 @@ struct Solver {
       constexpr int MC = W == 8 && MAXC <= (HAS_TIME ? 2 : 3) ? MAXC : 0;
       // icosphere-like graphs: interlayer in-degree <= 7 and intralayer <= 6
       // leave the last one and two slots padding on every node
-      if (MC > 0 && J.inter.deg <= W - 1 && J.intra.deg <= W - 2)
+      if (MC > 0 && J.inter.deg <= W - 1 && J.intra.deg <= W - 2 && !(J.flags & FLAG_FULL_SLOTS))
         r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, true, MC, (MC > 0 ? W - 1 : W), (MC > 0 ? W - 2 : W)>(
             (const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl, predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0,
             ratio, ub);
@@ -547,7 +547,7 @@ struct Solver {
       r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
                                                        predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0, ratio,
                                                        ub);
-    else if (J.inter.deg <= W - 1)  // the last interlayer slot is padding on every node
+    else if (J.inter.deg <= W - 1 && !(J.flags & FLAG_FULL_SLOTS))  // the last interlayer slot is padding on every node
       r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W, true, W - 1>((const SolverJob*)&J, D, Dmin, &sh,
                                                                     (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,
                                                                     bor.prev1, bor.odd ? 1 : 0, ratio, ub,

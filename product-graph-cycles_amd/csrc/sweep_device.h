@@ -420,7 +420,7 @@ struct Sweep {
     Slots A, B;
     if constexpr (ONLY_K1 && PF) {
       if (nb <= 0) return 1;
-      if (Sx.src_inv) return run_layers_k1_inv(ratio, rt, ub, dmin_r);
+      if (Sx.src_inv && !(uni((int)J->flags) & (int)FLAG_NO_CACHED_SOURCES)) return run_layers_k1_inv(ratio, rt, ub, dmin_r);
       return run_layers_k1(A, B, ratio, rt, ub, dmin_r);
     }
     bool inter;
@@ -990,7 +990,8 @@ struct SweepG {
   __device__ __forceinline__ int run_layers(double ratio, double ub) {
     const double rt = __dmul_rn(ratio, uni(J->time_const));
     if constexpr (MC > 0) {
-      if (Sx.src_inv && (K == 1 || Sa.src_inv)) return run_layers_cached(ratio, rt, ub);
+      if (Sx.src_inv && (K == 1 || Sa.src_inv) && !(uni((int)J->flags) & (int)FLAG_NO_CACHED_SOURCES))
+        return run_layers_cached(ratio, rt, ub);
     }
     for (int layer = 0; layer <= L; ++layer) {
       const int cur = (layer & 1) * K;

@@ -47,13 +47,30 @@ class pgc_result(C.Structure):
         return d
 
 
+class pgc_tuning(C.Structure):
+    _fields_ = [
+        ("grid_chunks_per_wave", C.c_int32),
+        ("pred_in_lds", C.c_int32),
+        ("wrap_in_lds", C.c_int32),
+        ("origins", C.c_int32),
+        ("cached_sources", C.c_int32),
+        ("short_slots", C.c_int32),
+        ("reserved", C.c_int32 * 2),
+    ]
+    FIELDS = ("grid_chunks_per_wave", "pred_in_lds", "wrap_in_lds", "origins", "cached_sources", "short_slots")
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in self.FIELDS}
+
+
 EXPORTED = [
     "pgc_last_error", "pgc_abi_version", "pgc_context_create", "pgc_context_destroy", "pgc_graph_generate",
     "pgc_graph_generate_device",
     "pgc_graph_from_arrays", "pgc_graph_set_branch_graph", "pgc_graph_info", "pgc_graph_get_product_space",
     "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
     "pgc_solve_ratio_batch", "pgc_graph_set_feat_diff", "pgc_graphs_set_feat_diff", "pgc_graph_derive",
-    "pgc_cycle_ratio_values", "pgc_cycle_ratio_grad", "pgc_solve_ratio_noalloc",
+    "pgc_cycle_ratio_values", "pgc_cycle_ratio_grad", "pgc_solve_ratio_noalloc", "pgc_context_set_tuning",
+    "pgc_context_get_tuning",
 ]
 
 _lib = None
@@ -66,6 +83,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise PgcError("libpgc.so is not built (%s); run `make -C product-graph-cycles_amd`" % LIB_PATH)
+    # PyTorch ships its own HIP runtime. In a process that uses both, torch's
+    # must be loaded first: loaded after libpgc's (ROCm's own), it finds no
+    # device. Importing torch here fixes the order.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P, I64, I32, D = C.c_void_p, C.c_int64, C.c_int32, C.c_double
     L.pgc_last_error.restype = C.c_char_p
@@ -73,6 +97,8 @@ def load():
     L.pgc_context_create.argtypes = [C.c_int, C.POINTER(P)]
     L.pgc_context_destroy.argtypes = [P]
     L.pgc_context_destroy.restype = None
+    L.pgc_context_set_tuning.argtypes = [P, C.POINTER(pgc_tuning)]
+    L.pgc_context_get_tuning.argtypes = [P, C.POINTER(pgc_tuning)]
     L.pgc_graph_generate.argtypes = [P, P, I64, P, I64, P, I64, P, I64, P, C.c_int, C.c_int, C.c_int,
                                      C.c_char_p, C.c_char_p, P, P, D, D, D, C.POINTER(P)]
     L.pgc_graph_generate_device.argtypes = L.pgc_graph_generate.argtypes
@@ -121,6 +147,23 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+    def set_tuning(self, choices=None):
+        """Launch choices (a dict with pgc_tuning's fields; missing fields and
+        None keep the built-in rule, -1)."""
+        t = pgc_tuning()
+        for k in pgc_tuning.FIELDS:
+            setattr(t, k, -1)
+        for k, v in (choices or {}).items():
+            if k not in pgc_tuning.FIELDS:
+                raise ValueError("unknown tuning field %r" % k)
+            setattr(t, k, -1 if v is None else int(v))
+        check(load().pgc_context_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self):
+        t = pgc_tuning()
+        check(load().pgc_context_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
 
     def close(self):
         if self._h:

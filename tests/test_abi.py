@@ -26,7 +26,7 @@ def test_header_matches_exports():
 
 
 def test_abi_version():
-    assert _lib.load().pgc_abi_version() == 2
+    assert _lib.load().pgc_abi_version() == 3
 
 
 def test_errors_are_status_codes():
@@ -97,3 +97,10 @@ def test_normals_helpers():
     # a curve centred on the origin is taken for a yz/xz curve
     centred = cpg.compute_normals_2d(radial, EY)
     assert not np.allclose(np.einsum("ij,ij->i", centred, radial), 1.0)
+
+
+def test_tuning_needs_a_context():
+    L = _lib.load()
+    t = _lib.pgc_tuning()
+    assert L.pgc_context_set_tuning(None, C.byref(t)) == -1
+    assert L.pgc_context_get_tuning(None, C.byref(t)) == -1
