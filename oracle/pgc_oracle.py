@@ -5,7 +5,8 @@ by the product package. Mirrors the reference's CPU path:
 product_graph_generators/product_spaces/product_space.cpp:19-132,
 product_graph_generators/energy/deformationEnergy.cpp:11-103,
 product_graph_generators/cost_time_ratio_solver/cost_time_ratio_solver.cpp:42-905,
-product_graph_generators/dijkstra_solver/dijkstra_solver.cpp:41-579.
+product_graph_generators/dijkstra_solver/dijkstra_solver.cpp:41-579,
+helper/utils.cpp:86-157 (normals).
 """
 from __future__ import annotations
 
@@ -55,6 +56,10 @@ def lib():
         L.orc_conj_energy.argtypes = [P, P, P, C.c_long, P, P, C.c_long, P, C.c_long, P, P, P, C.c_long, P,
                                       C.c_long, C.c_int, C.c_char_p, C.c_char_p, C.c_double, C.c_double,
                                       C.c_double, C.c_int, P]
+        L.orc_normals2d.restype = C.c_int
+        L.orc_normals2d.argtypes = [P, C.c_long, P, C.c_long, P, P]
+        L.orc_normals3d.restype = None
+        L.orc_normals3d.argtypes = [P, C.c_long, P, C.c_long, P]
         _LIB = L
     return _LIB
 
@@ -230,3 +235,23 @@ def conj_energy(VX, NX, VY, NY, EX, EY, rows, src, trgt, FD, reg=False, cost_mod
                          % (cost_mode, time_mode))
     return out[: nc * E].reshape(nc, E).T.copy()
 
+
+
+def normals2d(V, E):
+    """utils::normals2d (helper/utils.cpp:93-157). Returns (status, zero_dim,
+    normals): status 0, or 1 / 2 for the reference's two error returns."""
+    V = np.ascontiguousarray(V, np.float64)
+    E = np.ascontiguousarray(E, np.int32)
+    out = np.zeros((len(V), 3))
+    zd = np.zeros(1, np.int32)
+    st = lib().orc_normals2d(_p(V), len(V), _p(E), len(E), _p(out), _p(zd))
+    return int(st), int(zd[0]), out
+
+
+def normals3d(V, F):
+    """utils::normals3d = igl::per_vertex_normals, default weighting (helper/utils.cpp:86-90)."""
+    V = np.ascontiguousarray(V, np.float64)
+    F = np.ascontiguousarray(F, np.int32)
+    out = np.zeros((len(V), 3))
+    lib().orc_normals3d(_p(V), len(V), _p(F), len(F), _p(out))
+    return out

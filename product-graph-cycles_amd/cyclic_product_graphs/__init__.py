@@ -181,21 +181,56 @@ class product_graph_generator:
 
 
 # ------------------------------------------------------------- helpers
+def _normalize_rows(M):
+    """Each row divided by its Euclidean norm, with no guard: a zero row
+    becomes NaN, as Eigen 3.4's rowwise normalize() divides by the norm."""
+    nrm = np.sqrt(np.add.reduce(M * M, axis=1))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return M / nrm[:, None]
+
+
 def compute_normals_3d(V, F):
-    """Area-weighted per-vertex normals (utils::normals3d, helper/utils.cpp:86-90)."""
+    """Per-vertex normals (utils::normals3d, helper/utils.cpp:86-90), which is
+    igl::per_vertex_normals(V, F, N) of libigl b1bd5b1 with its default
+    weighting: every face adds its unit normal ((v1-v0) x (v2-v0), zero for a
+    degenerate face) times its double area to its three corners, face by face;
+    the sums are normalised (a vertex on no face gets NaN)."""
     V = np.asarray(V, np.float64)
     F = np.asarray(F, np.int64)
-    fn = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
-    n = np.zeros_like(V)
-    for k in range(3):
-        np.add.at(n, F[:, k], fn)
-    nrm = np.linalg.norm(n, axis=1, keepdims=True)
-    nrm[nrm == 0] = 1
-    return n / nrm
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    n = np.cross(p1 - p0, p2 - p0)
+    r = np.sqrt(np.add.reduce(n * n, axis=1))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        fn = np.where(r[:, None] == 0.0, 0.0, n / r[:, None])
+    # double area: the double areas projected on (x,y), (y,z), (z,x), each
+    # from corner 2, combined as the root of their summed squares
+    w = np.zeros(len(F))
+    for d in range(3):
+        e = (d + 1) % 3
+        pa = (p0[:, d] - p2[:, d]) * (p1[:, e] - p2[:, e]) - (p0[:, e] - p2[:, e]) * (p1[:, d] - p2[:, d])
+        w = w + pa * pa
+    w = np.sqrt(w)
+    N = np.zeros_like(V)
+    # corners in face order (np.add.at applies repeated indices in order)
+    np.add.at(N, F.reshape(-1), np.repeat(w[:, None] * fn, 3, axis=0))
+    return _normalize_rows(N)
 
 
 def compute_normals_2d(V, E):
-    """Outward vertex normals of a closed planar curve (utils::normals2d, helper/utils.cpp:93-157)."""
+    """Outward vertex normals of a closed planar curve (utils::normals2d,
+    helper/utils.cpp:93-157).
+
+    The curve must lie in a coordinate plane: the first coordinate column
+    whose sum is below 1e-4 in magnitude is taken as the dropped one (a curve
+    centred on the origin can make the wrong column qualify, as in the
+    reference), and the columns are permuted so that it comes last; the
+    normals are returned in that permuted frame, as the reference returns
+    them. Edge normals are the edge vectors turned by -90 degrees about z,
+    flipped when moving the points by a tenth of the smaller bounding-box side
+    along them does not grow the box in +x (the box values and that tenth are
+    float, as declared in the reference); a vertex normal is the
+    length-weighted sum of its two edges' normals, normalised (NaN for a zero
+    sum, as in Eigen's rowwise normalisation)."""
     V = np.array(V, np.float64)
     E = np.asarray(E, np.int64)
     if len(V) != len(E):
@@ -212,13 +247,14 @@ def compute_normals_2d(V, E):
     if zero != 2:
         V = V[:, [0, 2, 1]] if zero == 1 else V[:, [1, 2, 0]]
     ev = V[E[:, 0]] - V[E[:, 1]]
-    el = np.linalg.norm(ev, axis=1)
+    el = np.sqrt(np.add.reduce(ev * ev, axis=1))
     R = np.array([[0, 1, 0], [-1, 0, 0], [0, 0, 1]], np.float64)
-    en = ev @ R.T
-    en = en / np.linalg.norm(en, axis=1, keepdims=True)
+    en = _normalize_rows(ev @ R.T)
     maxX, minX = np.float32(V[:, 0].max()), np.float32(V[:, 0].min())
     maxY, minY = np.float32(V[:, 1].max()), np.float32(V[:, 1].min())
-    mult = np.float32(0.1 * min(abs(maxX - minX), abs(maxY - minY)))
+    # float - float, then 0.1 * float in double, narrowed to float
+    side = min(abs(maxX - minX), abs(maxY - minY))
+    mult = np.float32(0.1 * float(side))
     if np.float32((V + float(mult) * en)[:, 0].max()) < maxX:
         en = -en
     n = len(V)
@@ -226,7 +262,7 @@ def compute_normals_2d(V, E):
     for i in range(1, n + 1):
         j = i % n
         vn[j] = el[i - 1] * en[i - 1] + el[j] * en[j]
-    return vn / np.linalg.norm(vn, axis=1, keepdims=True)
+    return _normalize_rows(vn)
 
 
 def compute_local_thickness_3d():
