@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(_HERE, "libpgc.so")
 PGC_LAWLER_PICK_LAST = 1
 PGC_MINCOST_CPU_RULES = 2
 PGC_CORRECTED_ODD_L = 4
+# status codes (include/pgc.h)
+PGC_OK, PGC_ERR_ARG, PGC_ERR_HIP, PGC_ERR_SOLVER, PGC_ERR_CAPACITY, PGC_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 
 
 class PgcError(RuntimeError):
