@@ -74,7 +74,11 @@ typedef struct pgc_tuning {
                                    chase, when they fit: -1 yes, 0 no */
   int32_t cached_sources;       /* layer-invariant source slots held in registers: -1 yes, 0 no */
   int32_t short_slots;          /* skip slots that are padding on every node: -1 yes, 0 no */
-  int32_t reserved[2];
+  int32_t global_table;         /* 1: every graph keeps its distance table in global memory (a
+                                   validation choice: the same results); other values: the rule */
+  int32_t grid_workgroups;      /* n >= 1: graphs with the table in global memory are solved by n
+                                   workgroups of one cooperative launch (at most what the device
+                                   holds at once); other values: grid_chunks_per_wave decides */
 } pgc_tuning;
 
 const char* pgc_last_error(void);

@@ -221,7 +221,7 @@ struct DevAlloc {
 struct pgc_context {
   int device = 0;
   int num_cu = 0;
-  pgc_tuning tune = {-1, -1, -1, -1, -1, -1, {0, 0}};  // launch choices (-1: the built-in rule)
+  pgc_tuning tune = {-1, -1, -1, -1, -1, -1, -1, -1};  // launch choices (-1: the built-in rule)
   hipStream_t stream = nullptr;
   void* select_state = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -369,7 +369,7 @@ struct pgc_graph {
 extern "C" {
 
 const char* pgc_last_error(void) { return g_last_error.c_str(); }
-int pgc_abi_version(void) { return 3; }
+int pgc_abi_version(void) { return 4; }
 
 int pgc_context_create(int device, pgc_context** out) {
   if (!out) return fail(PGC_ERR_ARG, "out is null");
@@ -399,13 +399,16 @@ int pgc_context_create(int device, pgc_context** out) {
 int pgc_context_set_tuning(pgc_context* c, const pgc_tuning* t) {
   if (!c) return fail(PGC_ERR_ARG, "context is null");
   if (!t) {
-    c->tune = pgc_tuning{-1, -1, -1, -1, -1, -1, {0, 0}};
+    c->tune = pgc_tuning{-1, -1, -1, -1, -1, -1, -1, -1};
     return PGC_OK;
   }
   if (t->grid_chunks_per_wave < -1 || t->grid_chunks_per_wave > 64)
     return fail(PGC_ERR_ARG, "grid_chunks_per_wave must be -1, 0 or 1..64");
   for (int32_t v : {t->pred_in_lds, t->wrap_in_lds, t->origins, t->cached_sources, t->short_slots})
     if (v != -1 && v != 0) return fail(PGC_ERR_ARG, "placement choices must be -1 (rule) or 0 (off)");
+  if (t->global_table < -1 || t->global_table > 1) return fail(PGC_ERR_ARG, "global_table must be -1, 0 or 1");
+  if (t->grid_workgroups < -1 || t->grid_workgroups > 256)
+    return fail(PGC_ERR_ARG, "grid_workgroups must be -1, 0 or 1..256");
   c->tune = *t;
   return PGC_OK;
 }
@@ -617,7 +620,7 @@ int kernel_maxc(const pgc_graph* g, int K) {
     const char* e = std::getenv("PGC_GLOBAL_TABLE");
     return e && e[0] == '1';
   }();
-  if (force_global) return 0;
+  if (force_global || g->ctx->tune.global_table == 1) return 0;
   const int per = (int)((g->nChunks + (PGC_BLOCK / PGC_WAVE) - 1) / (PGC_BLOCK / PGC_WAVE));
   if (per > 8 || solver_lds_bytes((int)g->npl, K) > 160 * 1024 - 1024) return 0;
   if (g->maxw > 32) return 8;  // the runtime-width variant has the one LDS instance
@@ -824,8 +827,8 @@ static thread_local double g_phase_ms[5];
 // nodes per layer are solved by ceil(chunks / 16) workgroups, one chunk per
 // wave (C2's 720 chunks: 45 workgroups); smaller ones keep one workgroup, as
 // a device-wide barrier per step would cost more than their steps take.
-// PGC_GRID=0 turns it off; PGC_GRID=n forces n workgroups for every such job
-// (a validation switch for small graphs).
+// PGC_GRID=0 turns it off; PGC_GRID=n, or the tuning choice grid_workgroups = n,
+// forces n workgroups for every such job (a validation switch for small graphs).
 int grid_blocks(const pgc_graph* g, int K, bool mincost) {
   if (kernel_maxc(g, K) != 0 || g->maxw > 32) return 0;
   static const int env = [] {
@@ -837,6 +840,8 @@ int grid_blocks(const pgc_graph* g, int K, bool mincost) {
   int n;
   if (env > 0)
     n = env;
+  else if (g->ctx->tune.grid_workgroups > 0)
+    n = g->ctx->tune.grid_workgroups;
   else if (cpw == 0)
     n = 0;
   else if (cpw > 0)

@@ -501,7 +501,19 @@ struct Solver {
       const bool one = nC <= nw && !(J.flags & FLAG_NO_CACHED_SOURCES);
       const bool shortSlots = SX < W && J.inter.deg <= SX && J.intra.deg <= SI && !(J.flags & FLAG_FULL_SLOTS);
       int p0 = (int)gor.prev0, p1 = (int)gor.prev1, od = gor.odd ? 1 : 0;
-      if (one && shortSlots)
+      const bool k2 = W != 0 && one && K == 2 && J.inter.src_inv && J.intra.src_inv;
+      // conjugate graphs of valence-6 meshes: interlayer in-degree <= 10
+      constexpr int SX10 = W == 16 ? 10 : SX;
+      if (k2 && shortSlots && J.inter.deg <= SX10)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 1, SX10, SI, true, true>(
+            (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
+      else if (k2 && shortSlots)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 1, SX, SI, true, true>(
+            (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
+      else if (k2)
+        r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 1, W, W, true, true>(
+            (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
+      else if (one && shortSlots)
         r = sweep_layers_global<CT, HAS_TIME, MINCOST, W, false, 1, SX, SI, true>(
             (const SolverJob*)&J, D, Dmin, (Shared*)lsh, nullptr, 0, p0, p1, od, ratio, ub, gs, &bgen);
       else if (one)

@@ -941,6 +941,130 @@ struct SweepG {
     }
     return imp;
   }
+  // Device-wide sweep with K == 2 and one chunk per wave. Every step reads its
+  // rows from memory shared by the whole grid, so each dependent read costs a
+  // full memory round trip; this schedule has one per step. The interlayer
+  // step gathers both depth rows of its sources and takes the first minimum in
+  // (slot, depth) order in registers. That is the winner the minimum-over-
+  // depths row and the depth recovery give: rounding is monotonic, so
+  // min_k(D_k + w) = min_k(D_k) + w, and the first slot attaining the minimum
+  // and the first depth of that slot attaining it are the same in both. With
+  // no minimum row, the intralayer step writes only its depth row.
+  // The sources are held as 32-bit byte offsets into a row (a row has at most
+  // 2^28 entries here), so every gather is a scalar row base plus a vector
+  // offset and no 64-bit address stays live across the loop.
+  // The empty asm marks the offset as rewritten at every use: without it the
+  // compiler hoists a 64-bit address per (slot, row parity) out of the layer
+  // loop and spills them.
+  __device__ __forceinline__ double rget(const DP row, uint32_t off) const {
+    asm volatile("" : "+v"(off));
+    return *(const PGC_GLOBAL double*)((const PGC_GLOBAL char*)row + off);
+  }
+  __device__ __forceinline__ int run_layers_grid_k2(double ratio, double rt, double ub) {
+    const int c = wave;
+    const bool own = c < nC;
+    const int o = (own ? c : 0) * W * 64 + lane;
+    const int v = c * 64 + lane;
+    const bool act = own && v < npl;
+    // the interlayer sources stay in registers (layer 1's pattern); the
+    // intralayer ones are loaded with their costs (layer 0's pattern, one
+    // load each beside the cost loads, so no added wait)
+    uint32_t oX[WX];
+    {
+      const PGC_GLOBAL int* __restrict__ srcX = gp(Sx.src);
+#pragma unroll
+      for (int j = 0; j < WX; ++j) oX[j] = (uint32_t)srcX[Sx.lstride + o + j * 64] << 3;
+    }
+    const PGC_GLOBAL int* __restrict__ srcI = gp(Sa.src);
+    for (int layer = 0; layer <= L; ++layer) {
+      const int cur = (layer & 1) * 2;
+      const int prev = 2 - cur;
+      bool imp = false;
+      const bool check = layer == L - 1 && layer > 0;
+      if (layer > 0) {
+        if (own) {
+          CT cc[WX];
+          float t[WX];
+          load_ct(Sx, layer * Sx.lstride + o, cc, t);
+          const DP D0 = D + prev * ds, D1 = D + (prev + 1) * ds;
+          double d0[WX], d1[WX];
+#pragma unroll
+          for (int j = 0; j < WX; ++j) {
+            d0[j] = rget(D0, oX[j]);
+            d1[j] = rget(D1, oX[j]);
+          }
+          double best = PGC_INF;
+          uint32_t bo = (uint32_t)npl << 3;
+          int ks = 0;
+#pragma unroll
+          for (int j = 0; j < WX; ++j) {
+            const double w = weight(cc[j], t[j], ratio, rt);
+            const double a = d0[j] + w;
+            const bool b0 = a < best;
+            best = pgc_min(best, a);
+            const double b = d1[j] + w;
+            const bool b1 = b < best;
+            best = pgc_min(best, b);
+            bo = b0 || b1 ? oX[j] : bo;
+            ks = b1 ? 1 : (b0 ? 0 : ks);
+          }
+          if (act) {
+            double res = PGC_INF;
+            if (best < PGC_INF && best < ub) {
+              pred[(int64_t)layer * npl * 2 + v] = (int)(bo >> 3) + (layer - 1) * npl * 2 + ks * npl;
+              res = best;
+              imp = true;
+            }
+            dput(D + cur * ds + v, res);
+          }
+        }
+        sync();
+      }
+      if (layer < L) {
+        if (own) {
+          CT cc[WI];
+          float t[WI];
+          uint32_t oI[WI];
+#pragma unroll
+          for (int j = 0; j < WI; ++j) oI[j] = (uint32_t)srcI[o + j * 64] << 3;
+          load_ct(Sa, layer * Sa.lstride + o, cc, t);
+          const DP D0 = D + cur * ds;
+          double dv[WI];
+#pragma unroll
+          for (int j = 0; j < WI; ++j) dv[j] = rget(D0, oI[j]);
+          const double curv = (layer == 0 && act) ? dget(D + (cur + 1) * ds + v) : PGC_INF;
+          double best = PGC_INF;
+          uint32_t bo = (uint32_t)npl << 3;
+#pragma unroll
+          for (int j = 0; j < WI; ++j) {
+            const double nv = dv[j] + weight(cc[j], t[j], ratio, rt);
+            const bool better = nv < best;
+            best = pgc_min(best, nv);
+            bo = better ? oI[j] : bo;
+          }
+          if (act) {
+            double res = curv;
+            if (best < curv && best < ub) {
+              pred[(int64_t)layer * npl * 2 + npl + v] = (int)(bo >> 3) + layer * npl * 2;
+              res = best;
+              imp = true;
+            }
+            dput(D + (cur + 1) * ds + v, res);
+          }
+        }
+      } else if (act) {
+        // the closing layer has no intralayer edges: its depth-1 row is +inf
+        dput(D + (cur + 1) * ds + v, PGC_INF);
+      }
+      if (check) {
+        if (!bor(imp)) return 0;
+      } else {
+        sync();
+      }
+    }
+    return 1;
+  }
+
   __device__ __forceinline__ int run_layers_cached(double ratio, double rt, double ub) {
     constexpr int M = MC > 0 ? MC : 1;
     int sX[M][WX], sI[M][WI];
@@ -1024,8 +1148,11 @@ struct SweepG {
   }
 };
 
+// K2: the device-wide K == 2 schedule (run_layers_grid_k2), a separate
+// instance so that its registers are allocated apart from the general loops;
+// the caller has checked that both slot classes have layer-invariant sources.
 template <typename CT, bool HAS_TIME, bool MINCOST, int W, bool LD = false, int MC = 0, int WX = W, int WI = W,
-          bool GRID = false>
+          bool GRID = false, bool K2 = false>
 __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, double* Dg, double* Dming, Shared* shg,
                                                        int32_t* predLg, int predLds, int p0, int p1, int odd,
                                                        double ratio, double ub, GridState* gs = nullptr,
@@ -1061,7 +1188,10 @@ __device__ __noinline__ SweepState sweep_layers_global(const SolverJob* Jp, doub
   w.p1 = uni(p1);
   w.odd = uni(odd);
   SweepState r;
-  r.complete = w.run_layers(uni(ratio), uni(ub));
+  if constexpr (K2 && GRID && MC == 1 && W != 0)
+    r.complete = w.run_layers_grid_k2(uni(ratio), __dmul_rn(uni(ratio), uni(w.J->time_const)), uni(ub));
+  else
+    r.complete = w.run_layers(uni(ratio), uni(ub));
   r.p0 = w.p0;
   r.p1 = w.p1;
   r.odd = w.odd;

@@ -57,9 +57,11 @@ class pgc_tuning(C.Structure):
         ("origins", C.c_int32),
         ("cached_sources", C.c_int32),
         ("short_slots", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("global_table", C.c_int32),
+        ("grid_workgroups", C.c_int32),
     ]
-    FIELDS = ("grid_chunks_per_wave", "pred_in_lds", "wrap_in_lds", "origins", "cached_sources", "short_slots")
+    FIELDS = ("grid_chunks_per_wave", "pred_in_lds", "wrap_in_lds", "origins", "cached_sources", "short_slots",
+              "global_table", "grid_workgroups")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}

@@ -26,7 +26,7 @@ def test_header_matches_exports():
 
 
 def test_abi_version():
-    assert _lib.load().pgc_abi_version() == 3
+    assert _lib.load().pgc_abi_version() == 4
 
 
 def test_errors_are_status_codes():

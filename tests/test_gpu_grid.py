@@ -9,25 +9,26 @@ counters equal.
 
 Cases: conjugate graphs of 11,520 and 46,080 nodes per layer (12 and 45
 workgroups) with both solvers; and the small parity cases and the branching
-instances in a subprocess that forces the global table and a three-workgroup
-device-wide solve for every single solve (PGC_GLOBAL_TABLE=1, PGC_GRID=3).
-The full-size conjugate configuration (46,080 nodes per layer, 45
-workgroups) is in tests/test_gpu_configs.py.
+instances with the context's launch choices forcing the global table and a
+three-workgroup device-wide solve for every single solve (global_table = 1,
+grid_workgroups = 3), in this process: a second process on the GPU cannot
+make cooperative launches while this one holds the device. The full-size
+conjugate configuration (46,080 nodes per layer, 45 workgroups) is in
+tests/test_gpu_configs.py.
 """
-import os
-import subprocess
-import sys
+import itertools
 
 import numpy as np
 import pytest
 
 import pgc_oracle as O
 import cyclic_product_graphs as cpg
+import test_gpu_bnb
+import test_gpu_parity
+from cyclic_product_graphs import _lib
 from cyclic_product_graphs import synthetic as S
 
 pytestmark = pytest.mark.gpu
-
-HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.mark.parametrize("sub,ny,K,cost,time", [(3, 24, 2, "vanilla", "nomode"), (3, 10, 3, "vanilla", "vanilla"),
@@ -57,10 +58,40 @@ def test_conjugate_device_wide(sub, ny, K, cost, time, name):
     assert (st["sweeps"], st["bf_calls"], st["branches"]) == (stats[1], stats[0], stats[3])
 
 
+def _cases(fn):
+    """The argument sets of a test function's parametrize marks (their product)."""
+    axes = []
+    for m in getattr(fn, "pytestmark", []):
+        if m.name != "parametrize":
+            continue
+        names = [n.strip() for n in m.args[0].split(",")] if isinstance(m.args[0], str) else list(m.args[0])
+        vals = [getattr(v, "values", v) for v in m.args[1]]
+        axes.append([dict(zip(names, v if len(names) > 1 else (v,))) for v in vals])
+    for combo in itertools.product(*axes):
+        kw = {}
+        for d in combo:
+            kw.update(d)
+        yield kw
+
+
+SELECT = ("tiny", "meshes", "modes", "pruned", "full_slot_width", "corrected", "single_solves")
+
+
 def test_small_cases_device_wide_forced():
-    env = dict(os.environ, PGC_GLOBAL_TABLE="1", PGC_GRID="3")
-    cmd = [sys.executable, "-u", "-m", "pytest", "-x", "-q", "-m", "gpu", "-p", "no:cacheprovider",
-           os.path.join(HERE, "test_gpu_parity.py"), os.path.join(HERE, "test_gpu_bnb.py"),
-           "-k", "tiny or meshes or modes or pruned or full_slot_width or corrected or single_solves"]
-    r = subprocess.run(cmd, env=env, cwd=os.path.dirname(HERE), timeout=900)
-    assert r.returncode == 0
+    ctx = _lib.default_context(0)
+    saved = ctx.tuning()
+    ctx.set_tuning({"global_table": 1, "grid_workgroups": 3})
+    n = 0
+    try:
+        for mod in (test_gpu_parity, test_gpu_bnb):
+            for name in sorted(dir(mod)):
+                fn = getattr(mod, name)
+                if not (name.startswith("test_") and callable(fn) and any(k in name for k in SELECT)):
+                    continue
+                for kw in _cases(fn):
+                    print("case forced grid", name, kw, flush=True)
+                    fn(**kw)
+                    n += 1
+    finally:
+        ctx.set_tuning(saved)
+    assert n >= 150

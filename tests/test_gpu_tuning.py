@@ -72,7 +72,7 @@ def test_knob_keeps_results(knob, value):
             assert (res.sweeps, res.branches) == (stats[1], stats[3])
             g.close()
     ctx.set_tuning(None)
-    assert ctx.tuning() == {k: -1 for k in autotune.KNOBS}
+    assert ctx.tuning() == {k: -1 for k in _lib.pgc_tuning.FIELDS}
     ctx.close()
 
 
@@ -105,4 +105,12 @@ def test_tuning_argument_checks():
         ctx.set_tuning({"pred_in_lds": 1})
     with pytest.raises(_lib.PgcError):
         ctx.set_tuning({"grid_chunks_per_wave": -2})
+    with pytest.raises(_lib.PgcError):
+        ctx.set_tuning({"global_table": 2})
+    with pytest.raises(_lib.PgcError):
+        ctx.set_tuning({"grid_workgroups": 257})
+    ctx.set_tuning({"global_table": 1, "grid_workgroups": 3})
+    assert ctx.tuning()["global_table"] == 1 and ctx.tuning()["grid_workgroups"] == 3
+    ctx.set_tuning(None)
+    assert set(ctx.tuning().values()) == {-1}
     ctx.close()
