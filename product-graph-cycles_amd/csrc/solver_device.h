@@ -126,19 +126,21 @@ struct GridState {
 
 // Device-wide barrier: thread 0 of every workgroup adds one arrival to a
 // monotonic counter and waits until all workgroups have arrived at this
-// barrier (its ordinal times the grid size). FULL: a device-scope release
+// barrier (its ordinal times the grid size). Every wave first waits for its
+// own vector memory operations (the workgroup barrier does not), so that all
+// stores of the workgroup have reached its L2. FULL: a device-scope release
 // before the arrival (it writes back the XCD's L2, whose lines are not
 // coherent with the other XCDs' otherwise) and an acquire after the wait (it
 // invalidates the L2). Without FULL only the acquire: the barrier then orders
-// the data stored with device-scope (write-through) stores before it, which
-// every wave has completed before its workgroup arrives (the distance rows
-// between the steps of a sweep). A wait of more than two seconds (the wall
+// the data stored with device-scope (write-through) stores before it (the
+// distance rows between the steps of a sweep); plain stores (predecessors)
+// become visible to other workgroups at the next full barrier. A wait of more than two seconds (the wall
 // clock runs at 100 MHz) sets the abort flag instead of hanging; every later
 // barrier then returns at once. Returns false once the solve is aborted
 // (thread 0 reads the flag, the workgroup gets it through okFlag in LDS).
 template <bool FULL = true>
 __device__ __forceinline__ bool grid_sync(GridState* gs, uint64_t& bgen, uint32_t nblk, int* okFlag) {
-  if (!FULL) __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   bgen += 1;
   if (threadIdx.x == 0) {
@@ -529,6 +531,10 @@ struct Solver {
       gor.prev1 = (unsigned)r.p1;
       gor.odd = r.odd != 0;
       alive = alive && lsh->gridOk != 0;
+      // the sweep's last steps end in light barriers: a full one makes the
+      // predecessors they stored visible to every workgroup before the
+      // walks read them
+      sync();
       return r.complete;
     } else if constexpr (DYN) {
       // short slots for the interlayer / intralayer classes: 7 / 6 of 8 (icosphere

@@ -95,3 +95,34 @@ def test_small_cases_device_wide_forced():
     finally:
         ctx.set_tuning(saved)
     assert n >= 150
+
+
+@pytest.mark.parametrize("nblk", [3, 6, 12])
+def test_c1_size_with_idle_waves(nblk):
+    # a C1-size pair (icosphere s=4 x 128-point curve, max_depth 3; 41 chunks
+    # per layer) solved device-wide on 3 to 12 workgroups (48 to 192 waves,
+    # most of them without a chunk), against the one-workgroup LDS solve.
+    # Regression: the predecessors that the last steps of a sweep store must
+    # reach every workgroup before the leader walks them (a full barrier
+    # after the sweep, and every wave's stores drained before a barrier).
+    ctx = _lib.default_context(0)
+    p = S.make_pair(4, 128, seed=0)
+    fd = p["feat_diff"].astype(np.float64)
+
+    def solve():
+        g = _lib.Graph.generate(ctx, p["VX"], p["EX"], p["VY"], p["EY"], fd, False, False, False, "vanilla", "")
+        out = g.solve_ratio(3, _lib.PGC_LAWLER_PICK_LAST)
+        g.close()
+        return out
+
+    e0, r0, res0 = solve()
+    saved = ctx.tuning()
+    ctx.set_tuning({"global_table": 1, "grid_workgroups": nblk})
+    try:
+        print("case c1 grid", nblk, flush=True)
+        e1, r1, res1 = solve()
+    finally:
+        ctx.set_tuning(saved)
+    assert np.array_equal(e1, e0) and np.array_equal(r1, r0)
+    assert res1.objective == res0.objective
+    assert (res1.sweeps, res1.bf_calls, res1.branches) == (res0.sweeps, res0.bf_calls, res0.branches)
