@@ -138,6 +138,17 @@ int pgc_graph_get_cost(pgc_graph* g, double* out);             /* n_edges x n_co
 int pgc_graph_set_cost(pgc_graph* g, const double* cost);      /* same layout; replaces the energy */
 void pgc_graph_destroy(pgc_graph* g);
 
+/* The solver's in-edge layout of one class (0 intralayer, 1 interlayer):
+ * the device form of the reference's VECSET in-edge sets
+ * (ctr_bnb_subroutine.cuh:232-287, built by buildingInGraph,
+ * cost_time_ratio_solver.cpp:42-66), for inspection. Builds it if needed.
+ * Writes *n_slots and *width (slots per lane); if src and eid are not NULL
+ * and capacity >= *n_slots, copies the slots (slot j of lane i of chunk c on
+ * layer l at l*chunks*width*64 + (c*width + j)*64 + i): the source's index
+ * inside its layer (npl for padding) and the edge id (-1 for padding). */
+int pgc_graph_get_layout(pgc_graph* g, int cls, int32_t* src, int32_t* eid, int64_t capacity, int64_t* n_slots,
+                         int32_t* width);
+
 /* Lawler minimum cost/time-ratio cycle with branch and bound ("lawlergpu").
  * Replaces CostTimeRatioSolver::run -> solveWithBnbGPULawler
  * (cost_time_ratio_solver.cpp:868-900, cost_time_ratio_solver_cuda.cpp:175-331).

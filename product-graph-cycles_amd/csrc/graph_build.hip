@@ -707,6 +707,83 @@ __global__ void k_src_invariant(const int32_t* __restrict__ s, int64_t lstride, 
   }
 }
 
+// ---- in-edge layout of a generated non-conjugate product graph, in closed
+// form. Every layer repeats the X graph: node (l, x1) has the intralayer
+// in-edges (l, x0) -> (l, x1) of the X edges j = (x0, x1), edge id l*B + 2j,
+// on layers 0..L-1 (none on the closing layer), and the interlayer in-edges
+// (l-1, x0) -> (l, x1), edge id (l-1)*B + 2j+1 (j when pruned), plus the stay
+// edge (l-1, x1) -> (l, x1), edge id (l-1)*B + per_x*|EX| + x1, on layers
+// 1..L (layer L holds the closing copies). The slots of one layer (the
+// template) are built from the X graph, ordered by (source, edge id) as the
+// generic build orders them, and copied to every layer with the layer's edge
+// id base.
+//
+// in-degrees per class of the X graph: threads [0, nEX) take the X edges,
+// [nEX, nEX + nVX) the stay edges.
+__global__ void k_xcount(const int32_t* __restrict__ EX, int64_t nEX, int64_t nVX, int prune,
+                         int32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nEX) {
+    const int32_t x1 = EX[2 * i + 1];
+    if (!prune) atomicAdd(&cnt[x1], 1);
+    atomicAdd(&cnt[nVX + x1], 1);
+  } else if (i < nEX + nVX) {
+    atomicAdd(&cnt[nVX + (i - nEX)], 1);
+  }
+}
+
+// template slots (source vertex, edge id inside its layer) of the one layer;
+// the order inside a node is fixed afterwards by k_sort_slots
+__global__ void k_xfill(const int32_t* __restrict__ EX, int64_t nEX, int64_t nVX, int prune, int nChunks,
+                        const int32_t* __restrict__ width, int32_t* __restrict__ fill, int32_t* __restrict__ s0,
+                        int32_t* __restrict__ s1, int32_t* __restrict__ r0, int32_t* __restrict__ r1) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_x = prune ? 1 : 2;
+  int32_t x0, x1;
+  int64_t rI, rX;
+  if (i < nEX) {
+    x0 = EX[2 * i];
+    x1 = EX[2 * i + 1];
+    rI = 2 * i;
+    rX = per_x * i + (prune ? 0 : 1);
+  } else if (i < nEX + nVX) {
+    x0 = x1 = (int32_t)(i - nEX);
+    rI = -1;
+    rX = per_x * nEX + x0;
+  } else {
+    return;
+  }
+  const int c = x1 >> 6, lane = x1 & 63;
+  if (rI >= 0 && !prune) {
+    const int pos = atomicAdd(&fill[x1], 1);
+    const int64_t slot = ((int64_t)c * width[0] + pos) * 64 + lane;
+    s0[slot] = x0;
+    r0[slot] = (int32_t)rI;
+  }
+  const int pos = atomicAdd(&fill[nVX + x1], 1);
+  const int64_t slot = ((int64_t)c * width[1] + pos) * 64 + lane;
+  s1[slot] = x0;
+  r1[slot] = (int32_t)rX;
+}
+
+// every layer of one class from the template: src (padding -> the sentinel
+// npl) and edge id base + r (padding -> -1). Intralayer class (cls 0): layers
+// 0..L-1, base l*B; interlayer class: layers 1..L, base (l-1)*B. Layers
+// without in-edges of the class, and the kSlotPad tail, are all padding.
+__global__ void k_xreplicate(const int32_t* __restrict__ ts, const int32_t* __restrict__ tr, int64_t lstride,
+                             int64_t nLayers, int64_t total, int cls, int64_t B, int32_t npl,
+                             int32_t* __restrict__ s, int32_t* __restrict__ e) {
+  const int64_t L = nLayers - 1;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t l = i / lstride;
+    const int64_t o = i - l * lstride;
+    const bool active = l < nLayers && (cls == 0 ? l < L : l >= 1);
+    const int32_t r = active ? tr[o] : -1;
+    s[i] = r >= 0 ? ts[o] : npl;
+    e[i] = r >= 0 ? (int32_t)((cls == 0 ? l : l - 1) * B + r) : -1;
+  }
+}
+
 // per-slot cost (float q-units or raw double) and time gathered by edge id
 template <typename CT>
 __global__ void k_fill_costs(const int32_t* __restrict__ eid, const int32_t* __restrict__ s, int64_t nSlots,
@@ -878,6 +955,34 @@ hipError_t launch_layout_fill(const int32_t* src, const int32_t* trgt, int64_t E
   const int64_t nTot = (numNodes / npl + 1) * npl;
   hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nTot, 256)), dim3(256), 0, st, cnt, (uint32_t)npl, nTot, fd, nChunks,
                      width, s0, s1, e0, e1);
+  return hipGetLastError();
+}
+
+hipError_t launch_xlayout_count(const int32_t* EX, int64_t nEX, int64_t nVX, int prune, int32_t* cnt,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_xcount, dim3(nb(nEX + nVX, 256)), dim3(256), 0, st, EX, nEX, nVX, prune, cnt);
+  return hipGetLastError();
+}
+
+// template slots of one layer, ordered by (source, edge id) inside each node
+hipError_t launch_xlayout_fill(const int32_t* EX, int64_t nEX, int64_t nVX, int prune, int nChunks,
+                               const int32_t* width, const int32_t* cnt, int32_t* fill, int32_t* s0, int32_t* s1,
+                               int32_t* r0, int32_t* r1, hipStream_t st) {
+  hipLaunchKernelGGL(k_xfill, dim3(nb(nEX + nVX, 256)), dim3(256), 0, st, EX, nEX, nVX, prune, nChunks, width, fill,
+                     s0, s1, r0, r1);
+  FastDiv fd;
+  fd.init((uint32_t)nVX);
+  hipLaunchKernelGGL(k_sort_slots, dim3(nb(2 * nVX, 256)), dim3(256), 0, st, cnt, (uint32_t)nVX, nVX, fd, nChunks,
+                     width, s0, s1, r0, r1);
+  return hipGetLastError();
+}
+
+hipError_t launch_xlayout_replicate(const int32_t* ts, const int32_t* tr, int64_t lstride, int64_t nLayers,
+                                    int64_t total, int cls, int64_t B, int32_t npl, int32_t* s, int32_t* e,
+                                    hipStream_t st) {
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xreplicate, dim3((unsigned)std::min<int64_t>(nb(total, 256), 8192)), dim3(256), 0, st, ts, tr,
+                     lstride, nLayers, total, cls, B, npl, s, e);
   return hipGetLastError();
 }
 

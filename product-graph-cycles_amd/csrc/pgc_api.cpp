@@ -55,6 +55,11 @@ hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_pad_src(int32_t*, int64_t, int32_t, hipStream_t);
 hipError_t launch_fill_i32(int32_t*, int64_t, int32_t, hipStream_t);
 hipError_t launch_chunk_groups(const int32_t*, int64_t, int64_t, int, int32_t*, hipStream_t);
+hipError_t launch_xlayout_count(const int32_t*, int64_t, int64_t, int, int32_t*, hipStream_t);
+hipError_t launch_xlayout_fill(const int32_t*, int64_t, int64_t, int, int, const int32_t*, const int32_t*, int32_t*,
+                               int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t);
+hipError_t launch_xlayout_replicate(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int, int64_t, int32_t,
+                                    int32_t*, int32_t*, hipStream_t);
 // scratch.hip
 hipError_t scratch_setup(int device);
 void* scratch_ptr(int device);
@@ -467,8 +472,87 @@ int finish_graph_meta(pgc_graph* g) {
 
 constexpr int64_t kSlotPad = 32 * 64;
 
+// both classes get the solver's slot width: the smallest of 8, 16, 32 that
+// holds the largest in-degree, above 32 the largest in-degree rounded up to
+// a multiple of 8 (the runtime-width variant; beyond kMaxSlotWidth the graph
+// is rejected by check_width)
+int32_t slot_width(int32_t mx) { return mx <= 8 ? 8 : (mx <= 16 ? 16 : (mx <= 32 ? 32 : (mx + 7) / 8 * 8)); }
+
+// The layout of a generated non-conjugate product graph, in closed form from
+// its X graph (k_xcount/k_xfill/k_xreplicate): the same slots, in the same
+// order, as the generic build from SRC/TRGT below, at the cost of one pass
+// writing them. With three or more Y edges every edge into layer 0 comes from
+// the last layer (its source id exceeds npl) and moves to the closing layer;
+// with fewer the reference's strict wrap test (Q13) keeps some of them on
+// layer 0 or drops them, and the generic build handles that.
+int build_layout_closed(pgc_graph* g, DevAlloc& A) {
+  hipStream_t st = g->ctx->stream;
+  const int64_t npl = g->npl;
+  g->nLayers = g->L + 1;
+  g->nChunks = (int)((npl + 63) / 64);
+  const int nC = g->nChunks;
+  int32_t* d_fill = nullptr;
+  HIPCHK(A.get(&g->d_cnt, 2 * npl));
+  HIPCHK(A.get(&d_fill, 2 * npl));
+  HIPCHK(A.get(&g->d_width, 4));
+  HIPCHK(launch_fill_i32(g->d_cnt, 2 * npl, 0, st));
+  HIPCHK(launch_fill_i32(d_fill, 2 * npl, 0, st));
+  HIPCHK(launch_xlayout_count(g->d_ex, g->nEX, npl, g->prune, g->d_cnt, st));
+  HIPCHK(launch_layout_width(g->d_cnt, npl, g->d_width, st));
+  int32_t wn[4];
+  HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int32_t mx = std::max(wn[0], wn[1]);
+  const int32_t W = slot_width(mx);
+  g->width[0] = W;
+  g->width[1] = W;
+  g->max_in_degree = mx;
+  g->maxw = mx;
+  g->cls_deg[0] = wn[0];
+  g->cls_deg[1] = wn[1];
+  // each class has in-edges on L layers (intralayer 0..L-1, interlayer 1..L)
+  g->intra_slots_valid = (int64_t)wn[2] * g->L;
+  g->inter_slots_valid = (int64_t)wn[3] * g->L;
+  g->dropped = 0;
+  HIPCHK(hipMemcpyAsync(g->d_width, g->width, 2 * 4, hipMemcpyHostToDevice, st));
+  const int64_t tn = (int64_t)nC * W * 64;  // template slots per class
+  int32_t* d_t = nullptr;                   // [src0 | src1 | r0 | r1]
+  HIPCHK(A.get(&d_t, 4 * tn));
+  HIPCHK(launch_fill_i32(d_t, 4 * tn, -1, st));
+  HIPCHK(launch_xlayout_fill(g->d_ex, g->nEX, npl, g->prune, nC, g->d_width, g->d_cnt, d_fill, d_t, d_t + tn,
+                             d_t + 2 * tn, d_t + 3 * tn, st));
+  const int64_t B = (g->prune ? 1 : 2) * g->nEX + g->nVX;  // edges per layer
+  for (int c = 0; c < 2; ++c) {
+    const int64_t per_layer = tn;
+    const int64_t total = per_layer * g->nLayers;
+    if (total >= ((int64_t)1 << 31) - 1 - kSlotPad)
+      return fail(PGC_ERR_UNSUPPORTED, "in-edge layout exceeds 2^31 slots");
+    g->lstride[c] = (int32_t)per_layer;
+    g->nslots[c] = total;
+    HIPCHK(A.get(&g->d_s[c], total + kSlotPad));
+    HIPCHK(A.get(&g->d_e[c], total + kSlotPad));
+    HIPCHK(launch_xlayout_replicate(d_t + c * tn, d_t + (2 + c) * tn, per_layer, g->nLayers, total + kSlotPad, c, B,
+                                    (int32_t)npl, g->d_s[c], g->d_e[c], st));
+  }
+  if (W > 32) {
+    HIPCHK(A.get(&g->d_grp, 2 * (int64_t)nC));
+    HIPCHK(launch_fill_i32(g->d_grp, 2 * (int64_t)nC, 0, st));
+    HIPCHK(launch_chunk_groups(g->d_cnt, npl, npl, nC, g->d_grp, st));
+  }
+  g->src_inv[0] = 1;
+  g->src_inv[1] = 1;
+  A.put(d_t);
+  A.put(d_fill);
+  g->layout = true;
+  return PGC_OK;
+}
+
 int build_layout(pgc_graph* g, DevAlloc& A) {
   if (g->layout) return PGC_OK;
+  const char* generic = std::getenv("PGC_GENERIC_LAYOUT");
+  if (g->d_ex && !g->conjugate && !g->base && g->nEY >= 3 && g->L == g->nEY && g->npl == g->nVX &&
+      !(generic && generic[0] == '1'))
+    return build_layout_closed(g, A);
   hipStream_t st = g->ctx->stream;
   const int64_t npl = g->npl;
   g->nLayers = g->L + 1;
@@ -490,12 +574,8 @@ int build_layout(pgc_graph* g, DevAlloc& A) {
   int32_t wn[4];
   HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  // both classes get the solver's slot width: the smallest of 8, 16, 32 that
-  // holds the largest in-degree, above 32 the largest in-degree rounded up to
-  // a multiple of 8 (the runtime-width variant; beyond kMaxSlotWidth the
-  // graph is rejected by check_width)
   const int32_t mx = std::max(wn[0], wn[1]);
-  const int32_t W = mx <= 8 ? 8 : (mx <= 16 ? 16 : (mx <= 32 ? 32 : (mx + 7) / 8 * 8));
+  const int32_t W = slot_width(mx);
   g->width[0] = W;
   g->width[1] = W;
   g->max_in_degree = mx;
@@ -1264,6 +1344,23 @@ int pgc_graph_get_cost(pgc_graph* g, double* out) {
   HIPCHK(hipSetDevice(g->ctx->device));
   HIPCHK(hipMemcpyAsync(out, g->d_energy, g->E * g->ncols * 8, hipMemcpyDeviceToHost, g->ctx->stream));
   HIPCHK(hipStreamSynchronize(g->ctx->stream));
+  return PGC_OK;
+}
+
+int pgc_graph_get_layout(pgc_graph* g, int cls, int32_t* src, int32_t* eid, int64_t capacity, int64_t* n_slots,
+                         int32_t* width) {
+  if (!g || (cls != 0 && cls != 1) || !n_slots || !width) return fail(PGC_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(g->ctx->device));
+  int rc;
+  if ((rc = build_layout(g))) return rc;
+  *n_slots = g->nslots[cls];
+  *width = g->width[cls];
+  if (!src || !eid) return PGC_OK;
+  if (capacity < g->nslots[cls]) return fail(PGC_ERR_CAPACITY, "layout exceeds capacity");
+  hipStream_t st = g->ctx->stream;
+  HIPCHK(hipMemcpyAsync(src, g->d_s[cls], g->nslots[cls] * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(eid, g->d_e[cls], g->nslots[cls] * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return PGC_OK;
 }
 

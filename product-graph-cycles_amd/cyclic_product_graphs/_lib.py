@@ -74,7 +74,7 @@ EXPORTED = [
     "pgc_graph_get_cost", "pgc_graph_set_cost", "pgc_graph_destroy", "pgc_solve_ratio", "pgc_solve_min_cost",
     "pgc_solve_ratio_batch", "pgc_graph_set_feat_diff", "pgc_graphs_set_feat_diff", "pgc_graph_derive",
     "pgc_cycle_ratio_values", "pgc_cycle_ratio_grad", "pgc_solve_ratio_noalloc", "pgc_context_set_tuning",
-    "pgc_context_get_tuning",
+    "pgc_context_get_tuning", "pgc_graph_get_layout",
 ]
 
 _lib = None
@@ -124,6 +124,7 @@ def load():
     L.pgc_graph_derive.argtypes = [P, P, C.c_int, C.POINTER(P)]
     L.pgc_cycle_ratio_values.argtypes = [P, P, I64, P, P, P, P, P]
     L.pgc_cycle_ratio_grad.argtypes = [P, P, I64, P, P, P, P, P]
+    L.pgc_graph_get_layout.argtypes = [P, C.c_int, P, P, I64, C.POINTER(I64), C.POINTER(I32)]
     for name in EXPORTED:
         getattr(L, name)
     _lib = L
@@ -295,6 +296,18 @@ class Graph:
         out = np.empty((i["n_cost_cols"], i["n_edges"]), np.float64)
         check(load().pgc_graph_get_cost(self._h, ptr(out)))
         return out.T.copy()
+
+    def layout(self, cls):
+        """(src, eid, width) of the solver's in-edge slots of class cls (0
+        intralayer, 1 interlayer), built if needed: slot j of lane i of chunk c
+        on layer l at l*chunks*width*64 + (c*width + j)*64 + i."""
+        L = load()
+        n, w = C.c_int64(), C.c_int32()
+        check(L.pgc_graph_get_layout(self._h, int(cls), None, None, 0, C.byref(n), C.byref(w)))
+        src = np.empty(n.value, np.int32)
+        eid = np.empty(n.value, np.int32)
+        check(L.pgc_graph_get_layout(self._h, int(cls), ptr(src), ptr(eid), n.value, C.byref(n), C.byref(w)))
+        return src, eid, w.value
 
     def set_cost(self, cost):
         cost = np.asarray(cost, np.float64)
