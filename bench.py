@@ -298,7 +298,7 @@ def conjugate(steps=2):
     bps = info["n_edges"] * 8 + vx * 4
     achieved = bps * r["sweeps"] / (r["kernel_ms"] * 1e-3) / 1e9
     tr = _profile_traffic("solve_kernel_c2_traffic.json")
-    launches = r["branches"] + 1
+    launches = max(1, r["launches"])
     return {"workload": "conjugate, icosphere s=4 x 100-point curve, max_depth 2, regularised, vanilla/nomode",
             "tuning": tuned, "n_edges": info["n_edges"], "nodes_per_layer": info["nodes_per_layer"],
             "generate_s": sum(x[1] for x in runs) / steps, "lawler_solve_s": sum(x[2] for x in runs) / steps,
@@ -312,7 +312,7 @@ def conjugate(steps=2):
             "distance_table": "global memory (46080 nodes per layer exceed the LDS)"}
 
 
-def cap_case(reps=2):
+def cap_case(reps=2, implicit=False):
     """The C4 item that sets the batch makespan: pair 111 of the uneven batch
     (icosphere s=3, |VY| = 41, max_depth 1) with the feat_diff perturbation of
     tests/golden/bf_cap_case.npz, whose Bellman-Ford calls run to the sweep cap
@@ -324,13 +324,16 @@ def cap_case(reps=2):
     z = np.load(os.path.join(ROOT, "tests", "golden", "bf_cap_case.npz"))
     p, K = S.make_uneven_pairs(256, seed=0)[int(z["pair_index"])]
     ctx = _lib.default_context(0)
+    if implicit:
+        ctx = _lib.Context(0)
+        ctx.set_tuning({"implicit_costs": 1})
     for _ in range(reps):
         g = _lib.Graph.generate(ctx, p["VX"], p["EX"], p["VY"], p["EY"], z["feat_diff"].astype(np.float64), False,
                                 False, False, "vanilla", "")
         e, _, res = g.solve_ratio(K)
         d = res.as_dict()
         print(json.dumps({"workload": "C4 sweep-cap item (pair 111, npl 642, L 41, K 1)", "kernel_ms": res.kernel_ms,
-                          "sweeps": res.sweeps, "phase_ms": d["phase_ms"],
+                          "sweeps": res.sweeps, "phase_ms": d["phase_ms"], "implicit_costs": res.implicit_costs,
                           "same_cycle_as_fixture": bool(np.array_equal(e, z["edges"]))}), flush=True)
         g.close()
 
@@ -351,6 +354,8 @@ def main():
     ap.add_argument("--no-conj", action="store_true", help="skip the conjugate-graph measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a device copy of known size (for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--implicit", action="store_true",
+                    help="with --cap-case: recompute the slot costs from feat_diff (pgc_tuning.implicit_costs)")
     ap.add_argument("--cap-case", action="store_true",
                     help="run only the batch item that runs to the sweep cap (for profiling its solver launch)")
     ap.add_argument("--batch-only", action="store_true",
@@ -363,7 +368,7 @@ def main():
         copy_calibration()
         return
     if args.cap_case:
-        cap_case()
+        cap_case(implicit=args.implicit)
         return
     if args.batch_only:
         print(json.dumps(batched(args)))
@@ -429,7 +434,7 @@ def main():
     sweeps = sum(r[0][2].sweeps for r in results)
     kms = sum(r[0][2].kernel_ms for r in results)
     relax = sum(r[0][2].relaxations for r in results)
-    launches = sum(r[0][2].branches + 1 for r in results)  # upper-bound launch + one per branch
+    launches = sum(max(1, r[0][2].launches) for r in results)
     phases = {}
     for r in results:
         for k, v in r[0][2].as_dict()["phase_ms"].items():

@@ -57,6 +57,9 @@ typedef struct pgc_result {
   double phase_ms[5];         /* device time inside the solver kernels by phase: layer sweeps,
                                  closing-layer fold, wrap chase, cycle walks, cycle extraction */
   int32_t status;             /* PGC_OK, or this solve's own error (batched solves: per graph) */
+  int32_t launches;           /* solver kernel launches of the call (batched solves: 0 per graph) */
+  int32_t implicit_costs;     /* 1: the sweeps recomputed the slot costs from feat_diff
+                                 (pgc_tuning.implicit_costs), 0: they loaded them */
   int32_t reserved;
 } pgc_result;
 
@@ -79,6 +82,10 @@ typedef struct pgc_tuning {
   int32_t grid_workgroups;      /* n >= 1: graphs with the table in global memory are solved by n
                                    workgroups of one cooperative launch (at most what the device
                                    holds at once); other values: grid_chunks_per_wave decides */
+  int32_t implicit_costs;       /* 1: graphs generated in this context keep their feat_diff, and a
+                                   max_depth-1 solve whose sweep holds its rows, predecessors and
+                                   chain origins in LDS recomputes every slot's q-cost from it
+                                   instead of loading it (same results); other values: off */
 } pgc_tuning;
 
 const char* pgc_last_error(void);

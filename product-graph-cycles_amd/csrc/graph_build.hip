@@ -304,12 +304,15 @@ __global__ void k_select_step(SelectState* st, int shift) {
   }
 }
 
-// col <- min(round(s*col), 100*scaling) with s = scaling / (2 max(median, 0.001))
-__global__ void k_quantize(double* __restrict__ col, int64_t n, const SelectState* st, int even, double scaling) {
+// col <- min(round(s*col), 100*scaling) with s = scaling / (2 max(median, 0.001));
+// scale_out (if not null) <- s
+__global__ void k_quantize(double* __restrict__ col, int64_t n, const SelectState* st, int even, double scaling,
+                           double* __restrict__ scale_out) {
   const double med = even ? (st->value[0] + st->value[1]) / 2.0 : st->value[0];
   const double m = med > 0.001 ? med : 0.001;
   const double s = scaling / (2.0 * m);
   const double maxVal = 100 * scaling;
+  if (scale_out && blockIdx.x == 0 && threadIdx.x == 0) *scale_out = s;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double q = round(__dmul_rn(s, col[i]));
     if (q > maxVal) q = maxVal;
@@ -845,7 +848,29 @@ hipError_t launch_energy(const double* VX, const double* VY, const int32_t* rows
   return hipGetLastError();
 }
 
-hipError_t launch_quantize(double* col, int64_t n, void* selectState, hipStream_t st) {
+// out[y * nVX + x] = FD[x * nVY + y]: feat_diff by columns (one Y vertex's
+// values contiguous), through 32 x 32 tiles in LDS
+__global__ void k_transpose_fd(const double* __restrict__ FD, int64_t nVX, int64_t nVY, double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int64_t x0 = (int64_t)blockIdx.x * 32, y0 = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 8 rows per pass
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t x = x0 + r, y = y0 + tx;
+    if (x < nVX && y < nVY) tile[r][tx] = FD[x * nVY + y];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t y = y0 + r, x = x0 + tx;
+    if (x < nVX && y < nVY) out[y * nVX + x] = tile[tx][r];
+  }
+}
+
+hipError_t launch_transpose_fd(const double* FD, int64_t nVX, int64_t nVY, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_transpose_fd, dim3(nb(nVX, 32), nb(nVY, 32)), dim3(256), 0, st, FD, nVX, nVY, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_quantize(double* col, int64_t n, void* selectState, double* scale_out, hipStream_t st) {
   SelectState* S = (SelectState*)selectState;
   const int64_t k1 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
   const int64_t k2 = (n % 2 == 0) ? k1 + 1 : k1;
@@ -855,7 +880,7 @@ hipError_t launch_quantize(double* col, int64_t n, void* selectState, hipStream_
     hipLaunchKernelGGL(k_select_hist, dim3(g), dim3(256), 0, st, col, n, S, shift);
     hipLaunchKernelGGL(k_select_step, dim3(1), dim3(256), 0, st, S, shift);
   }
-  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, st, col, n, S, (int)(n % 2 == 0), 100000.0);
+  hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, st, col, n, S, (int)(n % 2 == 0), 100000.0, scale_out);
   return hipGetLastError();
 }
 

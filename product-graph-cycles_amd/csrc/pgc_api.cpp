@@ -35,7 +35,8 @@ hipError_t launch_product_space(const int32_t*, int64_t, const int32_t*, int64_t
                                 int32_t*, int64_t, hipStream_t);
 hipError_t launch_energy(const double*, const double*, const int32_t*, int64_t, const double*, int64_t, int, int,
                          double*, hipStream_t);
-hipError_t launch_quantize(double*, int64_t, void*, hipStream_t);
+hipError_t launch_quantize(double*, int64_t, void*, double*, hipStream_t);
+hipError_t launch_transpose_fd(const double*, int64_t, int64_t, double*, hipStream_t);
 hipError_t launch_quantize_batch(const QItem*, int, void*, void*, double*, hipStream_t);
 hipError_t launch_fill_costs_batch(const FItem*, int, hipStream_t);
 hipError_t launch_fill_i32_batch(const FillItem*, int, int32_t, hipStream_t);
@@ -226,7 +227,7 @@ struct DevAlloc {
 struct pgc_context {
   int device = 0;
   int num_cu = 0;
-  pgc_tuning tune = {-1, -1, -1, -1, -1, -1, -1, -1};  // launch choices (-1: the built-in rule)
+  pgc_tuning tune = {-1, -1, -1, -1, -1, -1, -1, -1, -1};  // launch choices (-1: the built-in rule)
   hipStream_t stream = nullptr;
   void* select_state = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -350,6 +351,12 @@ struct pgc_graph {
   bool released = false;  // destroyed by the caller while derived graphs live
   size_t cost_bytes[2] = {0, 0};
   size_t time_bytes[2] = {0, 0};
+  // implicit costs (tuning implicit_costs = 1): the graph's feat_diff by
+  // columns ([nVY][nVX] float64) and the scale of its last quantization
+  double* d_fdt = nullptr;
+  double* d_qscale = nullptr;
+  bool energy_fresh = false;  // the energy was computed from d_fdt's feat_diff and not quantized since
+  bool implicit_now = false;  // inside pgc_solve_ratio: this solve may recompute its costs
   ~pgc_graph() {
     (void)hipDeviceSynchronize();  // queued work may still use the graph's buffers
     js.release();
@@ -360,6 +367,8 @@ struct pgc_graph {
       d_vx = d_vy = nullptr;
       d_ex = d_ey = nullptr;
     }
+    if (d_fdt) cache_free(d_fdt);
+    if (d_qscale) cache_free(d_qscale);
     if (d_vx) cache_free(d_vx);
     if (d_vy) cache_free(d_vy);
     if (d_ex) cache_free(d_ex);
@@ -404,7 +413,7 @@ int pgc_context_create(int device, pgc_context** out) {
 int pgc_context_set_tuning(pgc_context* c, const pgc_tuning* t) {
   if (!c) return fail(PGC_ERR_ARG, "context is null");
   if (!t) {
-    c->tune = pgc_tuning{-1, -1, -1, -1, -1, -1, -1, -1};
+    c->tune = pgc_tuning{-1, -1, -1, -1, -1, -1, -1, -1, -1};
     return PGC_OK;
   }
   if (t->grid_chunks_per_wave < -1 || t->grid_chunks_per_wave > 64)
@@ -414,6 +423,7 @@ int pgc_context_set_tuning(pgc_context* c, const pgc_tuning* t) {
   if (t->global_table < -1 || t->global_table > 1) return fail(PGC_ERR_ARG, "global_table must be -1, 0 or 1");
   if (t->grid_workgroups < -1 || t->grid_workgroups > 256)
     return fail(PGC_ERR_ARG, "grid_workgroups must be -1, 0 or 1..256");
+  if (t->implicit_costs < -1 || t->implicit_costs > 1) return fail(PGC_ERR_ARG, "implicit_costs must be -1, 0 or 1");
   c->tune = *t;
   return PGC_OK;
 }
@@ -732,14 +742,35 @@ bool org_fits_lds(const pgc_graph* g, int K) {
   return b + (size_t)g->npl * 8 <= 160 * 1024 - 1024;
 }
 
-// dynamic LDS of a job's launch
-size_t job_lds_bytes(const pgc_graph* g, int K) {
+size_t job_lds_base(const pgc_graph* g, int K) {
   if (kernel_maxc(g, K) == 0) return 0;
   size_t b = solver_lds_bytes((int)g->npl, K);
   if (pred_fits_lds(g, K)) b += (size_t)(g->L + 1) * K * g->npl * 4;
   if (wrap_fits_lds(g, K)) b += (size_t)g->npl * 8;
   if (org_fits_lds(g, K)) b += (size_t)g->npl * 8;
   return b;
+}
+
+// LDS of the implicit-cost sweep's two staged feat_diff columns (after the
+// origin rows, 8-byte aligned)
+size_t implicit_lds_bytes(const pgc_graph* g) { return 4 + (size_t)(g->npl + 1) * 16; }
+
+// A ratio solve of g at depth K recomputes its slot costs from feat_diff: the
+// context asked for it, the energy in the graph is the fresh vanilla energy
+// of d_fdt (quantized once, by this solve), and the job runs the max_depth-1
+// sweep that holds its rows, predecessors and chain origins in LDS, with
+// layer-invariant sources in registers (generated graphs: closed-form layout).
+bool implicit_applies(const pgc_graph* g, int K) {
+  if (!g->implicit_now || !g->d_fdt || K != 1 || g->conjugate || g->ncols != 1 || g->line_integral) return false;
+  const pgc_graph* b = g->base ? g->base : g;  // a derived graph shares its base's Y edges
+  if (!b->d_ey || b->nEY != g->L || !g->src_inv[1] || g->ctx->tune.cached_sources == 0) return false;
+  if (kernel_maxc(g, K) != 1 || g->maxw > 8 || !org_fits_lds(g, K) || g->npl > 4 * PGC_BLOCK) return false;
+  return job_lds_base(g, K) + implicit_lds_bytes(g) <= 160 * 1024 - 1024;
+}
+
+// dynamic LDS of a job's launch
+size_t job_lds_bytes(const pgc_graph* g, int K) {
+  return job_lds_base(g, K) + (implicit_applies(g, K) ? implicit_lds_bytes(g) : 0);
 }
 
 // Solver scratch for depth K. min_cost: with the buffers only the min-cost
@@ -889,6 +920,11 @@ SolverJob make_job(pgc_graph* g, int K, int mode, uint32_t flags) {
   // validation switch for the host fallback of batched solves)
   if (const char* vs = std::getenv("PGC_VALID_STORE")) j.vc_cap = std::min(j.vc_cap, std::max(0, std::atoi(vs)));
   j.vc_stride = s.vc_stride;
+  if (mode != MODE_MIN_COST && implicit_applies(g, K)) {
+    j.icol = g->d_fdt;
+    j.iey = (g->base ? g->base : g)->d_ey;
+    j.iscale = g->d_qscale;
+  }
   return j;
 }
 
@@ -901,6 +937,7 @@ struct Timer {
 
 // one solver launch: upload job, run, read result (kernel time accumulated)
 static thread_local double g_phase_ms[5];
+static thread_local int32_t g_launches;  // solver launches of the current call
 
 // Workgroups of the device-wide solve of a job (0: one workgroup). Graphs
 // whose table lives in global memory and that have at least 128 chunks of 64
@@ -956,6 +993,7 @@ int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, doub
   kms += ms;
   const int64_t ticks[5] = {out.t_sweep, out.t_fold, out.t_chase, out.t_walk, out.t_extract};
   for (int i = 0; i < 5; ++i) g_phase_ms[i] += (double)ticks[i] * 1e-5;  // 100 MHz wall clock
+  g_launches++;
   return PGC_OK;
 }
 
@@ -1178,6 +1216,22 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   return PGC_OK;
 }
 
+// Implicit costs: a non-conjugate graph with the vanilla energy (one column,
+// no length term) generated in a context with implicit_costs = 1 keeps its
+// feat_diff by columns (fd: [nVX x nVY] float64 on the device).
+int keep_feat_diff(pgc_graph* g, const double* fd) {
+  const pgc_graph* b = g->base ? g->base : g;
+  if (g->ctx->tune.implicit_costs != 1 || b->conjugate || g->ncols != 1 || b->line_integral) {
+    if (g->d_fdt) cache_free(g->d_fdt);
+    g->d_fdt = nullptr;
+    return PGC_OK;
+  }
+  if (!g->d_fdt) HIPCHK(dalloc(&g->d_fdt, b->nVX * b->nVY));
+  if (!g->d_qscale) HIPCHK(dalloc(&g->d_qscale, 1));
+  HIPCHK(launch_transpose_fd(fd, b->nVX, b->nVY, g->d_fdt, g->ctx->stream));
+  return PGC_OK;
+}
+
 int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_t* dEX, int64_t nEX,
                   const double* dVY, int64_t nVY, const int32_t* dEY, int64_t nEY, const double* dFD, int conjugate,
                   int regularising, int prune, const char* cost_mode, const char* time_mode, const double* dNX,
@@ -1235,6 +1289,9 @@ int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   // numNodes / numLayers as the solver derives them (run(), :870-871)
   g->numNodes = nEY * nVXe;
   g->L = nEY;
+  int rc = keep_feat_diff(g.get(), dFD);
+  if (rc) return rc;
+  g->energy_fresh = true;
   if (prune) {
     // branchGraph[x] = out-neighbour vertex ids (product_space.cpp:122-131)
     std::vector<std::vector<int64_t>> bg(nVX);
@@ -1370,6 +1427,7 @@ int pgc_graph_set_cost(pgc_graph* g, const double* cost) {
   HIPCHK(hipMemcpyAsync(g->d_energy, cost, g->E * g->ncols * 8, hipMemcpyHostToDevice, g->ctx->stream));
   HIPCHK(hipStreamSynchronize(g->ctx->stream));
   g->cost_kind = 0;
+  g->energy_fresh = false;
   return PGC_OK;
 }
 
@@ -1404,6 +1462,9 @@ static int energy_from(pgc_graph* g, const double* feat_diff, int on_device) {
   HIPCHK(launch_energy(b->d_vx, b->d_vy, b->d_rows, b->E, fd, b->nVY, b->line_integral, b->length_mode,
                        g->d_energy, st));
   g->cost_kind = 0;
+  int rc = keep_feat_diff(g, fd);
+  if (rc) return rc;
+  g->energy_fresh = true;
   if (tmp) {
     HIPCHK(hipStreamSynchronize(st));
     cache_free(tmp);
@@ -1490,6 +1551,7 @@ int ratio_bnb_host(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_
                    int64_t capacity, pgc_result* res, const Timer& wall) {
   int rc;
   for (double& x : g_phase_ms) x = 0;
+  g_launches = 0;
   hipStream_t st = ctx->stream;
   const int64_t npl = g->npl;
   JobScratch& s = g->js;
@@ -1622,6 +1684,81 @@ int ratio_bnb_host(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_
   res->kernel_ms = kms;
   res->total_ms = wall.ms();
   for (int i = 0; i < 5; ++i) res->phase_ms[i] = g_phase_ms[i];
+  res->launches = g_launches;
+  res->implicit_costs = implicit_applies(g, K) ? 1 : 0;
+  res->status = PGC_OK;
+  return PGC_OK;
+}
+
+// The same solve as ratio_bnb_host with the branch and bound on the device:
+// upper bound, branch and bound, Lawler iterations and output in one launch of
+// the one-workgroup solver (Solver::full_ratio), two synchronizations in all.
+// The store of valid-branch cycles has one slot per branch id below npl (the
+// heap drops higher ids, Q9), so every valid branch fits, unless the graph's
+// store was sized before by a batch (64 slots) or PGC_VALID_STORE caps it:
+// then an overflow, or the branch cap, sets `redo`, and the caller runs the
+// host branch and bound on the same quantized costs (same result).
+int ratio_bnb_device(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_t* edge_ids, int32_t* rows,
+                     int64_t capacity, pgc_result* res, const Timer& wall, bool& redo) {
+  redo = false;
+  int rc;
+  hipStream_t st = ctx->stream;
+  DevAlloc A;
+  if ((rc = ensure_bnb(g, A, (int)g->npl))) return rc;
+  JobScratch& s = g->js;
+  const int64_t outCap = std::max<int64_t>(0, std::min<int64_t>(capacity, s.cyc_cap));
+  int32_t *d_out = nullptr, *d_rows = nullptr;
+  HIPCHK(dalloc(&d_out, std::max<int64_t>(outCap, 1)));
+  SolverJob job = make_job(g, K, MODE_FULL_RATIO, flags | FLAG_FRESH_JOB);
+  job.res = s.res;
+  job.out_eid = d_out;
+  job.out_cap = (int32_t)outCap;
+  if (!pred_fits_lds(g, K)) HIPCHK(launch_fill_i32(s.pred, s.predN, -1, st));
+  HIPCHK(hipMemcpyAsync(s.djob, &job, sizeof(job), hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  HIPCHK(launch_solver(s.djob, 1, 1, nullptr, kernel_maxc(g, K), g->maxw, false, g->d_time[0] != nullptr, false,
+                       job_lds_bytes(g, K), st));
+  HIPCHK(hipEventRecord(ctx->ev1, st));
+  JobResult r;
+  HIPCHK(hipMemcpyAsync(&r, s.res, sizeof(r), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float kms = 0;
+  HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+  if (r.error) {
+    cache_free(d_out);
+    if (r.error == ERR_VALID_STORE || r.error == ERR_BRANCH_CAP) {
+      redo = true;
+      return PGC_OK;
+    }
+    if (r.error == ERR_CYCLE_TOO_LONG && r.out_len > outCap)
+      return fail(PGC_ERR_CAPACITY, "output capacity too small: need " + std::to_string(r.out_len));
+    return fail(PGC_ERR_SOLVER, "solver error " + std::to_string(r.error));
+  }
+  const int64_t m = r.out_len;
+  if (edge_ids && m) HIPCHK(hipMemcpyAsync(edge_ids, d_out, m * 4, hipMemcpyDeviceToHost, st));
+  if (rows && m) {
+    HIPCHK(dalloc(&d_rows, m * g->ps_cols));
+    HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, d_out, m, d_rows, st));
+    HIPCHK(hipMemcpyAsync(rows, d_rows, m * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  cache_free(d_out);
+  if (d_rows) cache_free(d_rows);
+  std::memset(res, 0, sizeof(*res));
+  res->passes_layers_once = 1;
+  res->objective = (float)(r.out_cs / r.out_ts);
+  res->n_edges = m;
+  res->sweeps = r.sweeps;
+  res->bf_calls = r.bf_calls;
+  res->lawler_iters = r.lawler_iters;
+  res->branches = r.branches;
+  res->relaxations = relax_per_sweep(g, K) * (double)r.sweeps;
+  res->kernel_ms = kms;
+  res->total_ms = wall.ms();
+  const int64_t ticks[5] = {r.t_sweep, r.t_fold, r.t_chase, r.t_walk, r.t_extract};
+  for (int k = 0; k < 5; ++k) res->phase_ms[k] = (double)ticks[k] * 1e-5;  // 100 MHz wall clock
+  res->launches = 1;
+  res->implicit_costs = implicit_applies(g, K) ? 1 : 0;
   res->status = PGC_OK;
   return PGC_OK;
 }
@@ -1652,9 +1789,26 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   hipStream_t st = ctx->stream;
   int rc = prepare_ratio(g, K);
   if (rc) return rc;
+  // a fresh energy quantized once can be recomputed from feat_diff by the sweep
+  struct Implicit {
+    pgc_graph* g;
+    ~Implicit() { g->implicit_now = false; }
+  } implicit_scope{g};
+  g->implicit_now = g->energy_fresh && g->d_fdt;
+  g->energy_fresh = false;
   // robustFloatToIntConversion of every column, in place (:875-882)
-  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state, st));
+  for (int c = 0; c < g->ncols; ++c)
+    HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state,
+                           g->implicit_now ? g->d_qscale : nullptr, st));
   if ((rc = fill_costs(g, 1))) return rc;
+  // the device branch and bound wherever the one-workgroup solver runs the
+  // job (the device-wide solver has no device branch and bound);
+  // PGC_HOST_BNB=1 selects the host driver everywhere
+  const char* hb = std::getenv("PGC_HOST_BNB");
+  if (grid_blocks(g, K, false) == 0 && !(hb && hb[0] == '1')) {
+    bool redo = false;
+    if ((rc = ratio_bnb_device(ctx, g, K, flags, edge_ids, rows, capacity, res, wall, redo)) || !redo) return rc;
+  }
   return ratio_bnb_host(ctx, g, K, flags, edge_ids, rows, capacity, res, wall);
 }
 
@@ -1788,7 +1942,8 @@ int pgc_solve_ratio_noalloc(pgc_context* ctx, pgc_graph* g, int K, uint32_t flag
   sel = selb;
   // robustFloatToIntConversion of every column, in place (:875-882), then
   // the slot costs
-  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, sel, st));
+  g->energy_fresh = false;
+  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, sel, nullptr, st));
   for (int c = 0; c < 2; ++c)
     HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, g->ncols, false, g->d_cost[c],
                              g->ncols > 1 ? g->d_time[c] : nullptr, st));
@@ -1834,6 +1989,7 @@ int pgc_solve_ratio_noalloc(pgc_context* ctx, pgc_graph* g, int K, uint32_t flag
   res->total_ms = wall.ms();
   const int64_t ticks[5] = {r.t_sweep, r.t_fold, r.t_chase, r.t_walk, r.t_extract};
   for (int k = 0; k < 5; ++k) res->phase_ms[k] = (double)ticks[k] * 1e-5;
+  res->launches = 1;
   res->status = PGC_OK;
   return PGC_OK;
 }
@@ -1884,7 +2040,13 @@ int pgc_graphs_set_feat_diff(pgc_graph* const* graphs, const void* const* feat_d
   const int64_t chunk = 65535;
   for (int64_t a = 0; a < n; a += chunk)
     HIPCHK(launch_energy_batch(ctx->d_eitems + a, (int)std::min<int64_t>(chunk, n - a), st));
-  for (int64_t i = 0; i < n; ++i) graphs[i]->cost_kind = 0;
+  // the energies now come from these feat_diffs: a kept feat_diff is stale
+  for (int64_t i = 0; i < n; ++i) {
+    graphs[i]->cost_kind = 0;
+    graphs[i]->energy_fresh = false;
+    if (graphs[i]->d_fdt) cache_free(graphs[i]->d_fdt);
+    graphs[i]->d_fdt = nullptr;
+  }
   return PGC_OK;
 }
 
@@ -1930,6 +2092,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
   for (int64_t i = 0; i < n; ++i) {
     pgc_graph* g = graphs[i];
     const int K = max_depth[i];
+    g->energy_fresh = false;  // quantized in place below
     for (int c = 0; c < g->ncols; ++c) qitems.push_back({g->d_energy + (int64_t)c * g->E, g->E, 0, 0});
     for (int c = 0; c < 2; ++c)
       fitems.push_back({g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, (float*)g->d_cost[c],
@@ -2249,6 +2412,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
                        int64_t capacity, pgc_result* res) {
   Timer wall;
   for (double& x : g_phase_ms) x = 0;
+  g_launches = 0;
   if (!ctx || !g || !res) return fail(PGC_ERR_ARG, "invalid arguments");
   if (g->ctx != ctx) return fail(PGC_ERR_ARG, "graph belongs to another context");
   int rc = check_solvable(g, K);
@@ -2525,6 +2689,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   res->kernel_ms = kms;
   res->total_ms = wall.ms();
   for (int i = 0; i < 5; ++i) res->phase_ms[i] = g_phase_ms[i];
+  res->launches = g_launches;
   res->status = PGC_OK;
   return PGC_OK;
 }

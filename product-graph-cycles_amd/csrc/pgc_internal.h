@@ -162,6 +162,12 @@ struct SolverJob {
                           // after the wrap map, kept by the sweep, replace the wrap chase
   double* dtab;           // [(2K+1)*(npl+1)] distance table in global memory (graphs beyond the LDS)
   GridState* grid;        // barrier and broadcast state of a device-wide solve (cooperative launch)
+  // implicit costs (max_depth 1, org_lds): the sweep recomputes every slot's
+  // q-cost from feat_diff instead of loading it; nullptr: the slot costs
+  const double* icol;     // [nVY][npl] feat_diff (float64), column y of Y vertex y contiguous
+  const int32_t* iey;     // [L][2] Y edges: layer l's interlayer edges go from column iey[2(l-1)]
+                          // (sources) to column iey[2(l-1)+1] (targets)
+  const double* iscale;   // [1] the quantization scale s of the graph's energy
 };
 
 // One cycle of the raw-unit ratio value / gradient kernels.

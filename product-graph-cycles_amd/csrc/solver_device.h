@@ -565,6 +565,19 @@ struct Solver {
       r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
                                                        predLds ? 1 : 0, bor.prev0, bor.prev1, bor.odd ? 1 : 0, ratio,
                                                        ub);
+    else if (!MINCOST && !HAS_TIME && orgOn && J.icol) {
+      // implicit costs: the staging columns follow the origin rows (8-byte aligned)
+      PGC_LDS int32_t* st32 = OrgL + 2 * npl;
+      st32 += ((uint32_t)(uintptr_t)st32 & 4u) ? 1 : 0;
+      double* stage = (double*)(PGC_LDS double*)st32;
+      if (J.inter.deg <= W - 1 && !(J.flags & FLAG_FULL_SLOTS))
+        r = sweep_layers_k1_implicit<CT, MAXC, W, W - 1>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl,
+                                                         bor.prev0, bor.prev1, bor.odd ? 1 : 0, ratio, ub,
+                                                         (int32_t*)OrgL, stage);
+      else
+        r = sweep_layers_k1_implicit<CT, MAXC, W, W>((const SolverJob*)&J, D, Dmin, &sh, (int32_t*)Pl, bor.prev0,
+                                                     bor.prev1, bor.odd ? 1 : 0, ratio, ub, (int32_t*)OrgL, stage);
+    }
     else if (J.inter.deg <= W - 1 && !(J.flags & FLAG_FULL_SLOTS))  // the last interlayer slot is padding on every node
       r = sweep_layers<CT, HAS_TIME, MINCOST, MAXC, W, true, W - 1>((const SolverJob*)&J, D, Dmin, &sh,
                                                                     (int32_t*)Pl, predLds ? 1 : 0, bor.prev0,

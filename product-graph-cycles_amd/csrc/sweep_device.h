@@ -373,6 +373,102 @@ struct Sweep {
     return 1;
   }
 
+  // run_layers_k1_inv with implicit costs: a slot's q-cost is recomputed
+  // from feat_diff instead of loaded. The interlayer edge (l-1, x0) -> (l, x1)
+  // of a generated non-conjugate graph (vanilla cost, no time column, no
+  // length term) has the energy 0.5 (FD[x0, ys] + FD[x1, yd]) with (ys, yd)
+  // the Y edge of layer l-1 (deformationEnergy.cpp:66-69), and its slot cost
+  // is that energy quantized with the graph's scale s, min(round(s e), 1e7),
+  // stored as float (cost_time_ratio_solver.cpp:709-721, utils.hpp:196-209):
+  // the same operations in the same order here give the same bits. Column ys
+  // of the next layer is staged in LDS (stage[(l+1) & 1], entry npl = 0 for
+  // the padding sentinel) while the current layer is relaxed; FD[x1, yd] of
+  // the owned vertex is a register.
+  __device__ __forceinline__ float iq(double a, double b, double s) const {
+    const double e = __dmul_rn(0.5, __dadd_rn(a, b));
+    double q = round(__dmul_rn(s, e));
+    if (q > 10000000.0) q = 10000000.0;
+    return (float)q;
+  }
+  __device__ __forceinline__ int run_layers_k1_implicit(PGC_LDS double* stage, double ratio, double rt, double ub,
+                                                        double (&dmin_r)[MAXC]) {
+    const PGC_GLOBAL double* __restrict__ col = gp(J->icol);
+    const PGC_GLOBAL int32_t* __restrict__ ey = gp(J->iey);
+    const double sc = uni(*gp(J->iscale));
+    int s[MAXC][W];
+    {
+      const PGC_GLOBAL int* __restrict__ src = gp(Sx.src);
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) {
+        const int ci = chunk_of(i);
+        const int base = Sx.lstride + lane + (ci < nC ? ci * W * 64 : 0);
+#pragma unroll
+        for (int j = 0; j < WE; ++j) s[i][j] = src[base + j * 64];
+      }
+    }
+    // owned vertices (a pass past the last chunk reads vertex 0, unused)
+    int vo[MAXC];
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) vo[i] = max(vert(i), 0);
+    // stage layer 1's source column and its targets' values
+    double bt[MAXC];
+    {
+      const int ys = uni(ey[0]), yd = uni(ey[1]);
+      const PGC_GLOBAL double* cs = col + (int64_t)ys * npl;
+      const PGC_GLOBAL double* cd = col + (int64_t)yd * npl;
+      for (int x = threadIdx.x; x <= npl; x += PGC_BLOCK) {
+        stage[ds + x] = x < npl ? cs[x] : 0.0;
+        stage[x] = 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) bt[i] = cd[vo[i]];
+    }
+    __syncthreads();
+    for (int layer = 1; layer <= L; ++layer) {
+      // next layer's column and target values (in flight during this layer)
+      const int ln = min(layer + 1, L);
+      const int ys = uni(ey[2 * (ln - 1)]), yd = uni(ey[2 * (ln - 1) + 1]);
+      const PGC_GLOBAL double* cs = col + (int64_t)ys * npl;
+      const PGC_GLOBAL double* cd = col + (int64_t)yd * npl;
+      double nxt[4];  // npl <= 4 * PGC_BLOCK (checked by the host)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int x = threadIdx.x + k * PGC_BLOCK;
+        nxt[k] = x < npl ? cs[x] : 0.0;
+      }
+      double bn[MAXC];
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) bn[i] = cd[vo[i]];
+      // this layer's costs from the staged column
+      const PGC_LDS double* A = stage + (layer & 1) * ds;
+      SlotsCT b;
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) {
+        double a[W];
+#pragma unroll
+        for (int j = 0; j < WE; ++j) a[j] = A[s[i][j]];
+#pragma unroll
+        for (int j = 0; j < WE; ++j) b.c[i][j] = (CT)iq(a[j], bt[i], sc);
+      }
+      const bool imp = compute_inv(s, b, layer, ratio, rt, ub, dmin_r);
+      // the next layer's buffer was last read before the previous barrier
+      PGC_LDS double* An = stage + ((layer + 1) & 1) * ds;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int x = threadIdx.x + k * PGC_BLOCK;
+        if (x < npl) An[x] = nxt[k];
+      }
+#pragma unroll
+      for (int i = 0; i < MAXC; ++i) bt[i] = bn[i];
+      if (layer == L - 1) {
+        if (!bor(imp)) return 0;
+      } else {
+        __syncthreads();
+      }
+    }
+    return 1;
+  }
+
   __device__ __forceinline__ int run_layers_k1(Slots& A, Slots& B, double ratio, double rt, double ub,
                                                double (&dmin_r)[MAXC]) {
     const int nb = L;
@@ -516,6 +612,48 @@ __device__ __noinline__ SweepState sweep_layers(const SolverJob* Jp, double* Dg,
   w.odd = uni(odd);
   SweepState r;
   r.complete = w.run_layers(uni(ratio), uni(ub));
+  r.p0 = w.p0;
+  r.p1 = w.p1;
+  r.odd = w.odd;
+  return r;
+}
+
+// The K == 1 sweep with implicit costs (run_layers_k1_implicit), apart from
+// sweep_layers so that the materialized-cost sweep keeps its own register
+// allocation. stageLg: 2 (npl + 1) doubles of LDS after the origin rows.
+template <typename CT, int MAXC, int W, int WE>
+__device__ __noinline__ SweepState sweep_layers_k1_implicit(const SolverJob* Jp, double* Dg, double* Dming,
+                                                            Shared* shg, int32_t* predLg, int p0, int p1, int odd,
+                                                            double ratio, double ub, int32_t* orgLg, double* stageLg) {
+  Sweep<CT, false, false, MAXC, W, true, WE> w;
+  w.orgOn = 1;
+  w.orgL = (PGC_LDS int32_t*)uni(orgLg);
+  w.J = (const PGC_GLOBAL SolverJob*)uni(Jp);
+  w.D = (PGC_LDS double*)uni(Dg);
+  w.Dmin = (PGC_LDS double*)uni(Dming);
+  w.sh = (PGC_LDS Shared*)uni(shg);
+  w.pred = (PGC_GLOBAL int32_t*)uni(w.J->pred);
+  w.predL = (PGC_LDS int32_t*)uni(predLg);
+  w.predLds = 1;
+  w.Sa = uni_class(w.J->intra);
+  w.Sx = uni_class(w.J->inter);
+  w.npl = uni(w.J->npl);
+  w.ds = w.npl + 1;
+  w.K = 1;
+  w.L = uni(w.J->L);
+  w.nC = uni(w.J->nChunks);
+  w.wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  w.lane = threadIdx.x & 63;
+  w.nw = PGC_BLOCK / PGC_WAVE;
+  w.p0 = uni(p0);
+  w.p1 = uni(p1);
+  w.odd = uni(odd);
+  SweepState r;
+  const double ra = uni(ratio);
+  double dmin_r[MAXC];
+  r.complete = w.L > 0 ? w.run_layers_k1_implicit((PGC_LDS double*)uni(stageLg), ra,
+                                                  __dmul_rn(ra, uni(w.J->time_const)), uni(ub), dmin_r)
+                       : 1;
   r.p0 = w.p0;
   r.p1 = w.p1;
   r.odd = w.odd;

@@ -38,6 +38,8 @@ class pgc_result(C.Structure):
         ("total_ms", C.c_double),
         ("phase_ms", C.c_double * 5),
         ("status", C.c_int32),
+        ("launches", C.c_int32),
+        ("implicit_costs", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -59,9 +61,10 @@ class pgc_tuning(C.Structure):
         ("short_slots", C.c_int32),
         ("global_table", C.c_int32),
         ("grid_workgroups", C.c_int32),
+        ("implicit_costs", C.c_int32),
     ]
     FIELDS = ("grid_chunks_per_wave", "pred_in_lds", "wrap_in_lds", "origins", "cached_sources", "short_slots",
-              "global_table", "grid_workgroups")
+              "global_table", "grid_workgroups", "implicit_costs")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}

@@ -2,9 +2,12 @@
 
 tests/branching_cases.py holds 29 seeded tiny rings on which the Lawler solve
 splits branches (pinned against brute force in tests/test_oracle.py). Each is
-solved through the C-ABI by the host branch and bound (pgc_solve_ratio) and by
-the device-resident one (pgc_solve_ratio_batch), under both candidate pick
-rules, and compared with the oracle. Bar: edge ids and product-space rows
+solved through the C-ABI by pgc_solve_ratio, whose branch and bound runs on the
+device (one launch) or, with PGC_HOST_BNB=1, on the host around the device
+Lawler launches, and, with the device store of valid-branch cycles capped
+(PGC_VALID_STORE), on the device until the store overflows and then on the
+host; and by pgc_solve_ratio_batch (device branch and bound in the persistent
+launch), under both candidate pick rules, and compared with the oracle. Bar: edge ids and product-space rows
 bit-exact, the fp32 objective equal (1e-5 relative written as the tolerance,
 observed difference 0), and the Bellman-Ford call, sweep, Lawler iteration and
 branch counters equal.
@@ -50,12 +53,17 @@ def _same(eids, rows, res, orow, oe, oobj, stats):
         (stats[0], stats[1], stats[2], stats[3])
 
 
+@pytest.mark.parametrize("driver", ["device", "host", "device-store-0", "device-store-1"])
 @pytest.mark.parametrize("name", list(RULES))
-def test_single_solves_branching(name):
+def test_single_solves_branching(name, driver, monkeypatch):
+    if driver == "host":
+        monkeypatch.setenv("PGC_HOST_BNB", "1")
+    elif driver.startswith("device-store-"):
+        monkeypatch.setenv("PGC_VALID_STORE", driver[-1])
     ctx = _lib.default_context(0)
     branched = 0
     for seed, nx, ny, K in BRANCHING:
-        print("case", name, seed, nx, ny, K, flush=True)
+        print("case", name, driver, seed, nx, ny, K, flush=True)
         p = _pair(seed, nx, ny)
         orow, oe, oobj, stats = _oracle(p, K, RULES[name] != 0)
         g = _graph(ctx, p)

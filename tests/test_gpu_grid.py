@@ -16,6 +16,7 @@ make cooperative launches while this one holds the device. The full-size
 conjugate configuration (46,080 nodes per layer, 45 workgroups) is in
 tests/test_gpu_configs.py.
 """
+import inspect
 import itertools
 
 import numpy as np
@@ -88,9 +89,21 @@ def test_small_cases_device_wide_forced():
                 fn = getattr(mod, name)
                 if not (name.startswith("test_") and callable(fn) and any(k in name for k in SELECT)):
                     continue
+                takes_mp = "monkeypatch" in inspect.signature(fn).parameters
                 for kw in _cases(fn):
+                    # device-wide solves always take the host branch and
+                    # bound: the other driver settings would repeat the case
+                    if kw.get("driver", "device") != "device":
+                        continue
                     print("case forced grid", name, kw, flush=True)
-                    fn(**kw)
+                    if takes_mp:
+                        mp = pytest.MonkeyPatch()
+                        try:
+                            fn(monkeypatch=mp, **kw)
+                        finally:
+                            mp.undo()
+                    else:
+                        fn(**kw)
                     n += 1
     finally:
         ctx.set_tuning(saved)
