@@ -218,63 +218,41 @@ struct SelectState {
   double value[2];
 };
 
-__global__ void k_select_init(SelectState* st, int64_t k0, int64_t k1) {
-  const int t = threadIdx.x;
-  if (t < 256) {
-    st->hist[0][t] = 0;
-    st->hist[1][t] = 0;
-  }
-  if (t == 0) {
-    st->prefix[0] = st->prefix[1] = 0;
-    st->mask[0] = st->mask[1] = 0;
-    st->k[0] = k0;
-    st->k[1] = k1;
+// Single-column select: eight passes of 8-bit digits, most significant
+// first. Pass p histograms digit p of the keys that match each rank's prefix
+// (digits 0..p-1); every block of pass p first picks digit p-1 itself from
+// the previous pass's histograms (a 256-bin scan in LDS), so no launch sits
+// between two passes; block 0 records the state for the next pass (two
+// buffers: pass p reads buffer (p-1) & 1 and writes buffer p & 1).
+struct Select1State {
+  uint64_t prefix[2][2];  // [buffer][rank]
+  uint64_t mask[2][2];
+  int64_t k[2][2];        // rank inside the keys that match the prefix
+  uint32_t hist[8][2][256];  // [pass][rank][bin]
+};
+
+__global__ void k_select_init(Select1State* st, int64_t k0, int64_t k1) {
+  for (int i = threadIdx.x; i < 8 * 2 * 256; i += blockDim.x) (&st->hist[0][0][0])[i] = 0;
+  if (threadIdx.x == 0) {
+    st->prefix[0][0] = st->prefix[0][1] = 0;
+    st->mask[0][0] = st->mask[0][1] = 0;
+    st->k[0][0] = k0;
+    st->k[0][1] = k1;
   }
 }
 
-// Histograms of the next digit of the keys that match each rank's prefix.
-// In the first passes most keys share their high digits: when every active
-// lane of a wave has the same digit, the wave adds its two match counts with
-// one LDS atomic each instead of one per lane (the same totals).
-__global__ void k_select_hist(const double* __restrict__ x, int64_t n, SelectState* st, int shift) {
-  __shared__ uint32_t h[2][256];
-  h[0][threadIdx.x] = 0;
-  h[1][threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t p0 = st->prefix[0], m0 = st->mask[0], p1 = st->prefix[1], m1 = st->mask[1];
-  const int lane = threadIdx.x & 63;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t kk = dkey(x[i]);
-    const uint32_t b = (uint32_t)(kk >> shift) & 255u;
-    const bool in0 = (kk & m0) == p0, in1 = (kk & m1) == p1;
-    const uint64_t act = __ballot(1);
-    const uint32_t bf = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
-    if (__ballot(b == bf) == act) {
-      const uint64_t v0 = __ballot(in0), v1 = __ballot(in1);
-      if (lane == __ffsll((unsigned long long)act) - 1) {
-        if (v0) atomicAdd(&h[0][bf], (uint32_t)__popcll(v0));
-        if (v1) atomicAdd(&h[1][bf], (uint32_t)__popcll(v1));
-      }
-    } else {
-      if (in0) atomicAdd(&h[0][b], 1u);
-      if (in1) atomicAdd(&h[1][b], 1u);
-    }
-  }
-  __syncthreads();
-  if (h[0][threadIdx.x]) atomicAdd(&st->hist[0][threadIdx.x], h[0][threadIdx.x]);
-  if (h[1][threadIdx.x]) atomicAdd(&st->hist[1][threadIdx.x], h[1][threadIdx.x]);
-}
-
-// Picks each rank's digit: the first bin b with k < (keys in bins <= b), and
-// k becomes the rank inside that bin (bin 255 and k minus all the keys when
-// no bin qualifies, which a consistent histogram never gives). One thread per
-// bin: an inclusive scan of the 256 counts in LDS, then the one bin whose
-// range holds k writes the state. Launched with 256 threads.
-__global__ void k_select_step(SelectState* st, int shift) {
+// The digit at `shift` of each rank from histograms h (keys matching the
+// prefix): the first bin b with k < (keys in bins <= b); k becomes the rank
+// inside that bin (bin 255 and k minus all the keys when no bin qualifies,
+// which a consistent histogram never gives). All 256 threads of the block
+// take part (thread = bin) and all get the new state.
+__device__ void pick_digit(const uint32_t (*h)[256], int shift, uint64_t (&pre)[2], uint64_t (&msk)[2],
+                           int64_t (&k)[2]) {
   __shared__ int64_t sc[2][256];
+  __shared__ int64_t rk[2];
+  __shared__ uint64_t rp[2], rm[2];
   const int b = threadIdx.x;
-  const int64_t c0 = st->hist[0][b], c1 = st->hist[1][b];
-  const int64_t k0 = st->k[0], k1 = st->k[1];
+  const int64_t c0 = h[0][b], c1 = h[1][b];
   sc[0][b] = c0;
   sc[1][b] = c1;
   __syncthreads();
@@ -285,30 +263,104 @@ __global__ void k_select_step(SelectState* st, int shift) {
     sc[1][b] += a1;
     __syncthreads();
   }
-  // every read of st->k and st->hist is done (the scan's barriers) before
-  // the writes below
-  st->hist[0][b] = 0;
-  st->hist[1][b] = 0;
   for (int t = 0; t < 2; ++t) {
-    const int64_t k = t ? k1 : k0;
     const int64_t incl = sc[t][b], excl = incl - (t ? c1 : c0);
-    const bool hit = excl <= k && k < incl;
-    const bool none = b == 255 && k >= incl;
+    const bool hit = excl <= k[t] && k[t] < incl;
+    const bool none = b == 255 && k[t] >= incl;
     if (hit || none) {
-      const int64_t cum = hit ? excl : incl;
-      st->k[t] = k - cum;
-      st->prefix[t] |= ((uint64_t)b) << shift;
-      st->mask[t] |= ((uint64_t)255) << shift;
-      if (shift == 0) st->value[t] = dkey_inv(st->prefix[t]);
+      rk[t] = k[t] - (hit ? excl : incl);
+      rp[t] = pre[t] | (((uint64_t)b) << shift);
+      rm[t] = msk[t] | (((uint64_t)255) << shift);
     }
   }
+  __syncthreads();
+  for (int t = 0; t < 2; ++t) {
+    k[t] = rk[t];
+    pre[t] = rp[t];
+    msk[t] = rm[t];
+  }
+}
+
+// Histograms of digit `pass` of the keys that match each rank's prefix. In
+// the first passes most keys share their high digits: when every active lane
+// of a wave has the same digit, the wave adds its two match counts with one
+// LDS atomic each instead of one per lane (the same totals). A thread loads
+// kSelUnroll keys before it counts any, so their latencies overlap.
+constexpr int kSelUnroll = 8;
+__global__ void __launch_bounds__(256) k_select_hist(const double* __restrict__ x, int64_t n, Select1State* st,
+                                                     int pass) {
+  __shared__ uint32_t h[2][256];
+  h[0][threadIdx.x] = 0;
+  h[1][threadIdx.x] = 0;
+  uint64_t pre[2], msk[2];
+  int64_t k[2];
+  const int rb = pass == 0 ? 0 : (pass - 1) & 1;  // the state the previous pass used
+  for (int t = 0; t < 2; ++t) {
+    pre[t] = st->prefix[rb][t];
+    msk[t] = st->mask[rb][t];
+    k[t] = st->k[rb][t];
+  }
+  if (pass > 0) {
+    pick_digit(st->hist[pass - 1], 64 - 8 * pass, pre, msk, k);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      for (int t = 0; t < 2; ++t) {
+        st->prefix[pass & 1][t] = pre[t];
+        st->mask[pass & 1][t] = msk[t];
+        st->k[pass & 1][t] = k[t];
+      }
+    }
+  }
+  __syncthreads();
+  const int shift = 56 - 8 * pass;
+  const int lane = threadIdx.x & 63;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x * kSelUnroll;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x * kSelUnroll + threadIdx.x; base < n; base += step) {
+    double v[kSelUnroll];
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; ++u) {
+      const int64_t i = base + (int64_t)u * blockDim.x;
+      v[u] = i < n ? x[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSelUnroll; ++u) {
+      if (base + (int64_t)u * blockDim.x >= n) continue;
+      const uint64_t kk = dkey(v[u]);
+      const uint32_t b = (uint32_t)(kk >> shift) & 255u;
+      const bool in0 = (kk & msk[0]) == pre[0], in1 = (kk & msk[1]) == pre[1];
+      const uint64_t act = __ballot(1);
+      const uint32_t bf = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
+      if (__ballot(b == bf) == act) {
+        const uint64_t v0 = __ballot(in0), v1 = __ballot(in1);
+        if (lane == __ffsll((unsigned long long)act) - 1) {
+          if (v0) atomicAdd(&h[0][bf], (uint32_t)__popcll(v0));
+          if (v1) atomicAdd(&h[1][bf], (uint32_t)__popcll(v1));
+        }
+      } else {
+        if (in0) atomicAdd(&h[0][b], 1u);
+        if (in1) atomicAdd(&h[1][b], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  if (h[0][threadIdx.x]) atomicAdd(&st->hist[pass][0][threadIdx.x], h[0][threadIdx.x]);
+  if (h[1][threadIdx.x]) atomicAdd(&st->hist[pass][1][threadIdx.x], h[1][threadIdx.x]);
 }
 
 // col <- min(round(s*col), 100*scaling) with s = scaling / (2 max(median, 0.001));
 // scale_out (if not null) <- s
-__global__ void k_quantize(double* __restrict__ col, int64_t n, const SelectState* st, int even, double scaling,
-                           double* __restrict__ scale_out) {
-  const double med = even ? (st->value[0] + st->value[1]) / 2.0 : st->value[0];
+__global__ void __launch_bounds__(256) k_quantize(double* __restrict__ col, int64_t n, const Select1State* st, int even,
+                                                  double scaling, double* __restrict__ scale_out) {
+  // the last digit, from the eighth pass's histograms
+  uint64_t pre[2], msk[2];
+  int64_t k[2];
+  for (int t = 0; t < 2; ++t) {
+    pre[t] = st->prefix[1][t];
+    msk[t] = st->mask[1][t];
+    k[t] = st->k[1][t];
+  }
+  pick_digit(st->hist[7], 0, pre, msk, k);
+  const double v0 = dkey_inv(pre[0]), v1 = dkey_inv(pre[1]);
+  const double med = even ? (v0 + v1) / 2.0 : v0;
   const double m = med > 0.001 ? med : 0.001;
   const double s = scaling / (2.0 * m);
   const double maxVal = 100 * scaling;
@@ -871,20 +923,20 @@ hipError_t launch_transpose_fd(const double* FD, int64_t nVX, int64_t nVY, doubl
 }
 
 hipError_t launch_quantize(double* col, int64_t n, void* selectState, double* scale_out, hipStream_t st) {
-  SelectState* S = (SelectState*)selectState;
+  Select1State* S = (Select1State*)selectState;
   const int64_t k1 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
   const int64_t k2 = (n % 2 == 0) ? k1 + 1 : k1;
   hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, st, S, k1, k2);
+  const unsigned gh = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 256 * kSelUnroll - 1) / (256 * kSelUnroll)));
+  for (int pass = 0; pass < 8; ++pass) hipLaunchKernelGGL(k_select_hist, dim3(gh), dim3(256), 0, st, col, n, S, pass);
   const unsigned g = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 255) / 256));
-  for (int shift = 56; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(k_select_hist, dim3(g), dim3(256), 0, st, col, n, S, shift);
-    hipLaunchKernelGGL(k_select_step, dim3(1), dim3(256), 0, st, S, shift);
-  }
   hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, st, col, n, S, (int)(n % 2 == 0), 100000.0, scale_out);
   return hipGetLastError();
 }
 
-size_t select_state_bytes() { return sizeof(SelectState); }
+size_t select_state_bytes() { return sizeof(Select1State); }
+
+size_t bselect_state_bytes() { return sizeof(SelectState); }
 
 size_t bracket_state_bytes() { return sizeof(BracketState); }
 

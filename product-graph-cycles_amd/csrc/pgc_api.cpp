@@ -42,6 +42,7 @@ hipError_t launch_fill_costs_batch(const FItem*, int, hipStream_t);
 hipError_t launch_fill_i32_batch(const FillItem*, int, int32_t, hipStream_t);
 hipError_t launch_energy_batch(const EItem*, int, hipStream_t);
 size_t select_state_bytes();
+size_t bselect_state_bytes();
 size_t bracket_state_bytes();
 hipError_t launch_layout_count(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int32_t*, int32_t*,
                                hipStream_t);
@@ -2111,7 +2112,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     if ((rc = grow(&ctx->d_qitems, ctx->qitems_cap, (int64_t)qitems.size()))) return rc;
     if ((rc = grow(&ctx->d_fitems, ctx->fitems_cap, (int64_t)fitems.size()))) return rc;
     const int64_t nsel = (int64_t)std::min<size_t>(qitems.size(), (size_t)chunk);
-    if ((rc = grow(&ctx->d_select, ctx->select_cap, nsel * (int64_t)select_state_bytes()))) return rc;
+    if ((rc = grow(&ctx->d_select, ctx->select_cap, nsel * (int64_t)bselect_state_bytes()))) return rc;
     if ((rc = grow(&ctx->d_bracket, ctx->bracket_cap, nsel * (int64_t)bracket_state_bytes()))) return rc;
     // the median brackets' candidate regions, one per column
     int64_t candN = 0;

@@ -1104,16 +1104,18 @@ struct SweepG {
     const int o = (own ? c : 0) * W * 64 + lane;
     const int v = c * 64 + lane;
     const bool act = own && v < npl;
-    // the interlayer sources stay in registers (layer 1's pattern); the
-    // intralayer ones are loaded with their costs (layer 0's pattern, one
-    // load each beside the cost loads, so no added wait)
-    uint32_t oX[WX];
+    // the sources of both classes stay in registers (layer 1's interlayer
+    // and layer 0's intralayer pattern), so every gather of a step is issued
+    // at once, beside its cost loads
+    uint32_t oX[WX], oI[WI];
     {
       const PGC_GLOBAL int* __restrict__ srcX = gp(Sx.src);
+      const PGC_GLOBAL int* __restrict__ srcI = gp(Sa.src);
 #pragma unroll
       for (int j = 0; j < WX; ++j) oX[j] = (uint32_t)srcX[Sx.lstride + o + j * 64] << 3;
+#pragma unroll
+      for (int j = 0; j < WI; ++j) oI[j] = (uint32_t)srcI[o + j * 64] << 3;
     }
-    const PGC_GLOBAL int* __restrict__ srcI = gp(Sa.src);
     for (int layer = 0; layer <= L; ++layer) {
       const int cur = (layer & 1) * 2;
       const int prev = 2 - cur;
@@ -1162,9 +1164,6 @@ struct SweepG {
         if (own) {
           CT cc[WI];
           float t[WI];
-          uint32_t oI[WI];
-#pragma unroll
-          for (int j = 0; j < WI; ++j) oI[j] = (uint32_t)srcI[o + j * 64] << 3;
           load_ct(Sa, layer * Sa.lstride + o, cc, t);
           const DP D0 = D + cur * ds;
           double dv[WI];
