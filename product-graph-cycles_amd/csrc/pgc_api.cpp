@@ -344,6 +344,9 @@ struct pgc_graph {
   int64_t nEX = 0, nEY = 0;
   int prune = 0;
   bool conjugate = false;
+  // conjugate graphs: edges per layer; every layer has the same in-edge
+  // pattern and layer i's edges are [i*rep_B, (i+1)*rep_B) in the same order
+  int64_t rep_B = 0;
   int line_integral = 0, length_mode = 0;
   // a derived graph shares base's product space, ids, layout and energy
   // inputs (read-only) and owns its energy, costs and solver scratch
@@ -558,12 +561,103 @@ int build_layout_closed(pgc_graph* g, DevAlloc& A) {
   return PGC_OK;
 }
 
+// The layout of a conjugate product graph from its first two layers: every
+// layer has the same in-edge pattern, with edge ids offset by rep_B per layer
+// (k_conj_product_space). The generic build runs on layers 0 and 1 only (a
+// three-layer graph whose third layer holds the targets of layer 1's
+// interlayer edges); layer 0's intralayer slots and layer 1's interlayer
+// slots, whose edge ids are those of layer 0, are then copied to every layer
+// with the layer's edge-id base, as for build_layout_closed. With three or
+// more layers every edge into layer 0 comes from the last layer and moves to
+// the closing layer, so the pattern holds there too. Returns PGC_ERR_UNSUPPORTED
+// (nothing built) if the first two layers hold an edge of neither class.
+int build_layout_repeat(pgc_graph* g, DevAlloc& A) {
+  hipStream_t st = g->ctx->stream;
+  const int64_t npl = g->npl, B = g->rep_B;
+  const int64_t nTotM = 3 * npl;  // nodes of the three-layer graph
+  g->nLayers = g->L + 1;
+  g->nChunks = (int)((npl + 63) / 64);
+  const int nC = g->nChunks;
+  int32_t *d_dropped = nullptr, *d_fill = nullptr;
+  HIPCHK(A.get(&g->d_cnt, 2 * nTotM));
+  HIPCHK(A.get(&d_fill, 2 * nTotM));
+  HIPCHK(A.get(&d_dropped, 1));
+  HIPCHK(A.get(&g->d_width, 4));
+  HIPCHK(launch_fill_i32(g->d_cnt, 2 * nTotM, 0, st));
+  HIPCHK(launch_fill_i32(d_fill, 2 * nTotM, 0, st));
+  HIPCHK(launch_fill_i32(d_dropped, 1, 0, st));
+  HIPCHK(launch_layout_count(g->d_src, g->d_trgt, 2 * B, npl, 2 * npl, g->d_cnt, d_dropped, st));
+  HIPCHK(launch_layout_width(g->d_cnt, nTotM, g->d_width, st));
+  int32_t dropped = 0;
+  HIPCHK(hipMemcpyAsync(&dropped, d_dropped, 4, hipMemcpyDeviceToHost, st));
+  int32_t wn[4];
+  HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (dropped) {
+    A.put(d_dropped);
+    A.put(d_fill);
+    A.put(g->d_width);
+    A.put(g->d_cnt);
+    g->d_width = g->d_cnt = nullptr;
+    return PGC_ERR_UNSUPPORTED;
+  }
+  const int32_t mx = std::max(wn[0], wn[1]);
+  const int32_t W = slot_width(mx);
+  g->width[0] = W;
+  g->width[1] = W;
+  g->max_in_degree = mx;
+  g->maxw = mx;
+  g->cls_deg[0] = wn[0];
+  g->cls_deg[1] = wn[1];
+  // each class's edges of the two layers are two copies of one layer's
+  g->intra_slots_valid = (int64_t)wn[2] / 2 * g->L;
+  g->inter_slots_valid = (int64_t)wn[3] / 2 * g->L;
+  g->dropped = 0;
+  HIPCHK(hipMemcpyAsync(g->d_width, g->width, 2 * 4, hipMemcpyHostToDevice, st));
+  const int64_t tn = (int64_t)nC * W * 64;  // slots per layer and class
+  int32_t* d_m = nullptr;                   // [src0 | src1 | eid0 | eid1] of the three layers
+  HIPCHK(A.get(&d_m, 4 * 3 * tn));
+  HIPCHK(launch_fill_i32(d_m, 4 * 3 * tn, -1, st));
+  HIPCHK(launch_layout_fill(g->d_src, g->d_trgt, 2 * B, npl, 2 * npl, nC, d_fill, g->d_cnt, g->d_width, d_m,
+                            d_m + 3 * tn, d_m + 6 * tn, d_m + 9 * tn, st));
+  for (int c = 0; c < 2; ++c) {
+    const int64_t total = tn * g->nLayers;
+    if (total >= ((int64_t)1 << 31) - 1 - kSlotPad)
+      return fail(PGC_ERR_UNSUPPORTED, "in-edge layout exceeds 2^31 slots");
+    g->lstride[c] = (int32_t)tn;
+    g->nslots[c] = total;
+    HIPCHK(A.get(&g->d_s[c], total + kSlotPad));
+    HIPCHK(A.get(&g->d_e[c], total + kSlotPad));
+    // the template: layer 0 of the intralayer class, layer 1 of the interlayer class
+    const int64_t at = c == 0 ? 0 : tn;
+    HIPCHK(launch_xlayout_replicate(d_m + c * 3 * tn + at, d_m + (2 + c) * 3 * tn + at, tn, g->nLayers,
+                                    total + kSlotPad, c, B, (int32_t)npl, g->d_s[c], g->d_e[c], st));
+  }
+  if (W > 32) {
+    HIPCHK(A.get(&g->d_grp, 2 * (int64_t)nC));
+    HIPCHK(launch_fill_i32(g->d_grp, 2 * (int64_t)nC, 0, st));
+    HIPCHK(launch_chunk_groups(g->d_cnt, npl, nTotM, nC, g->d_grp, st));
+  }
+  g->src_inv[0] = 1;
+  g->src_inv[1] = 1;
+  A.put(d_m);
+  A.put(d_dropped);
+  A.put(d_fill);
+  g->layout = true;
+  return PGC_OK;
+}
+
 int build_layout(pgc_graph* g, DevAlloc& A) {
   if (g->layout) return PGC_OK;
   const char* generic = std::getenv("PGC_GENERIC_LAYOUT");
-  if (g->d_ex && !g->conjugate && !g->base && g->nEY >= 3 && g->L == g->nEY && g->npl == g->nVX &&
-      !(generic && generic[0] == '1'))
+  const bool gen = generic && generic[0] == '1';
+  if (g->d_ex && !g->conjugate && !g->base && g->nEY >= 3 && g->L == g->nEY && g->npl == g->nVX && !gen)
     return build_layout_closed(g, A);
+  if (g->conjugate && g->rep_B > 0 && g->L >= 3 && g->numNodes == g->L * g->npl && g->E == g->L * g->rep_B &&
+      !gen) {
+    const int rc = build_layout_repeat(g, A);
+    if (rc != PGC_ERR_UNSUPPORTED) return rc;
+  }
   hipStream_t st = g->ctx->stream;
   const int64_t npl = g->npl;
   g->nLayers = g->L + 1;
@@ -1185,6 +1279,7 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   // numNodes = max(SRC) + 1, numLayers = numNodes / npl as run() derives them
   int rc = finish_graph_meta(g.get());
   if (rc && rc != PGC_ERR_UNSUPPORTED) return rc;
+  g->rep_B = perLayer;
   // branchGraph (conj_product_space.cpp:170-225; both of its loops read EX(j, 0))
   const int64_t npl = g->npl;
   std::vector<std::vector<int64_t>> bg(npl);

@@ -1,6 +1,6 @@
-"""The solver's in-edge layout: the closed-form build of generated
-non-conjugate product graphs against the generic build from SRC/TRGT and
-against a numpy model of the layout.
+"""The solver's in-edge layout: the closed-form builds of generated product
+graphs against the generic build from SRC/TRGT and against a numpy model of
+the layout.
 
 The reference keeps, per node and class, the set of in-edges
 (buildingInGraph, cost_time_ratio_solver.cpp:42-66; VECSET,
@@ -10,8 +10,10 @@ its layer, edge id), padding pointing at the sentinel npl with edge id -1;
 an edge into layer 0 from a source id above npl moves to the closing layer L
 (the closing copies), an edge that is neither intralayer nor interlayer is
 dropped. The model below states that rule directly in numpy. Generated
-graphs with three or more Y edges take the closed-form build (the X graph's
-slots copied to every layer); PGC_GENERIC_LAYOUT=1 forces the generic build.
+non-conjugate graphs with three or more Y edges take the closed-form build
+(the X graph's slots copied to every layer), conjugate graphs with three or
+more layers a build from their first two layers (every layer repeats one
+pattern); PGC_GENERIC_LAYOUT=1 forces the generic build.
 Bar: slot arrays equal element for element (integer data), and the solves of
 both builds equal (rows, objective, counters).
 """
@@ -108,6 +110,44 @@ def test_closed_form_layout_equals_generic_and_model(name, make, prune, monkeypa
         assert np.array_equal(lay_c[c][1], lay_g[c][1]), "closed-form edge ids differ from the generic build"
         assert np.array_equal(lay_c[c][0], model[c][0]), "sources differ from the layout rule"
         assert np.array_equal(lay_c[c][1], model[c][1]), "edge ids differ from the layout rule"
+
+
+def _conj_graph(p, prune=False):
+    ctx = _lib.default_context(0)
+    return _lib.Graph.generate(ctx, p["VX"], p["EX"], p["VY"], p["EY"], p["feat_diff"].astype(np.float64), True,
+                               True, prune, "vanilla", "", NX=p["NX"].astype(np.float64),
+                               NY=p["NY"].astype(np.float64))
+
+
+CONJ = [
+    ("conj ico0 y3", lambda: S.make_pair(0, 3, seed=11, with_normals=True)),
+    ("conj ico1 y5", lambda: S.make_pair(1, 5, seed=12, with_normals=True)),
+    ("conj ico1 y8", lambda: S.make_pair(1, 8, seed=13, with_normals=True)),
+    ("conj bipyramid12 y4", lambda: S.make_pair(0, 4, seed=14, mesh=S.bipyramid(12), with_normals=True)),
+    ("conj ico0 y2 (generic)", lambda: S.make_pair(0, 2, seed=15, with_normals=True)),
+]
+
+
+@pytest.mark.parametrize("name,make", CONJ, ids=[c[0] for c in CONJ])
+def test_conjugate_layout_from_two_layers(name, make, monkeypatch):
+    # conjugate graphs repeat one layer's in-edge pattern: the layout is built
+    # from layers 0 and 1 and copied to every layer
+    print("case", name, flush=True)
+    p = make()
+    g_rep = _conj_graph(p)
+    monkeypatch.setenv("PGC_GENERIC_LAYOUT", "1")
+    g_gen = _conj_graph(p)
+    lay_g = [g_gen.layout(c) for c in (0, 1)]
+    monkeypatch.delenv("PGC_GENERIC_LAYOUT")
+    lay_r = [g_rep.layout(c) for c in (0, 1)]
+    _, src, trgt = O.conj_product_space(p["EX"], p["EY"])
+    npl = 3 * len(p["EX"])
+    W = lay_r[0][2]
+    model = model_layout(src, trgt, npl, W)
+    for c in (0, 1):
+        assert lay_g[c][2] == W
+        assert np.array_equal(lay_r[c][0], lay_g[c][0]) and np.array_equal(lay_r[c][1], lay_g[c][1])
+        assert np.array_equal(lay_r[c][0], model[c][0]) and np.array_equal(lay_r[c][1], model[c][1])
 
 
 @pytest.mark.parametrize("K", [1, 2, 3])
