@@ -60,6 +60,10 @@ def lib():
         L.orc_normals2d.argtypes = [P, C.c_long, P, C.c_long, P, P]
         L.orc_normals3d.restype = None
         L.orc_normals3d.argtypes = [P, C.c_long, P, C.c_long, P]
+        L.orc_exact_ratio.restype = C.c_int
+        L.orc_exact_ratio.argtypes = [P, P, C.c_int64, P, C.c_int64, C.c_int, C.c_int64, C.c_int64, P, P, P, P,
+                                      C.c_int64, P, P]
+        L.orc_exact_error.restype = C.c_char_p
         _LIB = L
     return _LIB
 
@@ -255,3 +259,26 @@ def normals3d(V, F):
     out = np.zeros((len(V), 3))
     lib().orc_normals3d(_p(V), len(V), _p(F), len(F), _p(out))
     return out
+
+
+def exact_ratio(src, trgt, q, npl, K, a0=None, b0=None):
+    """Exact minimum ratio of the once-wrapping cycles with integer costs q and
+    unit times (oracle/exact_ratio.cpp, Dinkelbach in int64). (a0, b0): a ratio
+    not below the minimum (default: max(q) + 1 over 1). Returns (a, b, start,
+    edges, iterations); edges is empty when no cycle is below a0/b0, which then
+    is the minimum (certified)."""
+    src = np.ascontiguousarray(src, np.int32)
+    trgt = np.ascontiguousarray(trgt, np.int32)
+    qi = np.ascontiguousarray(np.rint(np.asarray(q, np.float64)).astype(np.int64))
+    assert np.array_equal(qi.astype(np.float64), np.asarray(q, np.float64)), "costs must be integers"
+    if a0 is None:
+        a0, b0 = int(qi.max()) + 1, 1
+    n_nodes = int(src.max()) + 1
+    cap = (n_nodes // npl) * K + 8
+    path = np.zeros(cap, np.int32)
+    a, b, st, pl, it = (C.c_int64(), C.c_int64(), C.c_int32(), C.c_int64(), C.c_int64())
+    rc = lib().orc_exact_ratio(_p(src), _p(trgt), len(src), _p(qi), int(npl), int(K), int(a0), int(b0), C.byref(a),
+                               C.byref(b), C.byref(st), _p(path), cap, C.byref(pl), C.byref(it))
+    if rc != 0:
+        raise RuntimeError(lib().orc_exact_error().decode())
+    return a.value, b.value, st.value, path[:pl.value].copy(), it.value
