@@ -119,3 +119,14 @@ def test_most_medium_instances_certified_optimal():
         a, b, _, _, _ = O.exact_ratio(src, trgt, q, npl, K)
         certified += Fraction(int(q[edges].sum()), len(edges)) == Fraction(a, b)
     assert certified >= len(MEDIUM) - 3
+
+
+def test_c1_result_certified_optimal():
+    # the single-pair benchmark pair (icosphere s=4 x 128-point curve,
+    # max_depth 3): no once-wrapping cycle is below the oracle's cycle, which
+    # the device solve returns bit for bit (tests/test_gpu_parity.py)
+    p = S.make_pair(4, 128, seed=0)
+    src, trgt, edges, obj, stats, q = _solve(p, 3, True)
+    c, n = int(q[edges].sum()), len(edges)
+    _, _, _, below, iters = O.exact_ratio(src, trgt, q, len(p["VX"]), 3, c, n)
+    assert len(below) == 0 and iters == 1
