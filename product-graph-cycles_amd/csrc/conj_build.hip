@@ -146,10 +146,19 @@ __device__ void rotation_quat(V3 v2, V3 n2, V3 c2, V3 v3, V3 n3, V3 c3, double q
                     {Sxy - Syx, Szx + Sxz, Syz + Szy, -Sxx - Syy + Szz}};
   double V[4][4];
   jacobi4(N, V);
-  int best = 0;
-  for (int i = 1; i < 4; ++i)
-    if (N[i][i] > N[best][best]) best = i;
-  const double w = V[0][best], x = V[1][best], y = V[2][best], z = V[3][best];
+  // the eigenvector of the largest eigenvalue (the first of equal ones),
+  // selected with constant indices so that V stays in registers
+  double bv = N[0][0], w = V[0][0], x = V[1][0], y = V[2][0], z = V[3][0];
+#pragma unroll
+  for (int i = 1; i < 4; ++i) {
+    if (N[i][i] > bv) {
+      bv = N[i][i];
+      w = V[0][i];
+      x = V[1][i];
+      y = V[2][i];
+      z = V[3][i];
+    }
+  }
   const double n = sqrt(w * w + x * x + y * y + z * z);
   const double sg = w < 0 ? -1.0 : 1.0;
   q[0] = sg * w / n;
@@ -158,7 +167,7 @@ __device__ void rotation_quat(V3 v2, V3 n2, V3 c2, V3 v3, V3 n3, V3 c3, double q
   q[3] = sg * z / n;
 }
 
-__global__ void k_conj_quat(const double* __restrict__ VX, const double* __restrict__ NX,
+__global__ void __launch_bounds__(128) k_conj_quat(const double* __restrict__ VX, const double* __restrict__ NX,
                             const double* __restrict__ VY, const double* __restrict__ NY,
                             const int32_t* __restrict__ EX, int64_t nEX, const int32_t* __restrict__ EY,
                             int64_t nEY, int prune, double* __restrict__ quat) {
