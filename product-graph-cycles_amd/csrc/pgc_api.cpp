@@ -334,6 +334,10 @@ struct pgc_graph {
   // branch graph (host CSR over layer-0 nodes)
   bool has_bg = false;
   std::vector<int64_t> bg_off, bg_val;
+  // conjugate graphs: the X data their branch graph is built from at its
+  // first use (a min-cost split): EX(j, 0), the X out-edge lists
+  std::vector<int32_t> cbg_ex0, cbg_off, cbg_list;
+  int cbg_prune = 0;
   JobScratch js;
   // energy inputs, kept to recompute the energy from another feat_diff
   double *d_vx = nullptr, *d_vy = nullptr;
@@ -1280,36 +1284,57 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   int rc = finish_graph_meta(g.get());
   if (rc && rc != PGC_ERR_UNSUPPORTED) return rc;
   g->rep_B = perLayer;
-  // branchGraph (conj_product_space.cpp:170-225; both of its loops read EX(j, 0))
-  const int64_t npl = g->npl;
-  std::vector<std::vector<int64_t>> bg(npl);
-  for (int64_t j = 0; j < nEX; ++j) {
-    bg[j].push_back(j + nEX);
-    if (!prune) {
-      bg[j + nEX].push_back(j + 2 * nEX);
-      bg[j + 2 * nEX].push_back(j);
-    } else {
-      bg[j + nEX].push_back(j);
-    }
-    const int32_t v = EX_host[2 * j];
-    for (int32_t k = outOff[v]; k < outOff[v + 1]; ++k) {
-      const int64_t it = outList[k];
-      bg[j].push_back(it);
-      bg[j + nEX].push_back(it + nEX);
-      if (!prune) bg[j + 2 * nEX].push_back(it + 2 * nEX);
-    }
-  }
-  g->bg_off.assign(npl + 1, 0);
-  g->bg_val.clear();
-  for (int64_t v = 0; v < npl; ++v) {
-    std::sort(bg[v].begin(), bg[v].end());
-    bg[v].erase(std::unique(bg[v].begin(), bg[v].end()), bg[v].end());
-    g->bg_val.insert(g->bg_val.end(), bg[v].begin(), bg[v].end());
-    g->bg_off[v + 1] = (int64_t)g->bg_val.size();
-  }
-  g->has_bg = true;
+  // the branch graph is built at its first use (build_conj_bg)
+  g->cbg_ex0.resize(nEX);
+  for (int64_t j = 0; j < nEX; ++j) g->cbg_ex0[j] = EX_host[2 * j];
+  g->cbg_off = outOff;
+  g->cbg_list = outList;
+  g->cbg_prune = prune ? 1 : 0;
   *out = g.release();
   return PGC_OK;
+}
+
+// branchGraph of a conjugate product graph (conj_product_space.cpp:170-225;
+// both of its loops read EX(j, 0)): node j (non-degenerate) links to j+|EX|
+// and to the X out-edges it of EX(j, 0); node j+|EX| to j+2|EX| (to j when
+// pruned) and to it+|EX|; node j+2|EX| to j and to it+2|EX|. Values ascending,
+// without repeats, as the reference's sets hold them.
+void build_conj_bg(pgc_graph* g) {
+  const int64_t nEX = (int64_t)g->cbg_ex0.size(), npl = g->npl;
+  const bool prune = g->cbg_prune != 0;
+  const std::vector<int32_t>& off = g->cbg_off;
+  const std::vector<int32_t>& lst = g->cbg_list;
+  g->bg_off.assign(npl + 1, 0);
+  for (int64_t j = 0; j < nEX; ++j) {
+    const int64_t d = 1 + off[g->cbg_ex0[j] + 1] - off[g->cbg_ex0[j]];
+    g->bg_off[j + 1] = d;
+    g->bg_off[j + nEX + 1] = d;
+    if (!prune) g->bg_off[j + 2 * nEX + 1] = d;
+  }
+  for (int64_t v = 0; v < npl; ++v) g->bg_off[v + 1] += g->bg_off[v];
+  g->bg_val.assign(g->bg_off[npl], 0);
+  for (int64_t j = 0; j < nEX; ++j) {
+    const int32_t v = g->cbg_ex0[j];
+    for (int cls = 0; cls < (prune ? 2 : 3); ++cls) {
+      const int64_t node = j + cls * nEX;
+      int64_t* out = g->bg_val.data() + g->bg_off[node];
+      int64_t n = 0;
+      out[n++] = cls == 0 ? j + nEX : (cls == 1 ? (prune ? j : j + 2 * nEX) : j);
+      for (int32_t k = off[v]; k < off[v + 1]; ++k) out[n++] = (int64_t)lst[k] + cls * nEX;
+      std::sort(out, out + n);
+    }
+  }
+  // drop repeats inside each node's list, compacting the CSR
+  int64_t w = 0;
+  for (int64_t node = 0; node < npl; ++node) {
+    const int64_t b = g->bg_off[node], e = g->bg_off[node + 1];
+    g->bg_off[node] = w;
+    for (int64_t i = b; i < e; ++i)
+      if (i == b || g->bg_val[i] != g->bg_val[i - 1]) g->bg_val[w++] = g->bg_val[i];
+  }
+  g->bg_off[npl] = w;
+  g->bg_val.resize(w);
+  g->has_bg = true;
 }
 
 // Implicit costs: a non-conjugate graph with the vanilla energy (one column,
@@ -2530,6 +2555,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   bool bgReady = false;
   auto ensure_bg = [&]() -> int {
     if (bgReady) return PGC_OK;
+    if (!g->has_bg && g->conjugate && !g->cbg_off.empty()) build_conj_bg(g);
     if (!g->has_bg) {
       const int64_t n0 = g->lstride[0];  // layer-0 block of the intralayer class
       std::vector<int32_t> sl(n0), el(n0);
