@@ -347,6 +347,7 @@ struct pgc_graph {
   int32_t *d_ex = nullptr, *d_ey = nullptr;
   int64_t nEX = 0, nEY = 0;
   int prune = 0;
+  int32_t xdeg[4] = {-1, -1, -1, -1};  // X in-degrees: largest intralayer, interlayer; their sums (one layer)
   bool conjugate = false;
   // conjugate graphs: edges per layer; every layer has the same in-edge
   // pattern and layer i's edges are [i*rep_B, (i+1)*rep_B) in the same order
@@ -516,10 +517,14 @@ int build_layout_closed(pgc_graph* g, DevAlloc& A) {
   HIPCHK(launch_fill_i32(g->d_cnt, 2 * npl, 0, st));
   HIPCHK(launch_fill_i32(d_fill, 2 * npl, 0, st));
   HIPCHK(launch_xlayout_count(g->d_ex, g->nEX, npl, g->prune, g->d_cnt, st));
-  HIPCHK(launch_layout_width(g->d_cnt, npl, g->d_width, st));
   int32_t wn[4];
-  HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (g->xdeg[0] >= 0) {
+    std::copy(g->xdeg, g->xdeg + 4, wn);  // known from the host copy of EX
+  } else {
+    HIPCHK(launch_layout_width(g->d_cnt, npl, g->d_width, st));
+    HIPCHK(hipMemcpyAsync(wn, g->d_width, 4 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   const int32_t mx = std::max(wn[0], wn[1]);
   const int32_t W = slot_width(mx);
   g->width[0] = W;
@@ -532,7 +537,9 @@ int build_layout_closed(pgc_graph* g, DevAlloc& A) {
   g->intra_slots_valid = (int64_t)wn[2] * g->L;
   g->inter_slots_valid = (int64_t)wn[3] * g->L;
   g->dropped = 0;
-  HIPCHK(hipMemcpyAsync(g->d_width, g->width, 2 * 4, hipMemcpyHostToDevice, st));
+  // both classes' width, by a fill kernel: a copy from pageable host memory
+  // would wait for the queued work
+  HIPCHK(launch_fill_i32(g->d_width, 2, W, st));
   const int64_t tn = (int64_t)nC * W * 64;  // template slots per class
   int32_t* d_t = nullptr;                   // [src0 | src1 | r0 | r1]
   HIPCHK(A.get(&d_t, 4 * tn));
@@ -1413,6 +1420,21 @@ int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   int rc = keep_feat_diff(g.get(), dFD);
   if (rc) return rc;
   g->energy_fresh = true;
+  // the X graph's in-degrees per class (largest, sum): the closed-form layout
+  // takes its slot width from them without reading the device back
+  {
+    std::vector<int32_t> deg(nVX, 0);
+    for (int64_t j = 0; j < nEX; ++j) deg[EX_host[2 * j + 1]]++;
+    int32_t mi = 0, mx = 0;
+    for (int64_t x = 0; x < nVX; ++x) {
+      mi = std::max(mi, prune ? 0 : deg[x]);
+      mx = std::max(mx, deg[x] + 1);
+    }
+    g->xdeg[0] = mi;
+    g->xdeg[1] = mx;
+    g->xdeg[2] = prune ? 0 : (int32_t)nEX;
+    g->xdeg[3] = (int32_t)(nEX + nVX);
+  }
   if (prune) {
     // branchGraph[x] = out-neighbour vertex ids (product_space.cpp:122-131)
     std::vector<std::vector<int64_t>> bg(nVX);
