@@ -58,6 +58,27 @@ __device__ __forceinline__ SlotClass uni_class(const PGC_GLOBAL SlotClass& c) {
 // output distinguishes (distances are never NaN).
 __device__ __forceinline__ double pgc_min(double a, double b) { return __builtin_fmin(a, b); }
 
+// Depth of the winning source of an interlayer scan: the first kk < K whose
+// rounded sum row(kk) + bw equals best (the reference's loop order), 0 if
+// none. Up to four depths the rows are read together (one round trip instead
+// of up to K dependent ones); the first match is then taken in depth order.
+template <typename RowAt>
+__device__ __forceinline__ int recover_depth(const RowAt& row, int K, double bw, double best) {
+  if (K <= 4) {
+    double d[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) d[kk] = row(kk < K ? kk : K - 1);
+    int ks = 0;
+#pragma unroll
+    for (int kk = 3; kk >= 0; --kk)
+      if (kk < K && d[kk] + bw == best) ks = kk;
+    return ks;
+  }
+  for (int kk = 0; kk < K; ++kk)
+    if (row(kk) + bw == best) return kk;
+  return 0;
+}
+
 struct SweepState {
   int complete;  // 0 when a layer 0<l<L had no improvement
   int p0, p1, odd;  // barrier-or state (see BarrierOr)
@@ -174,12 +195,7 @@ struct Sweep {
       int ks = 0;
       if (!K1) {
         const double bw = weight(bc, bt, ratio, rt);
-        for (int kk = 0; kk < K; ++kk) {
-          if (D[(prev + kk) * ds + bs] + bw == best) {
-            ks = kk;
-            break;
-          }
-        }
+        ks = recover_depth([&](int kk) { return D[(prev + kk) * ds + bs]; }, K, bw, best);
       }
       const int64_t pi = (int64_t)layer * npl * K + v;
       const int pv = bs + (layer - 1) * npl * K + ks * npl;
@@ -808,12 +824,7 @@ struct SweepG {
       int ks = 0;
       if (K > 1) {
         const double bw = weight(bc, bt, ratio, rt);
-        for (int kk = 0; kk < K; ++kk) {
-          if (dget(D + (prev + kk) * ds + bs) + bw == best) {
-            ks = kk;
-            break;
-          }
-        }
+        ks = recover_depth([&](int kk) { return dget(D + (prev + kk) * ds + bs); }, K, bw, best);
       }
       const int64_t pi = (int64_t)layer * npl * K + v;
       const int pv = bs + (layer - 1) * npl * K + ks * npl;
@@ -1009,12 +1020,7 @@ struct SweepG {
         int ks = 0;
         if (K > 1) {
           const double bw = weight(bc, bt, ratio, rt);
-          for (int kk = 0; kk < K; ++kk) {
-            if (dget(D + (prev + kk) * ds + bs) + bw == best) {
-              ks = kk;
-              break;
-            }
-          }
+          ks = recover_depth([&](int kk) { return dget(D + (prev + kk) * ds + bs); }, K, bw, best);
         }
         const int64_t pi = (int64_t)layer * npl * K + v;
         const int pv = bs + (layer - 1) * npl * K + ks * npl;
