@@ -985,17 +985,28 @@ struct SweepG {
       if (HAS_TIME) t[j] = tim[b + j * 64];
     }
   }
+  // The cached steps load each chunk's costs while the wave relaxes its
+  // previous chunk (chunk i+1's loads are issued before chunk i's gathers),
+  // so only the first chunk of a step waits for its costs. A load past the
+  // last chunk rereads the current one (in bounds, unused).
   __device__ __forceinline__ bool inter_cached(int layer, int cur, int prev, const int (&sX)[MC > 0 ? MC : 1][WX],
                                                double ratio, double rt, double ub) {
     bool imp = false;
     const DP Dp = (K == 1) ? D + prev * ds : Dmin;
+    CT ccs[MC > 0 ? MC : 1][WX];
+    float ts[MC > 0 ? MC : 1][WX];
+    load_ct(Sx, layer * Sx.lstride + (wave < nC ? wave : 0) * W * 64 + lane, ccs[0], ts[0]);
 #pragma unroll
     for (int i = 0; i < MC; ++i) {
       const int c = wave + i * nw;
       if (c >= nC) break;
-      CT cc[WX];
-      float t[WX];
-      load_ct(Sx, layer * Sx.lstride + c * W * 64 + lane, cc, t);
+      if (i + 1 < MC) {
+        const int c1 = c + nw;
+        load_ct(Sx, layer * Sx.lstride + (c1 < nC ? c1 : c) * W * 64 + lane, ccs[i + 1 < MC ? i + 1 : i],
+                ts[i + 1 < MC ? i + 1 : i]);
+      }
+      const CT(&cc)[WX] = ccs[i];
+      const float(&t)[WX] = ts[i];
       const int v = c * 64 + lane;
       if (v >= npl) continue;
       const int(&s)[WX] = sX[i];
@@ -1039,13 +1050,20 @@ struct SweepG {
                                                double ratio, double rt, double ub) {
     bool imp = false;
     const DP Dk = D + (cur + k - 1) * ds;
+    CT ccs[MC > 0 ? MC : 1][WI];
+    float ts[MC > 0 ? MC : 1][WI];
+    load_ct(Sa, layer * Sa.lstride + (wave < nC ? wave : 0) * W * 64 + lane, ccs[0], ts[0]);
 #pragma unroll
     for (int i = 0; i < MC; ++i) {
       const int c = wave + i * nw;
       if (c >= nC) break;
-      CT cc[WI];
-      float t[WI];
-      load_ct(Sa, layer * Sa.lstride + c * W * 64 + lane, cc, t);
+      if (i + 1 < MC) {
+        const int c1 = c + nw;
+        load_ct(Sa, layer * Sa.lstride + (c1 < nC ? c1 : c) * W * 64 + lane, ccs[i + 1 < MC ? i + 1 : i],
+                ts[i + 1 < MC ? i + 1 : i]);
+      }
+      const CT(&cc)[WI] = ccs[i];
+      const float(&t)[WI] = ts[i];
       const int v = c * 64 + lane;
       if (v >= npl) continue;
       const int(&s)[WI] = sI[i];
