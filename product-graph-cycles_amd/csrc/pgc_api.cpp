@@ -919,7 +919,9 @@ int ensure_scratch(pgc_graph* g, int K, DevAlloc& A, bool min_cost = true) {
     HIPCHK(A.get(&s.lb_key, npl));
   }
   if (kernel_maxc(g, K) == 0) {
-    HIPCHK(A.get(&s.dtab, (int64_t)(2 * K + 1) * (npl + 1)));
+    // rows, the minimum row, and (device-wide K == 2 sweep) the two layers'
+    // interleaved depth pairs after them, 16-byte aligned
+    HIPCHK(A.get(&s.dtab, (int64_t)(2 * K + 1) * (npl + 1) + 4 * (npl + 1) + 2));
     HIPCHK(A.get(&s.grid, (grid_state_bytes() + 3) / 4));
   }
   return PGC_OK;

@@ -1112,6 +1112,12 @@ struct SweepG {
   // min_k(D_k + w) = min_k(D_k) + w, and the first slot attaining the minimum
   // and the first depth of that slot attaining it are the same in both. With
   // no minimum row, the intralayer step writes only its depth row.
+  // The interlayer gathers are the step's bound: scattered 8-byte reads, each
+  // lane its own line through the CU's L1. Every node therefore also stores
+  // its two depth values of layer l side by side (pairs[l & 1][v] = (D0, D1),
+  // after the table's rows and minimum row), and the interlayer step gathers
+  // each source's two depths with one 16-byte load: half the requests. The
+  // rows are still written; the rest of the solve reads them.
   // The sources are held as 32-bit byte offsets into a row (a row has at most
   // 2^28 entries here), so every gather is a scalar row base plus a vector
   // offset and no 64-bit address stays live across the loop.
@@ -1121,6 +1127,14 @@ struct SweepG {
   __device__ __forceinline__ double rget(const DP row, uint32_t off) const {
     asm volatile("" : "+v"(off));
     return *(const PGC_GLOBAL double*)((const PGC_GLOBAL char*)row + off);
+  }
+  // both depths of a node from the interleaved pairs (one 16-byte load);
+  // off is the node's 8-byte row offset, doubled
+  typedef double dpair __attribute__((ext_vector_type(2)));
+  __device__ __forceinline__ dpair pget(const DP pairs, uint32_t off) const {
+    off <<= 1;
+    asm volatile("" : "+v"(off));
+    return *(const PGC_GLOBAL dpair*)((const PGC_GLOBAL char*)pairs + off);
   }
   __device__ __forceinline__ int run_layers_grid_k2(double ratio, double rt, double ub) {
     const int c = wave;
@@ -1140,9 +1154,13 @@ struct SweepG {
 #pragma unroll
       for (int j = 0; j < WI; ++j) oI[j] = (uint32_t)srcI[o + j * 64] << 3;
     }
+    const DP P = D + ((2 * K + 1) * ds + 1) / 2 * 2;
+    // the padding sentinel of both pair buffers (the rows' entry npl is +inf)
+    if (blockIdx.x == 0 && threadIdx.x < 4) dput(P + (threadIdx.x >> 1) * 2 * ds + 2 * npl + (threadIdx.x & 1), PGC_INF);
     for (int layer = 0; layer <= L; ++layer) {
       const int cur = (layer & 1) * 2;
       const int prev = 2 - cur;
+      const DP Pc = P + (layer & 1) * 2 * ds;
       bool imp = false;
       const bool check = layer == L - 1 && layer > 0;
       if (layer > 0) {
@@ -1150,12 +1168,13 @@ struct SweepG {
           CT cc[WX];
           float t[WX];
           load_ct(Sx, layer * Sx.lstride + o, cc, t);
-          const DP D0 = D + prev * ds, D1 = D + (prev + 1) * ds;
+          const DP Pp = P + ((layer - 1) & 1) * 2 * ds;
           double d0[WX], d1[WX];
 #pragma unroll
           for (int j = 0; j < WX; ++j) {
-            d0[j] = rget(D0, oX[j]);
-            d1[j] = rget(D1, oX[j]);
+            const dpair pr = pget(Pp, oX[j]);
+            d0[j] = pr.x;
+            d1[j] = pr.y;
           }
           double best = PGC_INF;
           uint32_t bo = (uint32_t)npl << 3;
@@ -1180,6 +1199,7 @@ struct SweepG {
               imp = true;
             }
             dput(D + cur * ds + v, res);
+            dput(Pc + 2 * v, res);
           }
         }
         sync();
@@ -1211,6 +1231,9 @@ struct SweepG {
               imp = true;
             }
             dput(D + (cur + 1) * ds + v, res);
+            dput(Pc + 2 * v + 1, res);
+            // layer 0's depth-0 row was set before the sweep
+            if (layer == 0) dput(Pc + 2 * v, dget(D + cur * ds + v));
           }
         }
       } else if (act) {
