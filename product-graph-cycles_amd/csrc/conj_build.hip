@@ -167,27 +167,36 @@ __device__ void rotation_quat(V3 v2, V3 n2, V3 c2, V3 v3, V3 n3, V3 c3, double q
   q[3] = sg * z / n;
 }
 
-__global__ void __launch_bounds__(128) k_conj_quat(const double* __restrict__ VX, const double* __restrict__ NX,
-                            const double* __restrict__ VY, const double* __restrict__ NY,
-                            const int32_t* __restrict__ EX, int64_t nEX, const int32_t* __restrict__ EY,
-                            int64_t nEY, int prune, double* __restrict__ quat) {
-  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// A node's rotation depends only on its Y edge and its X edge: node j of
+// layer i uses Y edge i (i - 1 for the 2D-degenerate nodes [|EX|, 2|EX|)
+// when unpruned) and X edge j mod |EX|. So one rotation per (Y edge, X edge)
+// pair is computed, quat[ie * |EX| + ex], a third of the nodes, and the energy
+// looks a node's rotation up with quat_index.
+__device__ __forceinline__ int64_t quat_index(int64_t node, int64_t nEX, int64_t nEY, int prune) {
   const int64_t npl = 3 * nEX;
-  if (node >= npl * nEY) return;
   const int64_t i = node / npl, j = node - i * npl;
   const bool prev2d = j >= nEX && j < 2 * nEX && !prune;
   const int64_t ie = prev2d ? (i + nEY - 1) % nEY : i;
+  return ie * nEX + (j - (j / nEX) * nEX);
+}
+
+__global__ void __launch_bounds__(128) k_conj_quat(const double* __restrict__ VX, const double* __restrict__ NX,
+                            const double* __restrict__ VY, const double* __restrict__ NY,
+                            const int32_t* __restrict__ EX, int64_t nEX, const int32_t* __restrict__ EY,
+                            int64_t nEY, double* __restrict__ quat) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nEX * nEY) return;
+  const int64_t ie = u / nEX, ex = u - ie * nEX;
   const V3 e2 = vsub(row3(VY, EY[2 * ie + 1]), row3(VY, EY[2 * ie]));
   const V3 n2 = row3(NY, EY[2 * ie]);
   const V3 c2 = vcross(e2, n2);
-  const int64_t ex = j - (j / nEX) * nEX;
   const V3 e3 = vsub(row3(VX, EX[2 * ex + 1]), row3(VX, EX[2 * ex]));
   const V3 na = row3(NX, EX[2 * ex]), nb = row3(NX, EX[2 * ex + 1]);
   const V3 n3 = {(na.x + nb.x) / 2, (na.y + nb.y) / 2, (na.z + nb.z) / 2};
   const V3 c3 = vcross(e3, n3);
   double q[4];
   rotation_quat(e2, n2, c2, e3, n3, c3, q);
-  for (int k = 0; k < 4; ++k) quat[4 * node + k] = q[k];
+  for (int k = 0; k < 4; ++k) quat[4 * u + k] = q[k];
 }
 
 __device__ __forceinline__ double rho(double x, double alpha, double c, double pwr) {
@@ -199,16 +208,17 @@ __device__ __forceinline__ double rho(double x, double alpha, double c, double p
 __global__ void k_conj_energy(const double* __restrict__ VX, const double* __restrict__ VY, int64_t nVY,
                               const int32_t* __restrict__ rows, const int32_t* __restrict__ SRC,
                               const int32_t* __restrict__ TRGT, int64_t E, const double* __restrict__ FD,
-                              const double* __restrict__ quat, int kind, int lineIntegral, double alpha, double c,
-                              double pwr, double maxSO3Val, double maxFeatDiff, double* __restrict__ out) {
+                              const double* __restrict__ quat, int64_t nEX, int64_t nEY, int prune, int kind,
+                              int lineIntegral, double alpha, double c, double pwr, double maxSO3Val,
+                              double maxFeatDiff, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= E) return;
   const int32_t* r = rows + 6 * i;
   const int64_t y0 = r[0], y1 = r[1], y2 = r[2], x0 = r[3], x1 = r[4], x2 = r[5];
   double so3 = 0.0;
   if (quat) {
-    const double* a = quat + 4 * (int64_t)SRC[i];
-    const double* b = quat + 4 * (int64_t)TRGT[i];
+    const double* a = quat + 4 * quat_index(SRC[i], nEX, nEY, prune);
+    const double* b = quat + 4 * quat_index(TRGT[i], nEX, nEY, prune);
     double ip = fabs(a[0] * b[0] + (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]));
     if (ip > 1) ip = 1;
     so3 = isnan(ip) ? 0.0 : 2 * acos(ip);
@@ -273,20 +283,22 @@ hipError_t launch_conj_product_space(const int32_t* EX, int64_t nEX, const int32
 hipError_t launch_conj_quat(const double* VX, const double* NX, const double* VY, const double* NY,
                             const int32_t* EX, int64_t nEX, const int32_t* EY, int64_t nEY, int prune, double* quat,
                             hipStream_t st) {
-  const int64_t n = 3 * nEX * nEY;
+  (void)prune;
+  const int64_t n = nEX * nEY;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_conj_quat, dim3(nblk(n, 128)), dim3(128), 0, st, VX, NX, VY, NY, EX, nEX, EY, nEY, prune,
+  hipLaunchKernelGGL(k_conj_quat, dim3(nblk(n, 128)), dim3(128), 0, st, VX, NX, VY, NY, EX, nEX, EY, nEY,
                      quat);
   return hipGetLastError();
 }
 
 hipError_t launch_conj_energy(const double* VX, const double* VY, int64_t nVY, const int32_t* rows,
                               const int32_t* SRC, const int32_t* TRGT, int64_t E, const double* FD,
-                              const double* quat, int kind, int lineIntegral, double alpha, double c, double pwr,
-                              double maxSO3Val, double maxFeatDiff, double* out, hipStream_t st) {
+                              const double* quat, int64_t nEX, int64_t nEY, int prune, int kind, int lineIntegral,
+                              double alpha, double c, double pwr, double maxSO3Val, double maxFeatDiff, double* out,
+                              hipStream_t st) {
   if (E == 0) return hipSuccess;
   hipLaunchKernelGGL(k_conj_energy, dim3(nblk(E, 256)), dim3(256), 0, st, VX, VY, nVY, rows, SRC, TRGT, E, FD,
-                     quat, kind, lineIntegral, alpha, c, pwr, maxSO3Val, maxFeatDiff, out);
+                     quat, nEX, nEY, prune, kind, lineIntegral, alpha, c, pwr, maxSO3Val, maxFeatDiff, out);
   return hipGetLastError();
 }
 

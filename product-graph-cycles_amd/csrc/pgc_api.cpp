@@ -74,8 +74,8 @@ hipError_t launch_conj_product_space(const int32_t*, int64_t, const int32_t*, in
 hipError_t launch_conj_quat(const double*, const double*, const double*, const double*, const int32_t*, int64_t,
                             const int32_t*, int64_t, int, double*, hipStream_t);
 hipError_t launch_conj_energy(const double*, const double*, int64_t, const int32_t*, const int32_t*, const int32_t*,
-                              int64_t, const double*, const double*, int, int, double, double, double, double, double,
-                              double*, hipStream_t);
+                              int64_t, const double*, const double*, int64_t, int64_t, int, int, int, double, double,
+                              double, double, double, double*, hipStream_t);
 hipError_t launch_max_f64(const double*, int64_t, double*, hipStream_t);
 hipError_t launch_cycle_grad(const CycleItem*, int, hipStream_t);
 hipError_t launch_src_invariant(const int32_t*, int64_t, int64_t, int64_t, int32_t*, hipStream_t);
@@ -1269,7 +1269,7 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   HIPCHK(launch_conj_product_space(dEX, nEX, dEY, nEY, prune, dOff, dList, dPre, perLayer, intraTotal, g->d_rows,
                                    g->d_src, g->d_trgt, st));
   if (regularising) {
-    HIPCHK(dalloc(&dQuat, 4 * 3 * nEX * nEY));
+    HIPCHK(dalloc(&dQuat, 4 * nEX * nEY));  // one rotation per (Y edge, X edge) pair
     HIPCHK(launch_conj_quat(dVX, dNX, dVY, dNY, dEX, nEX, dEY, nEY, prune, dQuat, st));
   }
   // maxFeatDiff and maxSO3Val are float in the reference (:211-215)
@@ -1284,8 +1284,9 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   }
   const float maxSO3Val = (float)robust_loss(2 * M_PI, rl_alpha, rl_c, rl_pwr);
   const int lineIntegral = (cm == CM_LENGTH || tm == CM_LENGTH) ? 1 : 0;
-  HIPCHK(launch_conj_energy(dVX, dVY, nVY, g->d_rows, g->d_src, g->d_trgt, E, dFD, dQuat, kind, lineIntegral,
-                            rl_alpha, rl_c, rl_pwr, (double)maxSO3Val, (double)maxFeatDiff, g->d_energy, st));
+  HIPCHK(launch_conj_energy(dVX, dVY, nVY, g->d_rows, g->d_src, g->d_trgt, E, dFD, dQuat, nEX, nEY, prune, kind,
+                            lineIntegral, rl_alpha, rl_c, rl_pwr, (double)maxSO3Val, (double)maxFeatDiff, g->d_energy,
+                            st));
   HIPCHK(hipStreamSynchronize(st));
   for (void* p : {(void*)dOff, (void*)dList, (void*)dPre, (void*)dQuat, (void*)dMax})
     if (p) cache_free(p);
