@@ -30,48 +30,73 @@ namespace pgc {
 
 namespace {
 
+// The product space in two passes. Every layer has the same edge pattern
+// (layer i's edges are [i perLayer, (i + 1) perLayer)), so the first pass
+// lays out one layer's edges as descriptors (X edge j, out-edge it, kind), one
+// thread per X edge j in the order above; the second writes every edge of
+// every layer from its descriptor and its layer's Y edges, one thread per
+// edge, so the rows, sources and targets are written contiguously.
+// Kinds: 0, 1 intralayer (per (j, it) pair); 2, 3, 4 interlayer (per pair;
+// 3 only unpruned); 5, 6 the two interlayer edges closing j's block.
+__global__ void k_conj_desc(const int32_t* __restrict__ EX, int64_t nEX, int prune,
+                            const int32_t* __restrict__ outOff, const int32_t* __restrict__ outList,
+                            const int64_t* __restrict__ pre, int64_t intraTotal, int2* __restrict__ desc) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nEX) return;
+  const int s3 = EX[2 * j], m3 = EX[2 * j + 1];
+  int64_t a = 2 * pre[j];                                      // intralayer slots of j
+  int64_t b = intraTotal + (prune ? 2 : 3) * pre[j] + 2 * j;  // interlayer slots of j
+  const int jj = (int)j;
+  for (int k = outOff[m3]; k < outOff[m3 + 1]; ++k) {
+    const int it = outList[k];
+    if (EX[2 * it + 1] == s3) continue;
+    if (!prune) {
+      desc[a++] = make_int2(jj | (0 << 28), it);
+      desc[a++] = make_int2(jj | (1 << 28), it);
+    }
+    desc[b++] = make_int2(jj | (2 << 28), it);
+    if (!prune) desc[b++] = make_int2(jj | (3 << 28), it);
+    desc[b++] = make_int2(jj | (4 << 28), it);
+  }
+  desc[b++] = make_int2(jj | (5 << 28), -1);
+  desc[b++] = make_int2(jj | (6 << 28), -1);
+}
+
 __global__ void k_conj_product_space(const int32_t* __restrict__ EX, int64_t nEX, const int32_t* __restrict__ EY,
-                                     int64_t nEY, int prune, const int32_t* __restrict__ outOff,
-                                     const int32_t* __restrict__ outList, const int64_t* __restrict__ pre,
-                                     int64_t perLayer, int64_t intraTotal, int32_t* __restrict__ rows,
-                                     int32_t* __restrict__ src, int32_t* __restrict__ trgt) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nEY * nEX) return;
-  const int64_t i = g / nEX, j = g - i * nEX;
+                                     int64_t nEY, int prune, const int2* __restrict__ desc, int64_t perLayer,
+                                     int32_t* __restrict__ rows, int32_t* __restrict__ src,
+                                     int32_t* __restrict__ trgt) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEY * perLayer) return;
+  const int64_t i = e / perLayer, q = e - i * perLayer;
   const int64_t nx = (i + 1) % nEY;
   const int64_t npl = 3 * nEX;
   const int64_t cur = i * npl, nxt = nx * npl;
   const int64_t off2d = nEX, off3d = prune ? nEX : 2 * nEX;
   const int s2 = EY[2 * i], m2 = EY[2 * i + 1], t2 = EY[2 * nx + 1];
+  const int2 d = desc[q];
+  const int64_t j = d.x & ((1 << 28) - 1);
+  const int kind = d.x >> 28;
+  const int it = d.y;
   const int s3 = EX[2 * j], m3 = EX[2 * j + 1];
-  const int64_t base = i * perLayer;
-  int64_t a = base + 2 * pre[j];                                  // intralayer slots of j
-  int64_t b = base + intraTotal + (prune ? 2 : 3) * pre[j] + 2 * j;  // interlayer slots of j
-  auto emit = [&](int64_t e, int r0, int r1, int r2, int r3, int r4, int r5, int64_t s, int64_t t) {
-    int32_t* r = rows + 6 * e;
-    r[0] = r0;
-    r[1] = r1;
-    r[2] = r2;
-    r[3] = r3;
-    r[4] = r4;
-    r[5] = r5;
-    src[e] = (int32_t)s;
-    trgt[e] = (int32_t)t;
-  };
-  for (int k = outOff[m3]; k < outOff[m3 + 1]; ++k) {
-    const int it = outList[k];
-    const int t3 = EX[2 * it + 1];
-    if (t3 == s3) continue;
-    if (!prune) {
-      emit(a++, s2, s2, s2, s3, m3, t3, cur + off2d + j, cur + off2d + it);
-      emit(a++, s2, s2, m2, s3, m3, t3, cur + off2d + j, cur + it);
-    }
-    emit(b++, s2, m2, t2, s3, m3, t3, cur + j, nxt + it);
-    if (!prune) emit(b++, s2, m2, m2, s3, m3, t3, cur + j, nxt + off2d + it);
-    emit(b++, s2, m2, t2, m3, m3, t3, cur + off3d + j, nxt + it);
+  const int t3 = it >= 0 ? EX[2 * it + 1] : m3;
+  int r[6];
+  int64_t sv, tv;
+  switch (kind) {
+    case 0: r[0] = s2; r[1] = s2; r[2] = s2; r[3] = s3; r[4] = m3; r[5] = t3; sv = cur + off2d + j; tv = cur + off2d + it; break;
+    case 1: r[0] = s2; r[1] = s2; r[2] = m2; r[3] = s3; r[4] = m3; r[5] = t3; sv = cur + off2d + j; tv = cur + it; break;
+    case 2: r[0] = s2; r[1] = m2; r[2] = t2; r[3] = s3; r[4] = m3; r[5] = t3; sv = cur + j; tv = nxt + it; break;
+    case 3: r[0] = s2; r[1] = m2; r[2] = m2; r[3] = s3; r[4] = m3; r[5] = t3; sv = cur + j; tv = nxt + off2d + it; break;
+    case 4: r[0] = s2; r[1] = m2; r[2] = t2; r[3] = m3; r[4] = m3; r[5] = t3; sv = cur + off3d + j; tv = nxt + it; break;
+    case 5: r[0] = s2; r[1] = m2; r[2] = t2; r[3] = s3; r[4] = m3; r[5] = m3; sv = cur + j; tv = nxt + off3d + j; break;
+    default: r[0] = s2; r[1] = m2; r[2] = t2; r[3] = m3; r[4] = m3; r[5] = m3; sv = cur + off3d + j; tv = nxt + off3d + j; break;
   }
-  emit(b++, s2, m2, t2, s3, m3, m3, cur + j, nxt + off3d + j);
-  emit(b++, s2, m2, t2, m3, m3, m3, cur + off3d + j, nxt + off3d + j);
+  int2* rw = (int2*)(rows + 6 * e);  // 8-byte aligned: 24 e bytes
+  rw[0] = make_int2(r[0], r[1]);
+  rw[1] = make_int2(r[2], r[3]);
+  rw[2] = make_int2(r[4], r[5]);
+  src[e] = (int32_t)sv;
+  trgt[e] = (int32_t)tv;
 }
 
 struct V3 {
@@ -271,12 +296,14 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_conj_product_space(const int32_t* EX, int64_t nEX, const int32_t* EY, int64_t nEY, int prune,
                                      const int32_t* outOff, const int32_t* outList, const int64_t* pre,
-                                     int64_t perLayer, int64_t intraTotal, int32_t* rows, int32_t* src,
+                                     int64_t perLayer, int64_t intraTotal, int2* desc, int32_t* rows, int32_t* src,
                                      int32_t* trgt, hipStream_t st) {
-  const int64_t n = nEX * nEY;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_conj_product_space, dim3(nblk(n, 256)), dim3(256), 0, st, EX, nEX, EY, nEY, prune, outOff,
-                     outList, pre, perLayer, intraTotal, rows, src, trgt);
+  const int64_t n = nEY * perLayer;
+  if (n == 0 || nEX == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_conj_desc, dim3(nblk(nEX, 256)), dim3(256), 0, st, EX, nEX, prune, outOff, outList, pre,
+                     intraTotal, desc);
+  hipLaunchKernelGGL(k_conj_product_space, dim3(nblk(n, 256)), dim3(256), 0, st, EX, nEX, EY, nEY, prune, desc,
+                     perLayer, rows, src, trgt);
   return hipGetLastError();
 }
 

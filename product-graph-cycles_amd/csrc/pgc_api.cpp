@@ -69,8 +69,8 @@ std::mutex& scratch_call_lock(int device);
 hipError_t launch_cycle_value(const CycleItem*, int, double*, hipStream_t);
 // conj_build.hip
 hipError_t launch_conj_product_space(const int32_t*, int64_t, const int32_t*, int64_t, int, const int32_t*,
-                                     const int32_t*, const int64_t*, int64_t, int64_t, int32_t*, int32_t*, int32_t*,
-                                     hipStream_t);
+                                     const int32_t*, const int64_t*, int64_t, int64_t, int2*, int32_t*, int32_t*,
+                                     int32_t*, hipStream_t);
 hipError_t launch_conj_quat(const double*, const double*, const double*, const double*, const int32_t*, int64_t,
                             const int32_t*, int64_t, int, double*, hipStream_t);
 hipError_t launch_conj_energy(const double*, const double*, int64_t, const int32_t*, const int32_t*, const int32_t*,
@@ -1266,8 +1266,10 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   HIPCHK(hipMemcpyAsync(dOff, outOff.data(), (nVX + 1) * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dList, outList.data(), nEX * 4, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dPre, pre.data(), (nEX + 1) * 8, hipMemcpyHostToDevice, st));
-  HIPCHK(launch_conj_product_space(dEX, nEX, dEY, nEY, prune, dOff, dList, dPre, perLayer, intraTotal, g->d_rows,
-                                   g->d_src, g->d_trgt, st));
+  int2* dDesc = nullptr;  // one layer's edge descriptors
+  HIPCHK(cache_malloc((void**)&dDesc, (size_t)perLayer * sizeof(int2)));
+  HIPCHK(launch_conj_product_space(dEX, nEX, dEY, nEY, prune, dOff, dList, dPre, perLayer, intraTotal, dDesc,
+                                   g->d_rows, g->d_src, g->d_trgt, st));
   if (regularising) {
     HIPCHK(dalloc(&dQuat, 4 * nEX * nEY));  // one rotation per (Y edge, X edge) pair
     HIPCHK(launch_conj_quat(dVX, dNX, dVY, dNY, dEX, nEX, dEY, nEY, prune, dQuat, st));
@@ -1288,7 +1290,7 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
                             lineIntegral, rl_alpha, rl_c, rl_pwr, (double)maxSO3Val, (double)maxFeatDiff, g->d_energy,
                             st));
   HIPCHK(hipStreamSynchronize(st));
-  for (void* p : {(void*)dOff, (void*)dList, (void*)dPre, (void*)dQuat, (void*)dMax})
+  for (void* p : {(void*)dOff, (void*)dList, (void*)dPre, (void*)dQuat, (void*)dMax, (void*)dDesc})
     if (p) cache_free(p);
   // numNodes = max(SRC) + 1, numLayers = numNodes / npl as run() derives them
   int rc = finish_graph_meta(g.get());
