@@ -843,9 +843,12 @@ __global__ void k_xreplicate(const int32_t* __restrict__ ts, const int32_t* __re
 template <typename CT>
 __global__ void k_fill_costs(const int32_t* __restrict__ eid, const int32_t* __restrict__ s, int64_t nSlots,
                              const double* __restrict__ energy, int64_t E, int ncols, CT* __restrict__ cost,
-                             float* __restrict__ time) {
+                             float* __restrict__ time, int wmask, int deg) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nSlots) return;
+  // slots at or past the class's largest in-degree are padding on every node;
+  // the caller zeroed them when it allocated the buffers
+  if ((int)((i >> 6) & wmask) >= deg) return;
   // padding slots are the ones without an edge (their source is the
   // sentinel npl, never negative once k_pad_src has run)
   const int64_t e = eid[i];
@@ -1087,15 +1090,20 @@ hipError_t launch_pad_src(int32_t* s, int64_t n, int32_t npl, hipStream_t st) {
   return hipGetLastError();
 }
 
+// W: the layout's slot width when it is a power of two (chunk rows of W x 64
+// slots), with deg the class's largest in-degree: the fill skips the slots
+// j >= deg of every node, which the caller zeroed; W = 0 fills every slot.
 hipError_t launch_fill_costs(const int32_t* eid, const int32_t* s, int64_t nSlots, const double* energy, int64_t E,
-                             int ncols, bool asDouble, void* cost, float* time, hipStream_t st) {
+                             int ncols, bool asDouble, void* cost, float* time, int W, int deg, hipStream_t st) {
   if (nSlots == 0) return hipSuccess;
+  const bool skip = W > 0 && (W & (W - 1)) == 0 && deg >= 0 && deg < W;
+  const int wmask = skip ? W - 1 : 0, d = skip ? deg : 1;
   if (asDouble)
     hipLaunchKernelGGL(k_fill_costs<double>, dim3(nb(nSlots, 256)), dim3(256), 0, st, eid, s, nSlots, energy, E,
-                       ncols, (double*)cost, time);
+                       ncols, (double*)cost, time, wmask, d);
   else
     hipLaunchKernelGGL(k_fill_costs<float>, dim3(nb(nSlots, 256)), dim3(256), 0, st, eid, s, nSlots, energy, E,
-                       ncols, (float*)cost, time);
+                       ncols, (float*)cost, time, wmask, d);
   return hipGetLastError();
 }
 

@@ -50,7 +50,7 @@ hipError_t launch_layout_width(const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_layout_fill(const int32_t*, const int32_t*, int64_t, int64_t, int64_t, int, int32_t*,
                               const int32_t*, const int32_t*, int32_t*, int32_t*, int32_t*, int32_t*, hipStream_t);
 hipError_t launch_fill_costs(const int32_t*, const int32_t*, int64_t, const double*, int64_t, int, bool, void*,
-                             float*, hipStream_t);
+                             float*, int, int, hipStream_t);
 hipError_t launch_gather_rows(const int32_t*, int, const int32_t*, int64_t, int32_t*, hipStream_t);
 hipError_t launch_gather_strided(const int32_t*, int64_t, int64_t, int32_t*, hipStream_t);
 hipError_t launch_int_max(const int32_t*, int64_t, int32_t*, hipStream_t);
@@ -800,8 +800,11 @@ int fill_costs(pgc_graph* g, int kind) {
   hipStream_t st = g->ctx->stream;
   for (int c = 0; c < 2; ++c) {
     const bool withTime = kind == 1 && g->ncols > 1;
+    // the slots past the class's largest in-degree keep the zeros of the
+    // buffers' allocation (a graph's layout does not change)
     HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, g->ncols, kind == 2,
-                             g->d_cost[c], withTime ? g->d_time[c] : nullptr, st));
+                             g->d_cost[c], withTime ? g->d_time[c] : nullptr, g->width[c] <= 32 ? g->width[c] : 0,
+                             g->cls_deg[c], st));
   }
   return PGC_OK;
 }
@@ -2094,7 +2097,7 @@ int pgc_solve_ratio_noalloc(pgc_context* ctx, pgc_graph* g, int K, uint32_t flag
   for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, sel, nullptr, st));
   for (int c = 0; c < 2; ++c)
     HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, g->ncols, false, g->d_cost[c],
-                             g->ncols > 1 ? g->d_time[c] : nullptr, st));
+                             g->ncols > 1 ? g->d_time[c] : nullptr, 0, 0, st));
   // the complete solve in one launch: upper bound, branch and bound, output
   SolverJob job = make_job(g, K, MODE_FULL_RATIO, flags | FLAG_FRESH_JOB);
   job.vc_cap = s.vc_cap;
