@@ -470,18 +470,24 @@ void pgc_context_destroy(pgc_context* c) {
 
 namespace {
 
-int finish_graph_meta(pgc_graph* g) {
+// maxSrc: max(SRC) when the caller knows it in closed form, else -2 (a
+// reduction over SRC)
+int finish_graph_meta(pgc_graph* g, int64_t maxSrc = -2) {
   // numNodes = max(SRC) + 1, numLayers = numNodes / npl (run(), :870-871)
-  int32_t* d_max = nullptr;
-  HIPCHK(dalloc(&d_max, 1));
-  const int32_t init = -1;
-  HIPCHK(hipMemcpyAsync(d_max, &init, 4, hipMemcpyHostToDevice, g->ctx->stream));
-  HIPCHK(launch_int_max(g->d_src, g->E, d_max, g->ctx->stream));
-  int32_t mx = -1;
-  HIPCHK(hipMemcpyAsync(&mx, d_max, 4, hipMemcpyDeviceToHost, g->ctx->stream));
-  HIPCHK(hipStreamSynchronize(g->ctx->stream));
-  cache_free(d_max);
-  g->numNodes = (int64_t)mx + 1;
+  int64_t mx = maxSrc;
+  if (mx == -2) {
+    int32_t* d_max = nullptr;
+    HIPCHK(dalloc(&d_max, 1));
+    const int32_t init = -1;
+    HIPCHK(hipMemcpyAsync(d_max, &init, 4, hipMemcpyHostToDevice, g->ctx->stream));
+    HIPCHK(launch_int_max(g->d_src, g->E, d_max, g->ctx->stream));
+    int32_t m = -1;
+    HIPCHK(hipMemcpyAsync(&m, d_max, 4, hipMemcpyDeviceToHost, g->ctx->stream));
+    HIPCHK(hipStreamSynchronize(g->ctx->stream));
+    cache_free(d_max);
+    mx = m;
+  }
+  g->numNodes = mx + 1;
   g->L = g->numNodes / g->npl;
   if (g->L < 1) return fail(PGC_ERR_ARG, "product graph has fewer nodes than one layer");
   if (g->numNodes % g->npl != 0)
@@ -1295,8 +1301,12 @@ int generate_conj(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   HIPCHK(hipStreamSynchronize(st));
   for (void* p : {(void*)dOff, (void*)dList, (void*)dPre, (void*)dQuat, (void*)dMax, (void*)dDesc})
     if (p) cache_free(p);
-  // numNodes = max(SRC) + 1, numLayers = numNodes / npl as run() derives them
-  int rc = finish_graph_meta(g.get());
+  // numNodes = max(SRC) + 1, numLayers = numNodes / npl as run() derives them.
+  // max(SRC) in closed form: the last closing edge of the last layer starts
+  // at the 3D-degenerate node of the last X edge, off3d + |EX| - 1 in that
+  // layer (off3d = 2|EX|, or |EX| when pruned)
+  const int64_t maxSrc = E > 0 ? (nEY - 1) * 3 * nEX + (prune ? 2 : 3) * nEX - 1 : -1;
+  int rc = finish_graph_meta(g.get(), maxSrc);
   if (rc && rc != PGC_ERR_UNSUPPORTED) return rc;
   g->rep_B = perLayer;
   // the branch graph is built at its first use (build_conj_bg)

@@ -46,6 +46,10 @@ def test_conj_graph_construction(reg, cost, time, prune):
     g = _gen(p, reg, cost, time, prune)
     rows, src, trgt = O.conj_product_space(p["EX"], p["EY"], prune=prune)
     assert np.array_equal(g.get_product_space(), rows)
+    # numNodes = max(SRC) + 1 (taken in closed form for conjugate graphs):
+    # numLayers = numNodes // npl, the reference's integer division
+    info = g._graph.info()
+    assert info["n_layers"] == (int(src.max()) + 1) // (3 * len(p["EX"]))
     en = O.conj_energy(p["VX"], p["NX"], p["VY"], p["NY"], p["EX"], p["EY"], rows, src, trgt,
                        p["feat_diff"].astype(np.float64), reg=reg, cost_mode=cost, time_mode=time, prune=prune)
     got = g.get_cost_vector()
