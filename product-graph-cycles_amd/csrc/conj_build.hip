@@ -68,8 +68,9 @@ __global__ void k_conj_product_space(const int32_t* __restrict__ EX, int64_t nEX
                                      int32_t* __restrict__ trgt) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nEY * perLayer) return;
-  const int64_t i = e / perLayer, q = e - i * perLayer;
-  const int64_t nx = (i + 1) % nEY;
+  // edge ids are below 2^31: 32-bit division
+  const int64_t i = (uint32_t)e / (uint32_t)perLayer, q = e - i * perLayer;
+  const int64_t nx = i + 1 == nEY ? 0 : i + 1;
   const int64_t npl = 3 * nEX;
   const int64_t cur = i * npl, nxt = nx * npl;
   const int64_t off2d = nEX, off3d = prune ? nEX : 2 * nEX;
@@ -197,12 +198,16 @@ __device__ void rotation_quat(V3 v2, V3 n2, V3 c2, V3 v3, V3 n3, V3 c3, double q
 // when unpruned) and X edge j mod |EX|. So one rotation per (Y edge, X edge)
 // pair is computed, quat[ie * |EX| + ex], a third of the nodes, and the energy
 // looks a node's rotation up with quat_index.
-__device__ __forceinline__ int64_t quat_index(int64_t node, int64_t nEX, int64_t nEY, int prune) {
-  const int64_t npl = 3 * nEX;
-  const int64_t i = node / npl, j = node - i * npl;
-  const bool prev2d = j >= nEX && j < 2 * nEX && !prune;
-  const int64_t ie = prev2d ? (i + nEY - 1) % nEY : i;
-  return ie * nEX + (j - (j / nEX) * nEX);
+// (node ids are below 2^31; the layer is one 32-bit division, the rest
+// compares)
+__device__ __forceinline__ int64_t quat_index(int32_t node, int32_t nEX, int32_t nEY, int prune) {
+  const uint32_t npl = 3u * (uint32_t)nEX;
+  const uint32_t i = (uint32_t)node / npl, j = (uint32_t)node - i * npl;
+  const uint32_t n = (uint32_t)nEX;
+  const bool prev2d = j >= n && j < 2 * n && !prune;
+  const uint32_t ie = prev2d ? (i == 0 ? (uint32_t)nEY - 1 : i - 1) : i;
+  const uint32_t ex = j >= 2 * n ? j - 2 * n : (j >= n ? j - n : j);
+  return (int64_t)ie * nEX + ex;
 }
 
 __global__ void __launch_bounds__(128) k_conj_quat(const double* __restrict__ VX, const double* __restrict__ NX,
@@ -242,8 +247,8 @@ __global__ void k_conj_energy(const double* __restrict__ VX, const double* __res
   const int64_t y0 = r[0], y1 = r[1], y2 = r[2], x0 = r[3], x1 = r[4], x2 = r[5];
   double so3 = 0.0;
   if (quat) {
-    const double* a = quat + 4 * quat_index(SRC[i], nEX, nEY, prune);
-    const double* b = quat + 4 * quat_index(TRGT[i], nEX, nEY, prune);
+    const double* a = quat + 4 * quat_index(SRC[i], (int32_t)nEX, (int32_t)nEY, prune);
+    const double* b = quat + 4 * quat_index(TRGT[i], (int32_t)nEX, (int32_t)nEY, prune);
     double ip = fabs(a[0] * b[0] + (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]));
     if (ip > 1) ip = 1;
     so3 = isnan(ip) ? 0.0 : 2 * acos(ip);
