@@ -313,6 +313,9 @@ __global__ void __launch_bounds__(256) k_select_hist(const double* __restrict__ 
   __syncthreads();
   const int shift = 56 - 8 * pass;
   const int lane = threadIdx.x & 63;
+  // both ranks on one prefix (every pass for an odd count, the early ones
+  // for an even one): count the first histogram and copy it
+  const bool same = pre[0] == pre[1] && msk[0] == msk[1];
   const int64_t step = (int64_t)gridDim.x * blockDim.x * kSelUnroll;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x * kSelUnroll + threadIdx.x; base < n; base += step) {
     double v[kSelUnroll];
@@ -326,7 +329,7 @@ __global__ void __launch_bounds__(256) k_select_hist(const double* __restrict__ 
       if (base + (int64_t)u * blockDim.x >= n) continue;
       const uint64_t kk = dkey(v[u]);
       const uint32_t b = (uint32_t)(kk >> shift) & 255u;
-      const bool in0 = (kk & msk[0]) == pre[0], in1 = (kk & msk[1]) == pre[1];
+      const bool in0 = (kk & msk[0]) == pre[0], in1 = !same && (kk & msk[1]) == pre[1];
       const uint64_t act = __ballot(1);
       const uint32_t bf = (uint32_t)__builtin_amdgcn_readfirstlane((int)b);
       if (__ballot(b == bf) == act) {
@@ -342,6 +345,7 @@ __global__ void __launch_bounds__(256) k_select_hist(const double* __restrict__ 
     }
   }
   __syncthreads();
+  if (same) h[1][threadIdx.x] = h[0][threadIdx.x];
   if (h[0][threadIdx.x]) atomicAdd(&st->hist[pass][0][threadIdx.x], h[0][threadIdx.x]);
   if (h[1][threadIdx.x]) atomicAdd(&st->hist[pass][1][threadIdx.x], h[1][threadIdx.x]);
 }
@@ -930,7 +934,9 @@ hipError_t launch_quantize(double* col, int64_t n, void* selectState, double* sc
   const int64_t k1 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
   const int64_t k2 = (n % 2 == 0) ? k1 + 1 : k1;
   hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, st, S, k1, k2);
-  const unsigned gh = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 256 * kSelUnroll - 1) / (256 * kSelUnroll)));
+  // at most 512 workgroups: each adds its partial histograms with global
+  // atomics on the same 512 counters
+  const unsigned gh = (unsigned)std::min<int64_t>(512, std::max<int64_t>(1, (n + 256 * kSelUnroll - 1) / (256 * kSelUnroll)));
   for (int pass = 0; pass < 8; ++pass) hipLaunchKernelGGL(k_select_hist, dim3(gh), dim3(256), 0, st, col, n, S, pass);
   const unsigned g = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 255) / 256));
   hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, st, col, n, S, (int)(n % 2 == 0), 100000.0, scale_out);
