@@ -268,6 +268,9 @@ struct pgc_context {
   int64_t bracket_cap = 0;
   double* d_cand = nullptr;  // and the keys gathered inside the brackets
   int64_t cand_cap = 0;
+  // pinned host staging of small copies (grow-only, see stage())
+  uint8_t* pinned = nullptr;
+  size_t pinned_cap = 0;
 };
 
 namespace {
@@ -462,6 +465,7 @@ void pgc_context_destroy(pgc_context* c) {
     if (c->join[k]) (void)hipEventDestroy(c->join[k]);
   }
   if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -469,6 +473,65 @@ void pgc_context_destroy(pgc_context* c) {
 }  // extern "C"
 
 namespace {
+
+// Pinned host staging of the small copies of a solve (job upload, result,
+// output cycle, the X edge list): a copy from or to pageable memory goes
+// through the runtime's own staging and costs tens of microseconds even for
+// a few bytes. The context's buffer has an upload half [0, kStageUp) and a
+// download half after it; every caller synchronizes the stream before the
+// buffer is written again. nullptr: larger than the buffer may grow (the
+// caller copies directly).
+constexpr size_t kStageUp = (size_t)64 << 10;
+constexpr size_t kStageMax = (size_t)8 << 20;
+uint8_t* stage(pgc_context* c, size_t down) {
+  if (down > kStageMax) return nullptr;
+  const size_t need = kStageUp + down;
+  if (c->pinned_cap < need) {
+    // an upload from the old buffer may still be queued
+    if (c->pinned && hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_cap = 0;
+    const size_t cap = std::max(need, kStageUp + ((size_t)1 << 20));
+    void* p = nullptr;
+    if (hipHostMalloc(&p, cap, hipHostMallocDefault) != hipSuccess) return nullptr;
+    c->pinned = (uint8_t*)p;
+    c->pinned_cap = cap;
+  }
+  return c->pinned;
+}
+
+// Upload of a small host struct (through the upload half when it fits).
+int h2d_small(pgc_context* c, void* dst, const void* src, size_t bytes) {
+  uint8_t* p = bytes <= kStageUp ? stage(c, 0) : nullptr;
+  if (p) {
+    std::memcpy(p, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  return PGC_OK;
+}
+
+// Downloads of up to two device buffers and one stream synchronization.
+int d2h_sync(pgc_context* c, void* dst0, const void* src0, size_t b0, void* dst1 = nullptr,
+             const void* src1 = nullptr, size_t b1 = 0) {
+  const size_t a0 = (b0 + 255) & ~(size_t)255;
+  uint8_t* p = stage(c, a0 + b1);
+  if (!p) {
+    if (b0) HIPCHK(hipMemcpyAsync(dst0, src0, b0, hipMemcpyDeviceToHost, c->stream));
+    if (b1) HIPCHK(hipMemcpyAsync(dst1, src1, b1, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PGC_OK;
+  }
+  uint8_t* d = p + kStageUp;
+  if (b0) HIPCHK(hipMemcpyAsync(d, src0, b0, hipMemcpyDeviceToHost, c->stream));
+  if (b1) HIPCHK(hipMemcpyAsync(d + a0, src1, b1, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (b0) std::memcpy(dst0, d, b0);
+  if (b1) std::memcpy(dst1, d + a0, b1);
+  return PGC_OK;
+}
 
 // maxSrc: max(SRC) when the caller knows it in closed form, else -2 (a
 // reduction over SRC)
@@ -1091,7 +1154,7 @@ int grid_blocks(const pgc_graph* g, int K, bool mincost) {
 int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, double& kms) {
   pgc_context* c = g->ctx;
   hipStream_t st = c->stream;
-  HIPCHK(hipMemcpyAsync(g->js.djob, &j, sizeof(j), hipMemcpyHostToDevice, st));
+  if (int rc = h2d_small(c, g->js.djob, &j, sizeof(j))) return rc;
   const int maxc = kernel_maxc(g, j.K);
   const size_t lds = job_lds_bytes(g, j.K);
   const int nblk = j.grid ? grid_blocks(g, j.K, mincost) : 0;
@@ -1103,8 +1166,7 @@ int run_job(pgc_graph* g, const SolverJob& j, bool mincost, JobResult& out, doub
     HIPCHK(launch_solver(g->js.djob, 1, 1, nullptr, maxc, g->maxw, mincost, g->d_time[0] != nullptr,
                          g->cost_kind == 2, lds, st));
   HIPCHK(hipEventRecord(c->ev1, st));
-  HIPCHK(hipMemcpyAsync(&out, g->js.res, sizeof(out), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if (int rc = d2h_sync(c, &out, g->js.res, sizeof(out))) return rc;
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   kms += ms;
@@ -1387,8 +1449,7 @@ int generate_impl(pgc_context* ctx, const double* dVX, int64_t nVX, const int32_
   std::vector<int32_t> exh;
   if (!EX_host) {
     exh.resize(2 * nEX);
-    HIPCHK(hipMemcpyAsync(exh.data(), dEX, 2 * nEX * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    if (int rc = d2h_sync(ctx, exh.data(), dEX, (size_t)(2 * nEX * 4))) return rc;
     EX_host = exh.data();
   }
   int32_t mx = -1;
@@ -1875,14 +1936,13 @@ int ratio_bnb_device(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int3
   job.out_eid = d_out;
   job.out_cap = (int32_t)outCap;
   if (!pred_fits_lds(g, K)) HIPCHK(launch_fill_i32(s.pred, s.predN, -1, st));
-  HIPCHK(hipMemcpyAsync(s.djob, &job, sizeof(job), hipMemcpyHostToDevice, st));
+  if ((rc = h2d_small(ctx, s.djob, &job, sizeof(job)))) return rc;
   HIPCHK(hipEventRecord(ctx->ev0, st));
   HIPCHK(launch_solver(s.djob, 1, 1, nullptr, kernel_maxc(g, K), g->maxw, false, g->d_time[0] != nullptr, false,
                        job_lds_bytes(g, K), st));
   HIPCHK(hipEventRecord(ctx->ev1, st));
   JobResult r;
-  HIPCHK(hipMemcpyAsync(&r, s.res, sizeof(r), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  if ((rc = d2h_sync(ctx, &r, s.res, sizeof(r)))) return rc;
   float kms = 0;
   HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
   if (r.error) {
@@ -1896,13 +1956,13 @@ int ratio_bnb_device(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int3
     return fail(PGC_ERR_SOLVER, "solver error " + std::to_string(r.error));
   }
   const int64_t m = r.out_len;
-  if (edge_ids && m) HIPCHK(hipMemcpyAsync(edge_ids, d_out, m * 4, hipMemcpyDeviceToHost, st));
   if (rows && m) {
     HIPCHK(dalloc(&d_rows, m * g->ps_cols));
     HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, d_out, m, d_rows, st));
-    HIPCHK(hipMemcpyAsync(rows, d_rows, m * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
   }
-  HIPCHK(hipStreamSynchronize(st));
+  if ((rc = d2h_sync(ctx, edge_ids, d_out, edge_ids && m ? (size_t)m * 4 : 0, rows, d_rows,
+                     rows && m ? (size_t)(m * g->ps_cols * 4) : 0)))
+    return rc;
   cache_free(d_out);
   if (d_rows) cache_free(d_rows);
   std::memset(res, 0, sizeof(*res));

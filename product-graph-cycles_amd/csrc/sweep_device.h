@@ -1159,7 +1159,6 @@ struct SweepG {
     if (blockIdx.x == 0 && threadIdx.x < 4) dput(P + (threadIdx.x >> 1) * 2 * ds + 2 * npl + (threadIdx.x & 1), PGC_INF);
     for (int layer = 0; layer <= L; ++layer) {
       const int cur = (layer & 1) * 2;
-      const int prev = 2 - cur;
       const DP Pc = P + (layer & 1) * 2 * ds;
       bool imp = false;
       const bool check = layer == L - 1 && layer > 0;
