@@ -20,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <unordered_map>
+#include <initializer_list>
 #include <unordered_set>
 #include <string>
 #include <tuple>
@@ -501,36 +502,60 @@ uint8_t* stage(pgc_context* c, size_t down) {
   return c->pinned;
 }
 
-// Upload of a small host struct (through the upload half when it fits).
-int h2d_small(pgc_context* c, void* dst, const void* src, size_t bytes) {
-  uint8_t* p = bytes <= kStageUp ? stage(c, 0) : nullptr;
-  if (p) {
-    std::memcpy(p, src, bytes);
-    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, c->stream));
-  } else {
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+struct Xfer {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+
+// Uploads of small host buffers (through the upload half when they fit).
+int h2d_many(pgc_context* c, std::initializer_list<Xfer> xs) {
+  size_t tot = 0;
+  for (const Xfer& x : xs) tot += (x.bytes + 255) & ~(size_t)255;
+  uint8_t* p = tot <= kStageUp ? stage(c, 0) : nullptr;
+  size_t o = 0;
+  for (const Xfer& x : xs) {
+    if (!x.bytes) continue;
+    if (p) {
+      std::memcpy(p + o, x.src, x.bytes);
+      HIPCHK(hipMemcpyAsync(x.dst, p + o, x.bytes, hipMemcpyHostToDevice, c->stream));
+      o += (x.bytes + 255) & ~(size_t)255;
+    } else {
+      HIPCHK(hipMemcpyAsync(x.dst, x.src, x.bytes, hipMemcpyHostToDevice, c->stream));
+    }
   }
   return PGC_OK;
 }
 
-// Downloads of up to two device buffers and one stream synchronization.
+int h2d_small(pgc_context* c, void* dst, const void* src, size_t bytes) { return h2d_many(c, {{dst, src, bytes}}); }
+
+// Downloads of device buffers and one stream synchronization.
+int d2h_many(pgc_context* c, std::initializer_list<Xfer> xs) {
+  size_t tot = 0;
+  for (const Xfer& x : xs) tot += (x.bytes + 255) & ~(size_t)255;
+  uint8_t* p = stage(c, tot);
+  uint8_t* d = p ? p + kStageUp : nullptr;
+  size_t o = 0;
+  for (const Xfer& x : xs) {
+    if (!x.bytes) continue;
+    HIPCHK(hipMemcpyAsync(d ? d + o : x.dst, x.src, x.bytes, hipMemcpyDeviceToHost, c->stream));
+    o += (x.bytes + 255) & ~(size_t)255;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (d) {
+    o = 0;
+    for (const Xfer& x : xs) {
+      if (!x.bytes) continue;
+      std::memcpy(x.dst, d + o, x.bytes);
+      o += (x.bytes + 255) & ~(size_t)255;
+    }
+  }
+  return PGC_OK;
+}
+
 int d2h_sync(pgc_context* c, void* dst0, const void* src0, size_t b0, void* dst1 = nullptr,
              const void* src1 = nullptr, size_t b1 = 0) {
-  const size_t a0 = (b0 + 255) & ~(size_t)255;
-  uint8_t* p = stage(c, a0 + b1);
-  if (!p) {
-    if (b0) HIPCHK(hipMemcpyAsync(dst0, src0, b0, hipMemcpyDeviceToHost, c->stream));
-    if (b1) HIPCHK(hipMemcpyAsync(dst1, src1, b1, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    return PGC_OK;
-  }
-  uint8_t* d = p + kStageUp;
-  if (b0) HIPCHK(hipMemcpyAsync(d, src0, b0, hipMemcpyDeviceToHost, c->stream));
-  if (b1) HIPCHK(hipMemcpyAsync(d + a0, src1, b1, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (b0) std::memcpy(dst0, d, b0);
-  if (b1) std::memcpy(dst1, d + a0, b1);
-  return PGC_OK;
+  return d2h_many(c, {{dst0, src0, b0}, {dst1, src1, b1}});
 }
 
 // maxSrc: max(SRC) when the caller knows it in closed form, else -2 (a
@@ -2740,10 +2765,10 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
     sweeps += r.sweeps;
     bfc += r.bf_calls;
     if (r.status != ST_CAN_CONTAIN) continue;
-    HIPCHK(hipMemcpyAsync(keys.data(), s.lb_key, npl * 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(back.data(), s.back, npl * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(clen.data(), s.wlen, npl * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    if ((rc = d2h_many(ctx, {{keys.data(), s.lb_key, (size_t)npl * 8},
+                             {back.data(), s.back, (size_t)npl * 4},
+                             {clen.data(), s.wlen, (size_t)npl * 4}})))
+      return rc;
     // lower bounds in ascending (key, vertex) order (dijkstra_bnb_subroutine.cu:391-393)
     lbs.clear();
     bool anyFinite = false;
@@ -2780,8 +2805,7 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
             // column v of the chain table (stride npl), gathered on the device:
             // a strided copy of 4-byte rows costs the copy engine ~2 us per row
             HIPCHK(launch_gather_strided(s.chain + v, npl, clen[v], s.nodes_io, st));
-            HIPCHK(hipMemcpyAsync(cycle.data(), s.nodes_io, (size_t)clen[v] * 4, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
+            if ((rc = d2h_sync(ctx, cycle.data(), s.nodes_io, (size_t)clen[v] * 4))) return rc;
           }
         }
         break;
@@ -2871,11 +2895,9 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   if (cyc.size() > 1) {
     if ((int64_t)cyc.size() > s.cyc_cap) return fail(PGC_ERR_CAPACITY, "path longer than the cycle buffer");
     SolverJob lj = make_job(g, K, MODE_MIN_COST, flags);
-    HIPCHK(hipMemcpyAsync(s.djob, &lj, sizeof(lj), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(s.nodes_io, cyc.data(), cyc.size() * 4, hipMemcpyHostToDevice, st));
+    if ((rc = h2d_many(ctx, {{s.djob, &lj, sizeof(lj)}, {s.nodes_io, cyc.data(), cyc.size() * 4}}))) return rc;
     HIPCHK(launch_lookup(s.djob, s.nodes_io, (int)cyc.size(), s.eids_io, false, true, st));
-    HIPCHK(hipMemcpyAsync(eids.data(), s.eids_io, eids.size() * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    if ((rc = d2h_sync(ctx, eids.data(), s.eids_io, eids.size() * 4))) return rc;
   }
   std::vector<int32_t> outE;
   const int64_t loopmax = cycleLength + 1;
@@ -2892,12 +2914,11 @@ int pgc_solve_min_cost(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, in
   if (n > capacity) return fail(PGC_ERR_CAPACITY, "output capacity too small: need " + std::to_string(n));
   if (edge_ids) std::copy(outE.begin(), outE.end(), edge_ids);
   if (rows && n) {
-    HIPCHK(hipMemcpyAsync(s.eids_io, outE.data(), n * 4, hipMemcpyHostToDevice, st));
+    if ((rc = h2d_small(ctx, s.eids_io, outE.data(), (size_t)n * 4))) return rc;
     int32_t* d_out = nullptr;
     HIPCHK(dalloc(&d_out, n * g->ps_cols));
     HIPCHK(launch_gather_rows(g->d_rows, g->ps_cols, s.eids_io, n, d_out, st));
-    HIPCHK(hipMemcpyAsync(rows, d_out, n * g->ps_cols * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    if ((rc = d2h_sync(ctx, rows, d_out, (size_t)(n * g->ps_cols * 4)))) return rc;
     cache_free(d_out);
   }
   res->passes_layers_once = loops < 2 ? 1 : 0;
