@@ -261,7 +261,7 @@ struct pgc_context {
   int64_t eitems_cap = 0;
   // launch groups of a batch run concurrently on these streams (default priority)
   hipStream_t aux[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipEvent_t fork = nullptr;
+  hipEvent_t fork = nullptr, fork2 = nullptr;
   hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   uint8_t* d_select = nullptr;  // select states of the batched median
   int64_t select_cap = 0;
@@ -351,6 +351,7 @@ struct pgc_graph {
   int32_t *d_ex = nullptr, *d_ey = nullptr;
   int64_t nEX = 0, nEY = 0;
   int prune = 0;
+  int64_t last_sweeps = 0;  // Bellman-Ford sweeps of the graph's last ratio solve (batch ordering)
   int32_t xdeg[4] = {-1, -1, -1, -1};  // X in-degrees: largest intralayer, interlayer; their sums (one layer)
   bool conjugate = false;
   // conjugate graphs: edges per layer; every layer has the same in-edge
@@ -466,6 +467,7 @@ void pgc_context_destroy(pgc_context* c) {
     if (c->join[k]) (void)hipEventDestroy(c->join[k]);
   }
   if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->fork2) (void)hipEventDestroy(c->fork2);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1924,6 +1926,7 @@ int ratio_bnb_host(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32_
   res->objective = (float)(cs / ts);
   res->n_edges = n;
   res->sweeps = sweeps;
+  g->last_sweeps = sweeps;
   res->bf_calls = bfc;
   res->lawler_iters = its;
   res->branches = branches;
@@ -1995,6 +1998,7 @@ int ratio_bnb_device(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int3
   res->objective = (float)(r.out_cs / r.out_ts);
   res->n_edges = m;
   res->sweeps = r.sweeps;
+  g->last_sweeps = r.sweeps;
   res->bf_calls = r.bf_calls;
   res->lawler_iters = r.lawler_iters;
   res->branches = r.branches;
@@ -2332,66 +2336,37 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
       return fail(rc, "graph " + std::to_string(i) + ": " + g_last_error);
   }
   std::vector<SolverJob> jobs(n);
-  std::vector<QItem> qitems;
-  std::vector<FItem> fitems;
-  std::vector<FillItem> fills;
   for (int64_t i = 0; i < n; ++i) {
-    pgc_graph* g = graphs[i];
-    const int K = max_depth[i];
-    g->energy_fresh = false;  // quantized in place below
-    for (int c = 0; c < g->ncols; ++c) qitems.push_back({g->d_energy + (int64_t)c * g->E, g->E, 0, 0});
-    for (int c = 0; c < 2; ++c)
-      fitems.push_back({g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, (float*)g->d_cost[c],
-                        g->ncols > 1 ? g->d_time[c] : nullptr});
-    // a batch job is one launch: the kernel starts it with an empty cycle and,
-    // for a predecessor table in LDS, with -1 predecessors (no global table to
-    // reset, load or write back); global tables are reset by one fill launch
-    // for the whole batch
-    if (!pred_fits_lds(g, K)) fills.push_back({g->js.pred, g->js.predN});
-    jobs[i] = make_job(g, K, MODE_FULL_RATIO, flags);
+    graphs[i]->energy_fresh = false;  // quantized in place below
+    jobs[i] = make_job(graphs[i], max_depth[i], MODE_FULL_RATIO, flags);
     jobs[i].flags |= FLAG_FRESH_JOB;
   }
+  // Heavy items first: when every graph was solved before and a few of them
+  // ran far more sweeps (times their size) than the rest, those are prepared
+  // and launched first, and the other items are prepared while they run (the
+  // batch's makespan is its longest item; in repeated batches over the same
+  // graphs, a solve's sweep count is the best predictor of the next one's).
+  std::vector<char> heavy(n, 0);
+  int64_t nHeavy = 0;
   {
-    // chunks of at most 65535 columns (the launch grid's y limit)
-    const int64_t chunk = 65535;
-    if ((rc = grow(&ctx->d_qitems, ctx->qitems_cap, (int64_t)qitems.size()))) return rc;
-    if ((rc = grow(&ctx->d_fitems, ctx->fitems_cap, (int64_t)fitems.size()))) return rc;
-    const int64_t nsel = (int64_t)std::min<size_t>(qitems.size(), (size_t)chunk);
-    if ((rc = grow(&ctx->d_select, ctx->select_cap, nsel * (int64_t)bselect_state_bytes()))) return rc;
-    if ((rc = grow(&ctx->d_bracket, ctx->bracket_cap, nsel * (int64_t)bracket_state_bytes()))) return rc;
-    // the median brackets' candidate regions, one per column
-    int64_t candN = 0;
-    for (QItem& q : qitems) {
-      q.cand_off = candN;
-      q.cand_cap = median_cand_cap(q.n);
-      candN += q.cand_cap;
+    std::vector<double> est(n);
+    double mx = 0;
+    bool known = n >= 16;
+    for (int64_t i = 0; i < n && known; ++i) {
+      if (graphs[i]->last_sweeps <= 0) known = false;
+      est[i] = (double)graphs[i]->last_sweeps * (double)graphs[i]->E * max_depth[i];
+      mx = std::max(mx, est[i]);
     }
-    // exact size (no growth margin): it is 1/8 of the batch's energies
-    if (candN > ctx->cand_cap || !ctx->d_cand) {
-      if (ctx->d_cand) {
-        (void)hipDeviceSynchronize();  // queued work may still use the buffer
-        cache_free(ctx->d_cand);
-        ctx->d_cand = nullptr;
-        ctx->cand_cap = 0;
-      }
-      HIPCHK(dalloc(&ctx->d_cand, std::max<int64_t>(candN, 1)));
-      ctx->cand_cap = std::max<int64_t>(candN, 1);
+    if (known)
+      for (int64_t i = 0; i < n; ++i)
+        if (est[i] * 8 >= mx) {
+          heavy[i] = 1;
+          nHeavy++;
+        }
+    if (nHeavy * 16 > n) {
+      std::fill(heavy.begin(), heavy.end(), 0);
+      nHeavy = 0;
     }
-    HIPCHK(hipMemcpyAsync(ctx->d_qitems, qitems.data(), qitems.size() * sizeof(QItem), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->d_fitems, fitems.data(), fitems.size() * sizeof(FItem), hipMemcpyHostToDevice, st));
-    if (!fills.empty()) {
-      if ((rc = grow(&ctx->d_fills, ctx->fills_cap, (int64_t)fills.size()))) return rc;
-      HIPCHK(hipMemcpyAsync(ctx->d_fills, fills.data(), fills.size() * sizeof(FillItem), hipMemcpyHostToDevice, st));
-      for (int64_t a = 0; a < (int64_t)fills.size(); a += chunk)
-        HIPCHK(launch_fill_i32_batch(ctx->d_fills + a, (int)std::min<int64_t>(chunk, (int64_t)fills.size() - a), -1,
-                                     st));
-    }
-    for (int64_t a = 0; a < (int64_t)qitems.size(); a += chunk)
-      HIPCHK(launch_quantize_batch(ctx->d_qitems + a, (int)std::min<int64_t>(chunk, (int64_t)qitems.size() - a),
-                                   ctx->d_select, ctx->d_bracket, ctx->d_cand, st));
-    for (int64_t a = 0; a < (int64_t)fitems.size(); a += chunk)
-      HIPCHK(launch_fill_costs_batch(ctx->d_fitems + a, (int)std::min<int64_t>(chunk, (int64_t)fitems.size() - a),
-                                     st));
   }
   // batch tables: results and output cycles in one arena each
   if ((rc = grow(&ctx->d_jobs, ctx->jobs_cap, n))) return rc;
@@ -2403,6 +2378,40 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     jobs[i].out_eid = ctx->d_out + (offsets[i] - offsets[0]);
     jobs[i].out_cap = (int32_t)std::min<int64_t>(offsets[i + 1] - offsets[i], INT32_MAX);
   }
+  // preparation scratch, sized for the whole batch (the phases use it in turn
+  // on the context's stream)
+  const int64_t chunk = 65535;  // columns per launch (the launch grid's y limit)
+  {
+    int64_t nq = 0, candN = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      nq += graphs[i]->ncols;
+      for (int c = 0; c < graphs[i]->ncols; ++c) candN += median_cand_cap(graphs[i]->E);
+    }
+    if ((rc = grow(&ctx->d_qitems, ctx->qitems_cap, nq))) return rc;
+    if ((rc = grow(&ctx->d_fitems, ctx->fitems_cap, 2 * n))) return rc;
+    if ((rc = grow(&ctx->d_fills, ctx->fills_cap, n))) return rc;
+    const int64_t nsel = std::min<int64_t>(nq, chunk);
+    if ((rc = grow(&ctx->d_select, ctx->select_cap, nsel * (int64_t)bselect_state_bytes()))) return rc;
+    if ((rc = grow(&ctx->d_bracket, ctx->bracket_cap, nsel * (int64_t)bracket_state_bytes()))) return rc;
+    // exact size (no growth margin): it is 1/8 of the batch's energies
+    if (candN > ctx->cand_cap || !ctx->d_cand) {
+      if (ctx->d_cand) {
+        (void)hipDeviceSynchronize();  // queued work may still use the buffer
+        cache_free(ctx->d_cand);
+        ctx->d_cand = nullptr;
+        ctx->cand_cap = 0;
+      }
+      HIPCHK(dalloc(&ctx->d_cand, std::max<int64_t>(candN, 1)));
+      ctx->cand_cap = std::max<int64_t>(candN, 1);
+    }
+  }
+  for (int k = 0; k < 4; ++k) {
+    if (!ctx->aux[k]) HIPCHK(hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
+    if (!ctx->join[k]) HIPCHK(hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming));
+  }
+  for (hipEvent_t* e : {&ctx->fork, &ctx->fork2})
+    if (!*e) HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  HIPCHK(hipMemsetAsync(ctx->d_queue, 0, 64 * sizeof(int32_t), st));
   // launch groups: jobs sharing one kernel instantiation run in one
   // persistent launch, ordered largest first inside the group
   struct Key {
@@ -2412,62 +2421,109 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
       return std::tie(maxc, maxw, time) < std::tie(o.maxc, o.maxw, o.time);
     }
   };
-  std::map<Key, std::vector<int64_t>> groups;
-  for (int64_t i = 0; i < n; ++i) {
-    pgc_graph* g = graphs[i];
-    const int mc = kernel_maxc(g, max_depth[i]);
-    const int mw = g->maxw <= 8 ? 8 : (g->maxw <= 16 ? 16 : (g->maxw <= 32 ? 32 : kMaxSlotWidth));
-    groups[{mc, mw, g->d_time[0] != nullptr}].push_back(i);
-  }
-  if (groups.size() > 64) return fail(PGC_ERR_UNSUPPORTED, "too many launch groups");
-  std::vector<SolverJob> ordered;
-  ordered.reserve(n);
-  struct Launch {
-    Key k;
-    int64_t first, count;
-    size_t lds;
-  };
-  std::vector<Launch> launches;
-  for (auto& kv : groups) {
-    auto& idx = kv.second;
-    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
-      return graphs[a]->E * max_depth[a] > graphs[b]->E * max_depth[b];
-    });
-    size_t lds = 0;
-    for (int64_t i : idx) lds = std::max(lds, job_lds_bytes(graphs[i], max_depth[i]));
-    launches.push_back({kv.first, (int64_t)ordered.size(), (int64_t)idx.size(), lds});
-    for (int64_t i : idx) ordered.push_back(jobs[i]);
-  }
-  HIPCHK(hipMemcpyAsync(ctx->d_jobs, ordered.data(), n * sizeof(SolverJob), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(ctx->d_queue, 0, 64 * sizeof(int32_t), st));
-  HIPCHK(hipEventRecord(ctx->ev0, st));
-  if (launches.size() == 1) {
-    const Launch& L = launches[0];
-    const int grid = (int)std::min<int64_t>(L.count, std::max(1, ctx->num_cu));
-    HIPCHK(launch_solver(ctx->d_jobs + L.first, (int)L.count, grid, ctx->d_queue, L.k.maxc, L.k.maxw, false,
-                         L.k.time, false, L.lds, st));
-  } else {
+  int nLaunch = 0;
+  bool usedAux[4] = {false, false, false, false};
+  bool timing = false;
+  int64_t jobBase = 0;
+  // one phase: the quantization and slot costs of its items, then its
+  // launch groups on the auxiliary streams (the heavy phase on stream 3, the
+  // other on streams 0-2, so that no launch queues behind a heavy one)
+  auto phase = [&](const std::vector<int64_t>& items, int firstAux, int nAux) -> int {
+    if (items.empty()) return PGC_OK;
+    std::vector<QItem> qitems;
+    std::vector<FItem> fitems;
+    std::vector<FillItem> fills;
+    int64_t candN = 0;
+    for (int64_t i : items) {
+      pgc_graph* g = graphs[i];
+      for (int c = 0; c < g->ncols; ++c) {
+        const int64_t cap = median_cand_cap(g->E);
+        qitems.push_back({g->d_energy + (int64_t)c * g->E, g->E, candN, cap});
+        candN += cap;
+      }
+      for (int c = 0; c < 2; ++c)
+        fitems.push_back({g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, (float*)g->d_cost[c],
+                          g->ncols > 1 ? g->d_time[c] : nullptr});
+      // a batch job is one launch: the kernel starts it with an empty cycle
+      // and, for a predecessor table in LDS, with -1 predecessors (no global
+      // table to reset, load or write back); global tables are reset by one
+      // fill launch per phase
+      if (!pred_fits_lds(g, max_depth[i])) fills.push_back({g->js.pred, g->js.predN});
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_qitems, qitems.data(), qitems.size() * sizeof(QItem), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->d_fitems, fitems.data(), fitems.size() * sizeof(FItem), hipMemcpyHostToDevice, st));
+    if (!fills.empty()) {
+      HIPCHK(hipMemcpyAsync(ctx->d_fills, fills.data(), fills.size() * sizeof(FillItem), hipMemcpyHostToDevice, st));
+      for (int64_t a = 0; a < (int64_t)fills.size(); a += chunk)
+        HIPCHK(launch_fill_i32_batch(ctx->d_fills + a, (int)std::min<int64_t>(chunk, (int64_t)fills.size() - a), -1,
+                                     st));
+    }
+    for (int64_t a = 0; a < (int64_t)qitems.size(); a += chunk)
+      HIPCHK(launch_quantize_batch(ctx->d_qitems + a, (int)std::min<int64_t>(chunk, (int64_t)qitems.size() - a),
+                                   ctx->d_select, ctx->d_bracket, ctx->d_cand, st));
+    for (int64_t a = 0; a < (int64_t)fitems.size(); a += chunk)
+      HIPCHK(launch_fill_costs_batch(ctx->d_fitems + a, (int)std::min<int64_t>(chunk, (int64_t)fitems.size() - a),
+                                     st));
+    std::map<Key, std::vector<int64_t>> groups;
+    for (int64_t i : items) {
+      pgc_graph* g = graphs[i];
+      const int mc = kernel_maxc(g, max_depth[i]);
+      const int mw = g->maxw <= 8 ? 8 : (g->maxw <= 16 ? 16 : (g->maxw <= 32 ? 32 : kMaxSlotWidth));
+      groups[{mc, mw, g->d_time[0] != nullptr}].push_back(i);
+    }
+    if (nLaunch + (int)groups.size() > 64) return fail(PGC_ERR_UNSUPPORTED, "too many launch groups");
+    std::vector<SolverJob> ordered;
+    ordered.reserve(items.size());
+    struct Launch {
+      Key k;
+      int64_t first, count;
+      size_t lds;
+    };
+    std::vector<Launch> launches;
+    for (auto& kv : groups) {
+      auto& idx = kv.second;
+      std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
+        return graphs[x]->E * max_depth[x] > graphs[y]->E * max_depth[y];
+      });
+      size_t lds = 0;
+      for (int64_t i : idx) lds = std::max(lds, job_lds_bytes(graphs[i], max_depth[i]));
+      launches.push_back({kv.first, jobBase + (int64_t)ordered.size(), (int64_t)idx.size(), lds});
+      for (int64_t i : idx) ordered.push_back(jobs[i]);
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_jobs + jobBase, ordered.data(), ordered.size() * sizeof(SolverJob),
+                          hipMemcpyHostToDevice, st));
+    jobBase += (int64_t)ordered.size();
+    hipEvent_t fork = firstAux == 3 ? ctx->fork : ctx->fork2;
+    if (!timing) {
+      HIPCHK(hipEventRecord(ctx->ev0, st));
+      timing = true;
+    }
+    HIPCHK(hipEventRecord(fork, st));
+    for (int k = firstAux; k < firstAux + nAux; ++k) HIPCHK(hipStreamWaitEvent(ctx->aux[k], fork, 0));
     // the groups run concurrently: a group's persistent workgroups hold CUs
     // only while it has jobs, and the CUs a group leaves (while its longest
     // jobs finish) take the other groups' workgroups
-    for (int k = 0; k < 4; ++k) {
-      if (!ctx->aux[k]) HIPCHK(hipStreamCreateWithFlags(&ctx->aux[k], hipStreamNonBlocking));
-      if (!ctx->join[k]) HIPCHK(hipEventCreateWithFlags(&ctx->join[k], hipEventDisableTiming));
-    }
-    if (!ctx->fork) HIPCHK(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
-    HIPCHK(hipEventRecord(ctx->fork, st));
-    const int ns = (int)std::min<size_t>(4, launches.size());
-    for (int k = 0; k < ns; ++k) HIPCHK(hipStreamWaitEvent(ctx->aux[k], ctx->fork, 0));
     for (size_t li = 0; li < launches.size(); ++li) {
       const Launch& L = launches[li];
+      const int k = firstAux + (int)(li % nAux);
+      usedAux[k] = true;
       const int grid = (int)std::min<int64_t>(L.count, std::max(1, ctx->num_cu));
-      HIPCHK(launch_solver(ctx->d_jobs + L.first, (int)L.count, grid, ctx->d_queue + li, L.k.maxc, L.k.maxw, false,
-                           L.k.time, false, L.lds, ctx->aux[li % 4]));
+      HIPCHK(launch_solver(ctx->d_jobs + L.first, (int)L.count, grid, ctx->d_queue + nLaunch, L.k.maxc, L.k.maxw,
+                           false, L.k.time, false, L.lds, ctx->aux[k]));
+      nLaunch++;
     }
-    for (int k = 0; k < ns; ++k) {
-      HIPCHK(hipEventRecord(ctx->join[k], ctx->aux[k]));
-      HIPCHK(hipStreamWaitEvent(st, ctx->join[k], 0));
-    }
+    return PGC_OK;
+  };
+  {
+    std::vector<int64_t> first, rest;
+    for (int64_t i = 0; i < n; ++i) (heavy[i] ? first : rest).push_back(i);
+    if ((rc = phase(first, 3, 1))) return rc;
+    if ((rc = phase(rest, 0, nHeavy > 0 ? 3 : 4))) return rc;
+  }
+  for (int k = 0; k < 4; ++k) {
+    if (!usedAux[k]) continue;
+    HIPCHK(hipEventRecord(ctx->join[k], ctx->aux[k]));
+    HIPCHK(hipStreamWaitEvent(st, ctx->join[k], 0));
   }
   HIPCHK(hipEventRecord(ctx->ev1, st));
   std::vector<JobResult> out(n);
@@ -2560,6 +2616,7 @@ int pgc_solve_ratio_batch(pgc_context* ctx, pgc_graph* const* graphs, const int3
     o.objective = (float)(r.out_cs / r.out_ts);
     o.n_edges = r.out_len;
     o.sweeps = r.sweeps;
+    graphs[i]->last_sweeps = r.sweeps;
     o.bf_calls = r.bf_calls;
     o.lawler_iters = r.lawler_iters;
     o.branches = r.branches;

@@ -131,6 +131,37 @@ def test_sweep_cap_case_golden():
     gb.close()
 
 
+def test_batch_heavy_item_first_phase():
+    # A batch of graphs that were all solved before, one of which ran far more
+    # sweeps (the sweep-cap pair): that item is prepared and launched first and
+    # the others are prepared while it runs. Results equal those of single
+    # solves on copies of the graphs (both paths quantize the quantized costs
+    # again on the second solve, as the reference's in-place quantization does).
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "bf_cap_case.npz"))
+    pc, Kc = S.make_uneven_pairs(256, seed=0)[int(z["pair_index"])]
+    pc = dict(pc, feat_diff=z["feat_diff"])
+    items = [(pc, Kc, {})] + [c for c in _cases() if c[2] == {}][:17]
+    assert len(items) >= 16
+    ctx = _lib.default_context(0)
+    gb = [_graph(ctx, p, **kw) for p, K, kw in items]
+    gs = [_graph(ctx, p, **kw) for p, K, kw in items]
+    Ks = [K for _, K, _ in items]
+    for rnd in range(2):
+        print("heavy-first batch round", rnd, flush=True)
+        outs = _lib.solve_ratio_batch(ctx, gb, Ks)
+        for (eb, rb, resb), g, K in zip(outs, gs, Ks):
+            es, rs, ress = g.solve_ratio(K)
+            assert np.array_equal(eb, es) and np.array_equal(rb, rs)
+            assert resb.objective == ress.objective
+            assert (resb.sweeps, resb.bf_calls, resb.lawler_iters, resb.branches) == \
+                (ress.sweeps, ress.bf_calls, ress.lawler_iters, ress.branches)
+        if rnd == 0:
+            assert outs[0][2].sweeps > 8 * max(o[2].sweeps for o in outs[1:])
+    for g in gb + gs:
+        g.close()
+
+
 def test_batch_median_bracket_miss_and_overflow():
     # the batched median samples 8192 strided keys per column and brackets the
     # median's expected position; these columns defeat the sample (every
