@@ -229,11 +229,14 @@ struct Select1State {
   uint64_t mask[2][2];
   int64_t k[2][2];        // rank inside the keys that match the prefix
   uint32_t hist[8][2][256];  // [pass][rank][bin]
+  uint32_t rcnt[512];  // keys workgroup b gathered into its region of the gather buffer
+  uint32_t over;       // some region overflowed: the later passes read every key
 };
 
 __global__ void k_select_init(Select1State* st, int64_t k0, int64_t k1) {
   for (int i = threadIdx.x; i < 8 * 2 * 256; i += blockDim.x) (&st->hist[0][0][0])[i] = 0;
   if (threadIdx.x == 0) {
+    st->over = 0;
     st->prefix[0][0] = st->prefix[0][1] = 0;
     st->mask[0][0] = st->mask[0][1] = 0;
     st->k[0][0] = k0;
@@ -286,9 +289,16 @@ __device__ void pick_digit(const uint32_t (*h)[256], int shift, uint64_t (&pre)[
 // of a wave has the same digit, the wave adds its two match counts with one
 // LDS atomic each instead of one per lane (the same totals). A thread loads
 // kSelUnroll keys before it counts any, so their latencies overlap.
+// With a gather buffer (cand, ccap keys), pass kSelGather also gathers the
+// keys that match its prefix (either rank: the only keys any later pass
+// counts), each workgroup into its own region of ccap / gridDim keys (an LDS
+// counter, no global atomics), and in the later passes each workgroup reads
+// its region instead of its share of all n keys, unless a region overflowed.
+// The grid is the same for every pass.
 constexpr int kSelUnroll = 8;
+constexpr int kSelGather = 2;
 __global__ void __launch_bounds__(256) k_select_hist(const double* __restrict__ x, int64_t n, Select1State* st,
-                                                     int pass) {
+                                                     int pass, double* __restrict__ cand, int64_t ccap) {
   __shared__ uint32_t h[2][256];
   h[0][threadIdx.x] = 0;
   h[1][threadIdx.x] = 0;
@@ -316,8 +326,20 @@ __global__ void __launch_bounds__(256) k_select_hist(const double* __restrict__ 
   // both ranks on one prefix (every pass for an odd count, the early ones
   // for an even one): count the first histogram and copy it
   const bool same = pre[0] == pre[1] && msk[0] == msk[1];
-  const int64_t step = (int64_t)gridDim.x * blockDim.x * kSelUnroll;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x * kSelUnroll + threadIdx.x; base < n; base += step) {
+  const bool gather = cand && pass == kSelGather;
+  const int64_t rs = cand ? ccap / gridDim.x : 0;  // region of this workgroup
+  __shared__ uint32_t lc;
+  int64_t start = (int64_t)blockIdx.x * blockDim.x * kSelUnroll;
+  int64_t step = (int64_t)gridDim.x * blockDim.x * kSelUnroll;
+  if (cand && pass > kSelGather && __hip_atomic_load(&st->over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    x = cand + (int64_t)blockIdx.x * rs;
+    n = st->rcnt[blockIdx.x];
+    start = 0;
+    step = (int64_t)blockDim.x * kSelUnroll;
+  }
+  if (threadIdx.x == 0) lc = 0;
+  __syncthreads();
+  for (int64_t base = start + threadIdx.x; base < n; base += step) {
     double v[kSelUnroll];
 #pragma unroll
     for (int u = 0; u < kSelUnroll; ++u) {
@@ -342,9 +364,25 @@ __global__ void __launch_bounds__(256) k_select_hist(const double* __restrict__ 
         if (in0) atomicAdd(&h[0][b], 1u);
         if (in1) atomicAdd(&h[1][b], 1u);
       }
+      if (gather) {
+        const bool keep = in0 || ((kk & msk[1]) == pre[1]);
+        const uint64_t kb = __ballot(keep);
+        if (kb) {
+          const int first = __ffsll((unsigned long long)act) - 1;
+          uint32_t o = 0;
+          if (lane == first) o = atomicAdd(&lc, (uint32_t)__popcll(kb));
+          o = (uint32_t)__shfl((int)o, first);
+          const int64_t at = (int64_t)o + __popcll(kb & ((1ull << lane) - 1ull));
+          if (keep && at < rs) cand[(int64_t)blockIdx.x * rs + at] = v[u];
+        }
+      }
     }
   }
   __syncthreads();
+  if (gather && threadIdx.x == 0) {
+    st->rcnt[blockIdx.x] = (uint32_t)min<int64_t>(lc, rs);
+    if (lc > rs) atomicOr(&st->over, 1u);
+  }
   if (same) h[1][threadIdx.x] = h[0][threadIdx.x];
   if (h[0][threadIdx.x]) atomicAdd(&st->hist[pass][0][threadIdx.x], h[0][threadIdx.x]);
   if (h[1][threadIdx.x]) atomicAdd(&st->hist[pass][1][threadIdx.x], h[1][threadIdx.x]);
@@ -929,7 +967,10 @@ hipError_t launch_transpose_fd(const double* FD, int64_t nVX, int64_t nVY, doubl
   return hipGetLastError();
 }
 
-hipError_t launch_quantize(double* col, int64_t n, void* selectState, double* scale_out, hipStream_t st) {
+// cand: a buffer of ccap keys for the keys the later radix passes read, or
+// null (every pass reads all n keys)
+hipError_t launch_quantize(double* col, int64_t n, void* selectState, double* scale_out, double* cand, int64_t ccap,
+                           hipStream_t st) {
   Select1State* S = (Select1State*)selectState;
   const int64_t k1 = (n % 2 == 0) ? (n - 2) / 2 : n / 2;
   const int64_t k2 = (n % 2 == 0) ? k1 + 1 : k1;
@@ -937,7 +978,8 @@ hipError_t launch_quantize(double* col, int64_t n, void* selectState, double* sc
   // at most 512 workgroups: each adds its partial histograms with global
   // atomics on the same 512 counters
   const unsigned gh = (unsigned)std::min<int64_t>(512, std::max<int64_t>(1, (n + 256 * kSelUnroll - 1) / (256 * kSelUnroll)));
-  for (int pass = 0; pass < 8; ++pass) hipLaunchKernelGGL(k_select_hist, dim3(gh), dim3(256), 0, st, col, n, S, pass);
+  for (int pass = 0; pass < 8; ++pass)
+    hipLaunchKernelGGL(k_select_hist, dim3(gh), dim3(256), 0, st, col, n, S, pass, cand, cand ? ccap : 0);
   const unsigned g = (unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (n + 255) / 256));
   hipLaunchKernelGGL(k_quantize, dim3(g), dim3(256), 0, st, col, n, S, (int)(n % 2 == 0), 100000.0, scale_out);
   return hipGetLastError();

@@ -36,7 +36,7 @@ hipError_t launch_product_space(const int32_t*, int64_t, const int32_t*, int64_t
                                 int32_t*, int64_t, hipStream_t);
 hipError_t launch_energy(const double*, const double*, const int32_t*, int64_t, const double*, int64_t, int, int,
                          double*, hipStream_t);
-hipError_t launch_quantize(double*, int64_t, void*, double*, hipStream_t);
+hipError_t launch_quantize(double*, int64_t, void*, double*, double*, int64_t, hipStream_t);
 hipError_t launch_transpose_fd(const double*, int64_t, int64_t, double*, hipStream_t);
 hipError_t launch_quantize_batch(const QItem*, int, void*, void*, double*, hipStream_t);
 hipError_t launch_fill_costs_batch(const FItem*, int, hipStream_t);
@@ -269,6 +269,9 @@ struct pgc_context {
   int64_t bracket_cap = 0;
   double* d_cand = nullptr;  // and the keys gathered inside the brackets
   int64_t cand_cap = 0;
+  // the single-column median's gathered keys (grow-only)
+  double* d_qcand = nullptr;
+  int64_t qcand_cap = 0;
   // pinned host staging of small copies (grow-only, see stage())
   uint8_t* pinned = nullptr;
   size_t pinned_cap = 0;
@@ -456,7 +459,8 @@ void pgc_context_destroy(pgc_context* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   void* ps[] = {c->d_jobs, c->d_res, c->d_out, c->d_rows, c->d_queue, c->d_items, c->d_ceids, c->d_values,
-                c->d_qitems, c->d_fitems, c->d_eitems, c->d_select, c->d_bracket, c->d_cand, c->d_fills};
+                c->d_qitems, c->d_fitems, c->d_eitems, c->d_select, c->d_bracket, c->d_cand, c->d_fills,
+                c->d_qcand};
   for (void* p : ps)
     if (p) cache_free(p);
   for (int k = 0; k < 4; ++k) {
@@ -2046,10 +2050,31 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   } implicit_scope{g};
   g->implicit_now = g->energy_fresh && g->d_fdt;
   g->energy_fresh = false;
-  // robustFloatToIntConversion of every column, in place (:875-882)
+  // robustFloatToIntConversion of every column, in place (:875-882). For
+  // large columns the median's later radix passes read the keys its third
+  // pass gathers (those sharing the median's top 16 bits, when they fit an
+  // eighth of the column); below kGatherMin keys a pass costs mostly its
+  // fixed overhead and every pass reads the column.
+  constexpr int64_t kGatherMin = (int64_t)8 << 20;
+  double* qc = nullptr;
+  int64_t ccap = 0;
+  if (g->E >= kGatherMin) {
+    ccap = g->E / 8 + 1024;
+    if (ctx->qcand_cap < ccap) {
+      if (ctx->d_qcand) {
+        HIPCHK(hipStreamSynchronize(st));
+        cache_free(ctx->d_qcand);
+      }
+      ctx->d_qcand = nullptr;
+      ctx->qcand_cap = 0;
+      HIPCHK(dalloc(&ctx->d_qcand, ccap));
+      ctx->qcand_cap = ccap;
+    }
+    qc = ctx->d_qcand;
+  }
   for (int c = 0; c < g->ncols; ++c)
     HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, ctx->select_state,
-                           g->implicit_now ? g->d_qscale : nullptr, st));
+                           g->implicit_now ? g->d_qscale : nullptr, qc, ccap, st));
   if ((rc = fill_costs(g, 1))) return rc;
   // the device branch and bound wherever the one-workgroup solver runs the
   // job (the device-wide solver has no device branch and bound);
@@ -2193,7 +2218,8 @@ int pgc_solve_ratio_noalloc(pgc_context* ctx, pgc_graph* g, int K, uint32_t flag
   // robustFloatToIntConversion of every column, in place (:875-882), then
   // the slot costs
   g->energy_fresh = false;
-  for (int c = 0; c < g->ncols; ++c) HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, sel, nullptr, st));
+  for (int c = 0; c < g->ncols; ++c)
+    HIPCHK(launch_quantize(g->d_energy + (int64_t)c * g->E, g->E, sel, nullptr, nullptr, 0, st));
   for (int c = 0; c < 2; ++c)
     HIPCHK(launch_fill_costs(g->d_e[c], g->d_s[c], g->nslots[c], g->d_energy, g->E, g->ncols, false, g->d_cost[c],
                              g->ncols > 1 ? g->d_time[c] : nullptr, 0, 0, st));

@@ -6,7 +6,9 @@ objective equal (1e-5 relative written), sweep and branch counters equal.
 Cases: graphs that need the variant (non-conjugate max_depth 4 on an
 icosphere s=4, conjugate graphs of 2880 and 11520 nodes per layer), a batch
 mixing both variants, and the small parity cases in a subprocess that forces
-the variant for every graph (PGC_GLOBAL_TABLE=1).
+the variant for every graph (PGC_GLOBAL_TABLE=1). Also the single-solve median
+of columns of 8 M keys and more (gathered radix passes), whose quantized costs
+must equal the oracle's bit for bit.
 """
 import os
 import subprocess
@@ -91,3 +93,32 @@ def test_small_cases_with_global_table_forced():
     # output goes straight to this process's stdout (progress stays visible)
     r = subprocess.run(cmd, env=env, cwd=os.path.dirname(HERE), timeout=900)
     assert r.returncode == 0
+
+
+@pytest.mark.parametrize("kind", ["normal", "overflow", "odd"])
+def test_large_column_median_gather(kind):
+    # single solves of columns of 8 M keys and more: the median's third radix
+    # pass gathers the keys that share the median's top 16 bits and the later
+    # passes read them (each workgroup its own region), unless a region
+    # overflows (all keys equal: every pass reads the column); the quantized
+    # costs must equal the oracle's quantization (numpy-pinned median and
+    # rounding)
+    p = S.make_pair(4, 256, seed=11)
+    rows, src, trgt = O.product_space(p["EX"], p["EY"])
+    npl = len(p["VX"])
+    E = len(src)
+    assert E >= 8 << 20
+    rng = np.random.default_rng(4)
+    if kind == "overflow":
+        col = np.full(E, 3.0)
+        col[rng.integers(0, E, 1000)] = 7.0
+    else:
+        col = rng.uniform(0.1, 20.0, E)
+    if kind == "odd":
+        rows, src, trgt, col = rows[:-1], src[:-1], trgt[:-1], col[:-1]
+    print("case large column", kind, len(col), flush=True)
+    ctx = _lib.default_context(0)
+    g = _lib.Graph.from_arrays(ctx, rows, src, trgt, col, npl)
+    g.solve_ratio(1)
+    assert np.array_equal(g.cost()[:, 0], O.quantize(col[:, None])[:, 0])
+    g.close()
