@@ -2050,16 +2050,19 @@ int pgc_solve_ratio(pgc_context* ctx, pgc_graph* g, int K, uint32_t flags, int32
   } implicit_scope{g};
   g->implicit_now = g->energy_fresh && g->d_fdt;
   g->energy_fresh = false;
-  // robustFloatToIntConversion of every column, in place (:875-882). For
-  // large columns the median's later radix passes read the keys its third
-  // pass gathers (those sharing the median's top 16 bits, when they fit an
-  // eighth of the column); below kGatherMin keys a pass costs mostly its
-  // fixed overhead and every pass reads the column.
-  constexpr int64_t kGatherMin = (int64_t)8 << 20;
+  // robustFloatToIntConversion of every column, in place (:875-882). From
+  // kGatherMin keys on, the median's later radix passes read the keys its
+  // third pass gathers (those sharing the median's top 16 bits), each
+  // workgroup from its own region: sized for every key of the column below
+  // 8 M keys (no region can overflow), an eighth of it above (a region that
+  // overflows sends the later passes back to the column). Below kGatherMin
+  // a pass costs mostly its fixed overhead and every pass reads the column.
+  constexpr int64_t kGatherMin = (int64_t)1 << 20;
   double* qc = nullptr;
   int64_t ccap = 0;
   if (g->E >= kGatherMin) {
-    ccap = g->E / 8 + 1024;
+    // 512 workgroups, each with at most one tile of 2048 keys beyond E / 512
+    ccap = g->E >= ((int64_t)8 << 20) ? g->E / 8 + 1024 : g->E + 512 * 2048;
     if (ctx->qcand_cap < ccap) {
       if (ctx->d_qcand) {
         HIPCHK(hipStreamSynchronize(st));

@@ -7,8 +7,9 @@ Cases: graphs that need the variant (non-conjugate max_depth 4 on an
 icosphere s=4, conjugate graphs of 2880 and 11520 nodes per layer), a batch
 mixing both variants, and the small parity cases in a subprocess that forces
 the variant for every graph (PGC_GLOBAL_TABLE=1). Also the single-solve median
-of columns of 8 M keys and more (gathered radix passes), whose quantized costs
-must equal the oracle's bit for bit.
+of columns of 1 M keys and more (gathered radix passes; regions of the whole
+column below 8 M keys, of an eighth above), whose quantized costs must equal
+the oracle's bit for bit.
 """
 import os
 import subprocess
@@ -96,18 +97,20 @@ def test_small_cases_with_global_table_forced():
 
 
 @pytest.mark.parametrize("kind", ["normal", "overflow", "odd"])
-def test_large_column_median_gather(kind):
-    # single solves of columns of 8 M keys and more: the median's third radix
+@pytest.mark.parametrize("pair", [(4, 256), (3, 256)], ids=["8M", "2M"])
+def test_large_column_median_gather(kind, pair):
+    # single solves of columns of 1 M keys and more: the median's third radix
     # pass gathers the keys that share the median's top 16 bits and the later
-    # passes read them (each workgroup its own region), unless a region
-    # overflows (all keys equal: every pass reads the column); the quantized
-    # costs must equal the oracle's quantization (numpy-pinned median and
-    # rounding)
-    p = S.make_pair(4, 256, seed=11)
+    # passes read them (each workgroup its own region). Above 8 M keys a
+    # region holds an eighth of the column and can overflow (almost all keys
+    # equal: every later pass then reads the column); below, it holds the
+    # whole column. The quantized costs must equal the oracle's quantization
+    # (numpy-pinned median and rounding)
+    p = S.make_pair(*pair, seed=11)
     rows, src, trgt = O.product_space(p["EX"], p["EY"])
     npl = len(p["VX"])
     E = len(src)
-    assert E >= 8 << 20
+    assert E >= (8 << 20 if pair[0] == 4 else 1 << 20)
     rng = np.random.default_rng(4)
     if kind == "overflow":
         col = np.full(E, 3.0)
