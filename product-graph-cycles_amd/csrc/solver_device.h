@@ -470,6 +470,28 @@ struct Solver {
     return (int)(rem - depth * (uint32_t)npl + layer * (uint32_t)npl);
   }
 
+  // While the leader walks the upper bound's path back from the last layer,
+  // waves 1 to kWarmWaves read the predecessor table (global memory) from its
+  // end backwards, one load per 128-byte line, so that the leader's dependent
+  // loads find their lines in the L2 (after a sweep most of the table has
+  // left it). The values are discarded. (The cycle extraction's walks follow
+  // chains the chase has just read: warming there only adds contention.)
+  static constexpr int kWarmWaves = 4;
+  __device__ __forceinline__ void warm_pred_reverse() {
+    const PGC_GLOBAL int32_t* P = gp((const int32_t*)J.pred);
+    const int64_t n = (int64_t)(L + 1) * K * npl;
+    const int64_t lines = (n + 31) / 32;
+    const int t = (int)threadIdx.x - 64;
+    constexpr int nt = kWarmWaves * 64;
+    int acc = 0;
+    for (int64_t i = lines - 1 - t; i >= 0; i -= 2 * nt) {
+      const int a = P[i * 32];
+      const int b = i - nt >= 0 ? P[(i - nt) * 32] : 0;
+      acc ^= a ^ b;
+    }
+    if (acc == 0x7ffffffe && npl < 0) sh.i3 = acc;  // never taken: keeps the loads
+  }
+
   // predecessor table (expanded node ids) in LDS or global memory
   __device__ __forceinline__ int pget(int64_t i) const { return predLds ? Pl[i] : gp(J.pred)[i]; }
   __device__ __forceinline__ void pset(int64_t i, int v) const {
@@ -937,6 +959,8 @@ struct Solver {
             if (p == start) break;
           }
           sh.i0 = cnt;
+        } else if (!GRID && !predLds && (threadIdx.x >> 6) >= 1 && (threadIdx.x >> 6) <= kWarmWaves) {
+          warm_pred_reverse();
         }
         sync();
         const int cnt = sh.i0;
